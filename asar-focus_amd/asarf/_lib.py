@@ -1,0 +1,146 @@
+"""ctypes binding of libasarf.so (the C ABI declared in include/asarf.h).
+
+The product path is the HIP library: if it is missing this module raises instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libasarf.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG), "include", "asarf.h")
+
+OK, EINVAL, ESHAPE, EWORKSPACE, EHIP, ENOMEM, EUNSUPPORTED = range(7)
+Q1_IQ_PADDED_WIDTH, Q2_COS_NO_SQRT, Q4_DC_EXTRA_BLOCK = 1, 2, 4
+QUIRKS_REFERENCE = 7
+CF32, F32 = 0, 1
+RAW_CF32, RAW_U8, RAW_I8 = 0, 1, 2
+FORWARD, INVERSE = 0, 1
+FFT_ROWS, FFT_COLUMNS = 0, 1
+
+
+class AsarfError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"asarf status {status}: {msg}")
+        self.status = status
+
+
+class Image(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("x_size", ctypes.c_int), ("y_size", ctypes.c_int),
+                ("x_stride", ctypes.c_int), ("y_stride", ctypes.c_int), ("dtype", ctypes.c_int)]
+
+
+class Workspace(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
+
+
+class ChirpInfo(ctypes.Structure):
+    _fields_ = [("range_sampling_rate", ctypes.c_double), ("pulse_duration", ctypes.c_double),
+                ("n_samples", ctypes.c_int), ("pulse_bandwidth", ctypes.c_double), ("kr", ctypes.c_double),
+                ("apply_window", ctypes.c_int)]
+
+
+class IqResults(ctypes.Structure):
+    _fields_ = [("dc_i", ctypes.c_double), ("dc_q", ctypes.c_double), ("iq_gain", ctypes.c_double),
+                ("phase_error", ctypes.c_double), ("stddev_i", ctypes.c_double), ("stddev_q", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RdaParams(ctypes.Structure):
+    _fields_ = [("wavelength", ctypes.c_double), ("range_spacing", ctypes.c_double),
+                ("slant_range_first", ctypes.c_double), ("prf", ctypes.c_double),
+                ("az_bandwidth_fraction", ctypes.c_double), ("dc_poly", ctypes.c_double * 3),
+                ("vr_poly", ctypes.c_double * 3)]
+
+
+class FocusParams(ctypes.Structure):
+    _fields_ = [("na", ctypes.c_int), ("nr", ctypes.c_int), ("nc", ctypes.c_int), ("nrp", ctypes.c_int),
+                ("nap", ctypes.c_int), ("raw_format", ctypes.c_int), ("prf", ctypes.c_double),
+                ("wavelength", ctypes.c_double), ("range_spacing", ctypes.c_double),
+                ("slant_range_first", ctypes.c_double), ("az_bandwidth_fraction", ctypes.c_double),
+                ("vr_poly", ctypes.c_double * 3), ("n_total_samples", ctypes.c_size_t), ("quirks", ctypes.c_int),
+                ("imp", ctypes.c_int), ("azimuth_spacing", ctypes.c_double), ("reference_range", ctypes.c_double),
+                ("grsr_poly", ctypes.c_double * 5), ("gr_range_size", ctypes.c_int),
+                ("ground_spacing", ctypes.c_double), ("lines_before", ctypes.c_int), ("lines_after", ctypes.c_int)]
+
+
+class FocusResults(ctypes.Structure):
+    _fields_ = [("iq", IqResults), ("dc_poly", ctypes.c_double * 3), ("n_blocks", ctypes.c_int),
+                ("block_freq", ctypes.c_double * 64), ("block_x", ctypes.c_double * 64), ("nrv", ctypes.c_int),
+                ("crop_front", ctypes.c_int), ("crop_back", ctypes.c_int), ("crop_near", ctypes.c_int),
+                ("crop_far", ctypes.c_int), ("imp_rows", ctypes.c_int)]
+
+
+_vp, _ci, _cd, _cf, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
+_P = ctypes.POINTER
+SIGNATURES = {
+    "asarf_ctx_create": (_ci, [_ci, _vp, _P(_vp)]),
+    "asarf_ctx_destroy": (_ci, [_vp]),
+    "asarf_sync": (_ci, [_vp]),
+    "asarf_ctx_stream": (_vp, [_vp]),
+    "asarf_last_error": (ctypes.c_char_p, []),
+    "asarf_version": (ctypes.c_char_p, []),
+    "asarf_image_alloc": (_ci, [_vp, _ci, _ci, _ci, _ci, _ci, _P(Image)]),
+    "asarf_image_free": (_ci, [_vp, _P(Image)]),
+    "asarf_image_zero": (_ci, [_vp, _P(Image)]),
+    "asarf_image_bytes": (_sz, [_P(Image)]),
+    "asarf_workspace_alloc": (_ci, [_vp, _sz, _P(Workspace)]),
+    "asarf_workspace_free": (_ci, [_vp, _P(Workspace)]),
+    "asarf_image_adopt": (_ci, [_P(Image), _ci, _ci, _ci, _ci, _ci, _P(Workspace)]),
+    "asarf_image_release": (_ci, [_P(Image), _P(Workspace)]),
+    "asarf_image_subset_copy": (_ci, [_vp, _P(Image), _ci, _ci, _ci, _ci, _vp, _sz]),
+    "asarf_fft_padding_size": (_ci, [_ci]),
+    "asarf_generate_chirp": (_ci, [_P(ChirpInfo), _ci, _vp]),
+    "asarf_polyfit": (_ci, [_vp, _vp, _ci, _ci, _vp]),
+    "asarf_polyval": (_cd, [_vp, _ci, _cd]),
+    "asarf_iq_correction": (_ci, [_vp, _P(Image), _sz, _ci, _P(IqResults)]),
+    "asarf_doppler_centroid": (_ci, [_vp, _P(Image), _cd, _ci, _vp, _vp, _vp, _ci, _P(_ci)]),
+    "asarf_range_compression": (_ci, [_vp, _P(Image), _vp, _ci, _ci, _P(Workspace)]),
+    "asarf_range_doppler": (_ci, [_vp, _P(RdaParams), _P(Image), _P(Image), _P(Workspace)]),
+    "asarf_rcmc_window_size": (_ci, []),
+    "asarf_az_multilook": (_ci, [_vp, _P(Image), _cd, _P(Workspace), _P(_ci), _P(_cd)]),
+    "asarf_range_spreading_loss": (_ci, [_vp, _vp, _ci, _ci, _cd, _cd, _cd]),
+    "asarf_srgr": (_ci, [_vp, _vp, _ci, _ci, _vp, _cd, _cd, _ci, _vp]),
+    "asarf_srgr_fit_spherical": (_ci, [_cd, _cd, _ci, _cd, _cd, _cd, _vp, _P(_ci)]),
+    "asarf_fft_c2c": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_long, _ci, _ci, _cf]),
+    "asarf_plan_create": (_ci, [_vp, _P(FocusParams), _vp, _P(_vp)]),
+    "asarf_plan_destroy": (_ci, [_vp]),
+    "asarf_plan_device_bytes": (_sz, [_vp]),
+    "asarf_plan_execute": (_ci, [_vp, _vp, _P(Image), _vp]),
+    "asarf_plan_imp_capacity": (_ci, [_vp, _P(_ci), _P(_ci)]),
+    "asarf_plan_results": (_ci, [_vp, _P(FocusResults)]),
+    "asarf_plan_set_timing": (_ci, [_vp, _ci]),
+    "asarf_plan_kernel_times": (_ci, [_vp, _P(ctypes.c_char_p), _P(_cf), _ci, _P(_ci)]),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (the HIP library is required; run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status: int):
+    if status != OK:
+        msg = lib().asarf_last_error()
+        raise AsarfError(status, msg.decode() if msg else "")
+    return status
+
+
+def last_error() -> str:
+    m = lib().asarf_last_error()
+    return m.decode() if m else ""

@@ -1,0 +1,264 @@
+"""Python-side mirror of the reference's focusing interfaces over the C ABI, with torch tensors as device memory.
+
+Each function names the reference function it mirrors; arguments keep the reference's meaning. Device work runs
+on the torch stream the Context was created with, so torch events and synchronisation order correctly with it.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .configs import SceneConfig
+
+
+class Context:
+    """asarf_ctx bound to a CUDA (HIP) device and a torch stream (CudaInit, cuda_device_init.h:26-43)."""
+
+    def __init__(self, device: int = 0, stream: torch.cuda.Stream | None = None):
+        self.device = device
+        self.stream = stream if stream is not None else torch.cuda.current_stream(device)
+        h = ctypes.c_void_p()
+        L.check(L.lib().asarf_ctx_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)))
+        self.h = h
+
+    def sync(self):
+        L.check(L.lib().asarf_sync(self.h))
+
+    def close(self):
+        if self.h:
+            L.lib().asarf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def image(t: torch.Tensor, x_size: int, y_size: int) -> L.Image:
+    """asarf_image view of a 2-D torch tensor [y_stride, x_stride] (complex64 or float32)."""
+    assert t.is_cuda and t.is_contiguous() and t.dim() == 2
+    dtype = L.CF32 if t.dtype == torch.complex64 else L.F32
+    assert t.dtype in (torch.complex64, torch.float32)
+    return L.Image(t.data_ptr(), x_size, y_size, t.shape[1], t.shape[0], dtype)
+
+
+def iq_correction(ctx: Context, t: torch.Tensor, x_size: int, y_size: int, n_total: int = 0,
+                  quirks: int = L.QUIRKS_REFERENCE) -> dict:
+    """RawDataCorrection + ZeroNaNs (sar/iq_correction.cu:188-273)."""
+    img = image(t, x_size, y_size)
+    r = L.IqResults()
+    L.check(L.lib().asarf_iq_correction(ctx.h, ctypes.byref(img), n_total, quirks, ctypes.byref(r)))
+    return r.as_dict()
+
+
+def doppler_centroid(ctx: Context, t: torch.Tensor, x_size: int, y_size: int, prf: float,
+                     quirks: int = L.QUIRKS_REFERENCE):
+    """CalculateDopplerCentroid (sar/fractional_doppler_centroid.cu:98-162) -> (poly[3], block_x, block_freq)."""
+    img = image(t, x_size, y_size)
+    poly = np.zeros(3)
+    bf = np.zeros(64)
+    bx = np.zeros(64)
+    nb = ctypes.c_int()
+    L.check(L.lib().asarf_doppler_centroid(ctx.h, ctypes.byref(img), prf, quirks, poly.ctypes.data, bf.ctypes.data,
+                                           bx.ctypes.data, 64, ctypes.byref(nb)))
+    return poly, bx[:nb.value].copy(), bf[:nb.value].copy()
+
+
+def range_compression(ctx: Context, t: torch.Tensor, x_size: int, y_size: int, chirp: np.ndarray, nc: int) -> int:
+    """RangeCompression (sar/range_compression.cu:57-111), in place; returns the new x_size."""
+    img = image(t, x_size, y_size)
+    ch = np.ascontiguousarray(chirp, np.complex64)
+    L.check(L.lib().asarf_range_compression(ctx.h, ctypes.byref(img), ch.ctypes.data, ch.size, nc, None))
+    return img.x_size
+
+
+def rda_params(cfg: SceneConfig, dc_poly) -> L.RdaParams:
+    p = L.RdaParams()
+    p.wavelength, p.range_spacing, p.slant_range_first = cfg.wavelength, cfg.range_spacing, cfg.r_first
+    p.prf, p.az_bandwidth_fraction = cfg.prf, cfg.az_bandwidth_fraction
+    for i in range(3):
+        p.dc_poly[i] = dc_poly[i]
+        p.vr_poly[i] = cfg.vr_poly[i]
+    return p
+
+
+def range_doppler(ctx: Context, params: L.RdaParams, src: torch.Tensor, x_size: int, y_size: int,
+                  out: torch.Tensor | None = None) -> torch.Tensor:
+    """RangeDopplerAlgorithm (sar/range_doppler_algorithm.cu:312-407); src is consumed as scratch."""
+    s = image(src, x_size, y_size)
+    if out is None:
+        out = torch.empty_like(src)
+    o = image(out, x_size, y_size)
+    L.check(L.lib().asarf_range_doppler(ctx.h, ctypes.byref(params), ctypes.byref(s), ctypes.byref(o), None))
+    return out
+
+
+def az_multilook(ctx: Context, slc: torch.Tensor, x_size: int, y_size: int, az_spacing: float):
+    """AzimuthMultiLook (sar/az_multilook.cu:82-101) -> (float tensor [rows, x_size], new azimuth spacing)."""
+    img = image(slc, x_size, y_size)
+    ratio = np.float32(12.5 / az_spacing)
+    rows = int(np.round(np.float32(y_size) / ratio))
+    out = torch.empty((max(rows, 1), x_size), dtype=torch.float32, device=slc.device)
+    ws = L.Workspace(out.data_ptr(), out.numel() * 4)
+    nrow = ctypes.c_int()
+    sp = ctypes.c_double()
+    L.check(L.lib().asarf_az_multilook(ctx.h, ctypes.byref(img), az_spacing, ctypes.byref(ws), ctypes.byref(nrow),
+                                       ctypes.byref(sp)))
+    return out[: nrow.value], sp.value
+
+
+def range_spreading_loss(ctx: Context, img: torch.Tensor, r_first: float, dr: float, r_ref: float = 800000.0):
+    """RangeSpreadingLossCorrection (sar/range_spreading_loss.cu:42-69), in place."""
+    assert img.dtype == torch.float32 and img.is_contiguous()
+    L.check(L.lib().asarf_range_spreading_loss(ctx.h, img.data_ptr(), img.shape[1], img.shape[0], r_first, dr, r_ref))
+    return img
+
+
+def srgr(ctx: Context, img: torch.Tensor, grsr5, dr: float, gs: float, out_nr: int) -> torch.Tensor:
+    """SlantRangeToGroundRangeInterpolation kernel part (sar/srgr.cu:49-71)."""
+    g = np.ascontiguousarray(grsr5, np.float64)
+    out = torch.empty((img.shape[0], out_nr), dtype=torch.float32, device=img.device)
+    L.check(L.lib().asarf_srgr(ctx.h, img.data_ptr(), img.shape[1], img.shape[0], g.ctypes.data, dr, gs, out_nr,
+                               out.data_ptr()))
+    return out
+
+
+def srgr_fit_spherical(r_first, dr, range_size, orbit_radius, earth_radius, gs=12.5):
+    g = np.zeros(5)
+    n = ctypes.c_int()
+    L.check(L.lib().asarf_srgr_fit_spherical(r_first, dr, range_size, orbit_radius, earth_radius, gs, g.ctypes.data,
+                                             ctypes.byref(n)))
+    return g, n.value
+
+
+def fft(ctx: Context, t: torch.Tensor, layout: int = L.FFT_ROWS, inverse: bool = False, scale: float = 1.0,
+        n: int | None = None, batch: int | None = None):
+    """Batched C2C FFT in place (rows: transforms along the last axis; columns: along the first)."""
+    assert t.dtype == torch.complex64 and t.is_contiguous() and t.is_cuda and t.dim() == 2
+    if layout == L.FFT_ROWS:
+        n = t.shape[1] if n is None else n
+        batch = t.shape[0] if batch is None else batch
+    else:
+        n = t.shape[0] if n is None else n
+        batch = t.shape[1] if batch is None else batch
+    L.check(L.lib().asarf_fft_c2c(ctx.h, t.data_ptr(), n, batch, t.shape[1], layout,
+                                  L.INVERSE if inverse else L.FORWARD, scale))
+    return t
+
+
+def generate_chirp(cfg: SceneConfig, padding: int) -> np.ndarray:
+    """GenerateChirpData (sar/sar_chirp.cc:70-104) via the library's FP64 host implementation."""
+    ci = L.ChirpInfo(cfg.fs, cfg.pulse_len, cfg.nc, cfg.bandwidth, cfg.kr, int(cfg.apply_range_window))
+    out = np.zeros(padding, np.complex64)
+    L.check(L.lib().asarf_generate_chirp(ctypes.byref(ci), padding, out.ctypes.data))
+    return out
+
+
+def padding_size(n: int) -> int:
+    return L.lib().asarf_fft_padding_size(int(n))
+
+
+def subset_copy(ctx: Context, img: L.Image, x0: int, y0: int, w: int, h: int, dst: torch.Tensor):
+    eb = 8 if img.dtype == L.CF32 else 4
+    L.check(L.lib().asarf_image_subset_copy(ctx.h, ctypes.byref(img), x0, y0, w, h, dst.data_ptr(),
+                                            dst.stride(0) * eb))
+    return dst
+
+
+class Plan:
+    """Fused focusing plan (the GPU stage of main.cc:233-327) for one scene configuration."""
+
+    def __init__(self, ctx: Context, cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
+                 chirp: np.ndarray | None = None, orbit_height: float | None = None):
+        self.ctx = ctx
+        self.cfg = cfg
+        p = L.FocusParams()
+        p.na, p.nr, p.nc = cfg.na, cfg.nr, cfg.nc
+        p.nrp = cfg.nrp or padding_size(cfg.nr + cfg.nc)
+        p.nap = cfg.nap or padding_size(cfg.na)
+        p.raw_format = L.RAW_U8 if cfg.raw_format == "u8_ers" else L.RAW_I8
+        p.prf, p.wavelength, p.range_spacing = cfg.prf, cfg.wavelength, cfg.range_spacing
+        p.slant_range_first, p.az_bandwidth_fraction = cfg.r_first, cfg.az_bandwidth_fraction
+        for i, v in enumerate(cfg.vr_poly):
+            p.vr_poly[i] = v
+        p.n_total_samples = 0
+        p.quirks = quirks
+        p.imp = int(imp)
+        self.nrv = cfg.nr - cfg.nc
+        if imp:
+            near = 8
+            nr_s = self.nrv - 16
+            h = cfg.orbit_height if orbit_height is None else orbit_height
+            grsr, ng = srgr_fit_spherical(cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, nr_s,
+                                          cfg.earth_radius + h, cfg.earth_radius, 12.5)
+            p.azimuth_spacing = cfg.az_spacing
+            p.reference_range = 800000.0
+            for i in range(5):
+                p.grsr_poly[i] = grsr[i]
+            p.gr_range_size = ng
+            p.ground_spacing = 12.5
+            self.grsr, self.ng = grsr, ng
+        self.params = p
+        self.chirp = generate_chirp(cfg, p.nrp) if chirp is None else np.ascontiguousarray(chirp, np.complex64)
+        h = ctypes.c_void_p()
+        L.check(L.lib().asarf_plan_create(ctx.h, ctypes.byref(p), self.chirp.ctypes.data, ctypes.byref(h)))
+        self.h = h
+        rows, cols = ctypes.c_int(), ctypes.c_int()
+        L.check(L.lib().asarf_plan_imp_capacity(h, ctypes.byref(rows), ctypes.byref(cols)))
+        self.imp_shape = (rows.value, cols.value)
+        self.imp_out = None
+        if imp:
+            self.imp_out = torch.zeros(self.imp_shape, dtype=torch.float32, device=f"cuda:{ctx.device}")
+        self.slc = L.Image()
+
+    @property
+    def device_bytes(self) -> int:
+        return int(L.lib().asarf_plan_device_bytes(self.h))
+
+    def execute(self, raw: torch.Tensor):
+        assert raw.is_cuda and raw.is_contiguous() and raw.numel() == self.cfg.na * self.cfg.nr * 2
+        L.check(L.lib().asarf_plan_execute(self.h, raw.data_ptr(), ctypes.byref(self.slc),
+                                           self.imp_out.data_ptr() if self.imp_out is not None else None))
+        return self.slc
+
+    def slc_tensor(self) -> torch.Tensor:
+        """Copy of the focused SLC (na x nrv complex64)."""
+        out = torch.empty((self.slc.y_size, self.slc.x_size), dtype=torch.complex64,
+                          device=f"cuda:{self.ctx.device}")
+        subset_copy(self.ctx, self.slc, 0, 0, self.slc.x_size, self.slc.y_size, out)
+        return out
+
+    def results(self) -> dict:
+        r = L.FocusResults()
+        L.check(L.lib().asarf_plan_results(self.h, ctypes.byref(r)))
+        nb = r.n_blocks
+        return {"iq": r.iq.as_dict(), "dc_poly": list(r.dc_poly), "n_blocks": nb,
+                "block_freq": list(r.block_freq[:nb]), "block_x": list(r.block_x[:nb]), "nrv": r.nrv,
+                "crop": (r.crop_front, r.crop_back, r.crop_near, r.crop_far), "imp_rows": r.imp_rows}
+
+    def set_timing(self, on: bool):
+        L.check(L.lib().asarf_plan_set_timing(self.h, int(on)))
+
+    def kernel_times(self) -> dict:
+        names = (ctypes.c_char_p * 32)()
+        ms = (ctypes.c_float * 32)()
+        n = ctypes.c_int()
+        L.check(L.lib().asarf_plan_kernel_times(self.h, names, ms, 32, ctypes.byref(n)))
+        return {names[i].decode(): ms[i] for i in range(min(n.value, 32))}
+
+    def close(self):
+        if self.h:
+            L.lib().asarf_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

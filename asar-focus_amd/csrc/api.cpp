@@ -1,0 +1,829 @@
+// C ABI of libasarf.so: context, memory model, FFT planning, per-stage entry points and the fused plan.
+// Every entry point validates its arguments before touching the device.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "asarf.h"
+#include "internal.h"
+#include "poly.h"
+
+using namespace asarf;
+
+// ---------------------------------------------------------------------------------------------- errors ----
+namespace asarf {
+static thread_local char t_err[512] = "";
+
+int set_error(int status, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(t_err, sizeof(t_err), fmt, ap);
+    va_end(ap);
+    return status;
+}
+int clear_error() {
+    t_err[0] = 0;
+    return ASARF_OK;
+}
+}  // namespace asarf
+
+#define HIP_CHECK(expr)                                                                                   \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return set_error(e_ == hipErrorOutOfMemory ? ASARF_ENOMEM : ASARF_EHIP, "%s: %s (%s:%d)", #expr, \
+                             hipGetErrorString(e_), __FILE__, __LINE__);                                  \
+    } while (0)
+
+#define ST_CHECK(expr)             \
+    do {                           \
+        int s_ = (expr);           \
+        if (s_ != ASARF_OK) return s_; \
+    } while (0)
+
+extern "C" const char* asarf_last_error(void) { return t_err; }
+extern "C" const char* asarf_version(void) { return "asarf 0.1.0 (gfx950)"; }
+
+// ---------------------------------------------------------------------------------------------- context ----
+extern "C" int asarf_ctx_create(int device, void* stream, asarf_ctx** out) {
+    if (!out) return set_error(ASARF_EINVAL, "asarf_ctx_create: out is null");
+    if (device < 0) return set_error(ASARF_EINVAL, "asarf_ctx_create: device %d", device);
+    auto* c = new asarf_ctx();
+    c->device = device;
+    c->stream = static_cast<hipStream_t>(stream);
+    c->own_stream = false;
+    *out = c;
+    return clear_error();
+}
+
+namespace asarf {
+int ctx_init(asarf_ctx* c) {
+    if (c->initialized) return ASARF_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= c->device)
+        return set_error(ASARF_EHIP, "no HIP device %d (%d visible): %s", c->device, n, hipGetErrorString(e));
+    HIP_CHECK(hipSetDevice(c->device));
+    int lds = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    c->max_lds = (size_t)lds;
+    set_max_lds_bytes(c->max_lds);
+    c->initialized = true;
+    return ASARF_OK;
+}
+
+int ctx_scratch(asarf_ctx* c, size_t bytes, void** out) {
+    if (bytes > c->scratch_bytes) {
+        if (c->d_scratch) {
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            HIP_CHECK(hipFree(c->d_scratch));
+            c->d_scratch = nullptr;
+            c->scratch_bytes = 0;
+        }
+        const size_t want = std::max(bytes, (size_t)1 << 20);
+        HIP_CHECK(hipMalloc(&c->d_scratch, want));
+        c->scratch_bytes = want;
+    }
+    *out = c->d_scratch;
+    return ASARF_OK;
+}
+
+int ctx_pinned(asarf_ctx* c, size_t bytes, void** out) {
+    if (bytes > c->pinned_bytes) {
+        if (c->h_pinned) {
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            HIP_CHECK(hipHostFree(c->h_pinned));
+            c->h_pinned = nullptr;
+            c->pinned_bytes = 0;
+        }
+        const size_t want = std::max(bytes, (size_t)1 << 16);
+        HIP_CHECK(hipHostMalloc(&c->h_pinned, want, hipHostMallocDefault));
+        c->pinned_bytes = want;
+    }
+    *out = c->h_pinned;
+    return ASARF_OK;
+}
+
+// Radix order: 16s first, then one 8/4/2 for the remaining power of two, then 3, 5, 7.
+bool radix_plan(int n, int lanes, std::vector<int>& out) {
+    out.clear();
+    if (n < 2) return false;
+    int m = n;
+    while (m % 16 == 0) {
+        out.push_back(16);
+        m /= 16;
+    }
+    if (m % 8 == 0) {
+        out.push_back(8);
+        m /= 8;
+    } else if (m % 4 == 0) {
+        out.push_back(4);
+        m /= 4;
+    } else if (m % 2 == 0) {
+        out.push_back(2);
+        m /= 2;
+    }
+    for (int f : {3, 5, 7})
+        while (m % f == 0) {
+            out.push_back(f);
+            m /= f;
+        }
+    if (m != 1 || (int)out.size() > kMaxPasses) return false;
+    // every pass must fit the per-lane register budget: ceil((n/R)/lanes) butterflies of R elements <= kMaxE
+    for (int r : out) {
+        const int nb = (n / r + lanes - 1) / lanes;
+        if (nb > kMaxE / r) {
+            // split a 16 into 4x4 or an 8 into 4x2 and retry once
+            return false;
+        }
+    }
+    return true;
+}
+
+static bool radix_plan_relaxed(int n, int lanes, std::vector<int>& out) {
+    if (radix_plan(n, lanes, out)) return true;
+    // replace 16s by 4x4 (more passes, smaller register blocks)
+    std::vector<int> alt;
+    for (int r : out) {
+        if (r == 16) {
+            alt.push_back(4);
+            alt.push_back(4);
+        } else {
+            alt.push_back(r);
+        }
+    }
+    if ((int)alt.size() > kMaxPasses) return false;
+    for (int r : alt) {
+        const int nb = (n / r + lanes - 1) / lanes;
+        if (nb > kMaxE / r) return false;
+    }
+    out = alt;
+    return !out.empty();
+}
+
+float2* get_twiddles(asarf_ctx* c, int n, int* status) {
+    auto it = c->tw_tables.find(n);
+    if (it != c->tw_tables.end()) return it->second;
+    std::vector<float2> h(n);
+    for (int m = 0; m < n; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)n;
+        h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    float2* d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(float2) * (size_t)n);
+    if (e != hipSuccess) {
+        *status = set_error(ASARF_ENOMEM, "twiddle table %d: %s", n, hipGetErrorString(e));
+        return nullptr;
+    }
+    e = hipMemcpy(d, h.data(), sizeof(float2) * (size_t)n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        *status = set_error(ASARF_EHIP, "twiddle upload: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    c->tw_tables[n] = d;
+    return d;
+}
+
+const FftPlanHost* get_plan(asarf_ctx* c, int n, int lanes, int* status) {
+    const auto key = std::make_pair(n, lanes);
+    auto it = c->plans.find(key);
+    if (it != c->plans.end()) return it->second.get();
+    auto p = std::make_unique<FftPlanHost>();
+    p->n = n;
+    if (!radix_plan_relaxed(n, lanes, p->radix)) {
+        *status = set_error(ASARF_EUNSUPPORTED, "no Stockham plan for length %d with %d lanes per transform", n, lanes);
+        return nullptr;
+    }
+    p->d_tw = get_twiddles(c, n, status);
+    if (!p->d_tw) return nullptr;
+    const FftPlanHost* raw = p.get();
+    c->plans[key] = std::move(p);
+    return raw;
+}
+
+int get_col_split(asarf_ctx* c, int n, ColSplit* cs) {
+    const int lanes = kThreads / kColTile;
+    const size_t lds_cap = c->max_lds / (sizeof(float2) * (kColTile + 1));
+    std::vector<int> tmp;
+    cs->n = n;
+    int st = ASARF_OK;
+    if ((size_t)n <= lds_cap && radix_plan_relaxed(n, lanes, tmp)) {
+        cs->n1 = n;
+        cs->n2 = 1;
+        cs->p1 = get_plan(c, n, lanes, &st);
+        cs->p2 = nullptr;
+        cs->full = nullptr;
+        return cs->p1 ? ASARF_OK : st;
+    }
+    int best1 = 0, best2 = 0;
+    for (int n1 = 2; n1 <= n / 2; ++n1) {
+        if (n % n1) continue;
+        const int n2 = n / n1;
+        if ((size_t)n1 > lds_cap || (size_t)n2 > lds_cap) continue;
+        if (!radix_plan_relaxed(n1, lanes, tmp) || !radix_plan_relaxed(n2, lanes, tmp)) continue;
+        if (!best1 || std::max(n1, n2) < std::max(best1, best2) ||
+            (std::max(n1, n2) == std::max(best1, best2) && n1 < best1)) {
+            best1 = n1;
+            best2 = n2;
+        }
+    }
+    if (!best1) return set_error(ASARF_EUNSUPPORTED, "no two-pass split for column transform length %d", n);
+    cs->n1 = best1;
+    cs->n2 = best2;
+    cs->p1 = get_plan(c, best1, lanes, &st);
+    if (!cs->p1) return st;
+    cs->p2 = get_plan(c, best2, lanes, &st);
+    if (!cs->p2) return st;
+    FftPlanHost* full = new FftPlanHost();
+    full->n = n;
+    full->d_tw = get_twiddles(c, n, &st);
+    if (!full->d_tw) {
+        delete full;
+        return st;
+    }
+    c->plans[std::make_pair(n, -1)] = std::unique_ptr<FftPlanHost>(full);
+    cs->full = full;
+    return ASARF_OK;
+}
+}  // namespace asarf
+
+extern "C" int asarf_ctx_destroy(asarf_ctx* c) {
+    if (!c) return set_error(ASARF_EINVAL, "asarf_ctx_destroy: null");
+    if (c->initialized) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (auto& kv : c->tw_tables) (void)hipFree(kv.second);
+        if (c->d_scratch) (void)hipFree(c->d_scratch);
+        if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+        if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return clear_error();
+}
+
+extern "C" int asarf_sync(asarf_ctx* c) {
+    if (!c) return set_error(ASARF_EINVAL, "asarf_sync: null ctx");
+    if (!c->initialized) return clear_error();
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    return clear_error();
+}
+
+extern "C" void* asarf_ctx_stream(asarf_ctx* c) {
+    if (!c) return nullptr;
+    if (ctx_init(c) != ASARF_OK) return nullptr;
+    return c->stream;
+}
+
+// ---------------------------------------------------------------------------------------- memory model ----
+static size_t elem_bytes(int dtype) { return dtype == ASARF_F32 ? 4 : 8; }
+
+extern "C" size_t asarf_image_bytes(const asarf_image* img) {
+    if (!img) return 0;
+    return (size_t)img->x_stride * (size_t)img->y_stride * elem_bytes(img->dtype);
+}
+
+static int check_geometry(int xs, int ys, int xstr, int ystr, int dtype, const char* who) {
+    if (dtype != ASARF_CF32 && dtype != ASARF_F32) return set_error(ASARF_EINVAL, "%s: dtype %d", who, dtype);
+    if (xs < 0 || ys < 0 || xstr <= 0 || ystr <= 0) return set_error(ASARF_ESHAPE, "%s: negative size", who);
+    if (xstr < xs || ystr < ys) return set_error(ASARF_ESHAPE, "%s: Stride smaller than dimension", who);
+    return ASARF_OK;
+}
+
+extern "C" int asarf_image_alloc(asarf_ctx* c, int xs, int ys, int xstr, int ystr, int dtype, asarf_image* out) {
+    if (!c || !out) return set_error(ASARF_EINVAL, "asarf_image_alloc: null argument");
+    ST_CHECK(check_geometry(xs, ys, xstr, ystr, dtype, "asarf_image_alloc"));
+    ST_CHECK(ctx_init(c));
+    const size_t bytes = (size_t)xstr * ystr * elem_bytes(dtype);
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, bytes));
+    HIP_CHECK(hipMemsetAsync(p, 0, bytes, c->stream));
+    *out = asarf_image{p, xs, ys, xstr, ystr, dtype};
+    return clear_error();
+}
+
+extern "C" int asarf_image_free(asarf_ctx* c, asarf_image* img) {
+    if (!c || !img) return set_error(ASARF_EINVAL, "asarf_image_free: null argument");
+    if (img->data) {
+        ST_CHECK(ctx_init(c));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipFree(img->data));
+    }
+    *img = asarf_image{nullptr, 0, 0, 0, 0, img->dtype};
+    return clear_error();
+}
+
+extern "C" int asarf_image_zero(asarf_ctx* c, asarf_image* img) {
+    if (!c || !img || !img->data) return set_error(ASARF_EINVAL, "asarf_image_zero: null argument");
+    ST_CHECK(ctx_init(c));
+    HIP_CHECK(hipMemsetAsync(img->data, 0, asarf_image_bytes(img), c->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_workspace_alloc(asarf_ctx* c, size_t bytes, asarf_workspace* out) {
+    if (!c || !out) return set_error(ASARF_EINVAL, "asarf_workspace_alloc: null argument");
+    if (bytes == 0) return set_error(ASARF_EINVAL, "asarf_workspace_alloc: zero bytes");
+    ST_CHECK(ctx_init(c));
+    void* p = nullptr;
+    HIP_CHECK(hipMalloc(&p, bytes));
+    *out = asarf_workspace{p, bytes};
+    return clear_error();
+}
+
+extern "C" int asarf_workspace_free(asarf_ctx* c, asarf_workspace* ws) {
+    if (!c || !ws) return set_error(ASARF_EINVAL, "asarf_workspace_free: null argument");
+    if (ws->data) {
+        ST_CHECK(ctx_init(c));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        HIP_CHECK(hipFree(ws->data));
+    }
+    *ws = asarf_workspace{nullptr, 0};
+    return clear_error();
+}
+
+extern "C" int asarf_image_adopt(asarf_image* img, int xs, int ys, int xstr, int ystr, int dtype, asarf_workspace* ws) {
+    if (!img || !ws) return set_error(ASARF_EINVAL, "asarf_image_adopt: null argument");
+    if (img->data) return set_error(ASARF_EINVAL, "asarf_image_adopt: DevicePaddedImage already initialized");
+    ST_CHECK(check_geometry(xs, ys, xstr, ystr, dtype, "asarf_image_adopt"));
+    const size_t need = (size_t)xstr * ystr * elem_bytes(dtype);
+    if (!ws->data || ws->bytes < need)
+        return set_error(ASARF_EWORKSPACE, "asarf_image_adopt: External workspace does not have enough memory "
+                                           "(%zu < %zu)", ws->bytes, need);
+    *img = asarf_image{ws->data, xs, ys, xstr, ystr, dtype};
+    *ws = asarf_workspace{nullptr, 0};
+    return clear_error();
+}
+
+extern "C" int asarf_image_release(asarf_image* img, asarf_workspace* ws) {
+    if (!img || !ws) return set_error(ASARF_EINVAL, "asarf_image_release: null argument");
+    if (ws->data) return set_error(ASARF_EINVAL, "asarf_image_release: workspace not empty");
+    *ws = asarf_workspace{img->data, asarf_image_bytes(img)};
+    *img = asarf_image{nullptr, 0, 0, 0, 0, img->dtype};
+    return clear_error();
+}
+
+extern "C" int asarf_image_subset_copy(asarf_ctx* c, const asarf_image* img, int x0, int y0, int w, int h, void* dst,
+                                       size_t pitch) {
+    if (!c || !img || !img->data || !dst) return set_error(ASARF_EINVAL, "asarf_image_subset_copy: null argument");
+    if (x0 < 0 || y0 < 0 || w <= 0 || h <= 0 || x0 + w > img->x_size || y0 + h > img->y_size)
+        return set_error(ASARF_ESHAPE, "asarf_image_subset_copy: window (%d,%d,%d,%d) outside %dx%d", x0, y0, w, h,
+                         img->x_size, img->y_size);
+    const size_t eb = elem_bytes(img->dtype);
+    if (pitch < (size_t)w * eb) return set_error(ASARF_ESHAPE, "asarf_image_subset_copy: pitch %zu too small", pitch);
+    ST_CHECK(ctx_init(c));
+    const char* src = static_cast<const char*>(img->data) + ((size_t)y0 * img->x_stride + x0) * eb;
+    HIP_CHECK(hipMemcpy2DAsync(dst, pitch, src, (size_t)img->x_stride * eb, (size_t)w * eb, h, hipMemcpyDefault,
+                               c->stream));
+    return clear_error();
+}
+
+// ---------------------------------------------------------------------------------------------- stages ----
+static int check_cf32(const asarf_image* img, const char* who) {
+    if (!img || !img->data) return set_error(ASARF_EINVAL, "%s: null image", who);
+    if (img->dtype != ASARF_CF32) return set_error(ASARF_EINVAL, "%s: image must be cf32", who);
+    ST_CHECK(check_geometry(img->x_size, img->y_size, img->x_stride, img->y_stride, img->dtype, who));
+    if (img->x_size <= 0 || img->y_size <= 0) return set_error(ASARF_ESHAPE, "%s: empty image", who);
+    return ASARF_OK;
+}
+
+// I/Q correction coefficients from logical and padding moments (see k_stats.hip for the derivation).
+static IqCoef iq_coefficients(const double* t, int xs, int ys, size_t n_total, int quirks, asarf_iq_results* r) {
+    const double nL = t[0], SI = t[1], SQ = t[2], SII = t[3], SQQ = t[4], SIQ = t[5];
+    const double nP = t[6], PI_ = t[7], PQ = t[8], PII = t[9], PQQ = t[10], PIQ = t[11];
+    const double ntot = n_total ? (double)n_total : (double)xs * (double)ys;
+    const double dci = SI / ntot, dcq = SQ / ntot;
+    r->dc_i = dci;
+    r->dc_q = dcq;
+    r->stddev_i = std::sqrt((SII - 2.0 * dci * SI + nL * dci * dci) / ntot);
+    r->stddev_q = std::sqrt((SQQ - 2.0 * dcq * SQ + nL * dcq * dcq) / ntot);
+    double n = nL, sI = SI, sQ = SQ, sII = SII, sQQ = SQQ, sIQ = SIQ;
+    if (quirks & ASARF_Q1_IQ_PADDED_WIDTH) {
+        n += nP;
+        sI += PI_;
+        sQ += PQ;
+        sII += PII;
+        sQQ += PQQ;
+        sIQ += PIQ;
+    }
+    const double aii = sII - 2.0 * dci * sI + n * dci * dci;
+    const double aqq = sQQ - 2.0 * dcq * sQ + n * dcq * dcq;
+    const double g = std::sqrt(aii / aqq);
+    const double aiq = (sIQ - dcq * sI - dci * sQ + n * dci * dcq) / g;
+    const double d = aiq / (aii / (g * g));
+    const double cosc = (quirks & ASARF_Q2_COS_NO_SQRT) ? (1.0 - d * d) : std::sqrt(1.0 - d * d);
+    r->iq_gain = g;
+    r->phase_error = std::asin(d) * 180.0 / M_PI;
+    IqCoef c;
+    c.dc_i = (float)dci;
+    c.dc_q = (float)dcq;
+    c.inv_gain = (float)(1.0 / g);
+    c.sin_corr = (float)d;
+    c.inv_cos = (float)(1.0 / cosc);
+    c.identity = 0;
+    return c;
+}
+
+extern "C" int asarf_iq_correction(asarf_ctx* c, asarf_image* img, size_t n_total, int quirks, asarf_iq_results* res) {
+    if (!c || !res) return set_error(ASARF_EINVAL, "asarf_iq_correction: null argument");
+    ST_CHECK(check_cf32(img, "asarf_iq_correction"));
+    if (quirks & ~ASARF_QUIRKS_REFERENCE) return set_error(ASARF_EINVAL, "asarf_iq_correction: unknown quirk bits");
+    ST_CHECK(ctx_init(c));
+    const int nwg = std::min(img->y_size, 1024);
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(double) * (size_t)(nwg + 1) * 12, &scratch));
+    double* partials = static_cast<double*>(scratch);
+    double* totals = partials + (size_t)nwg * 12;
+    auto* f2 = static_cast<float2*>(img->data);
+    HIP_CHECK(launch_moments_cf32(f2, img->x_stride, img->x_size, img->y_size, img->x_stride, partials, nwg, c->stream));
+    HIP_CHECK(launch_reduce_f64(partials, nwg, 12, totals, c->stream));
+    void* hp = nullptr;
+    ST_CHECK(ctx_pinned(c, sizeof(double) * 12, &hp));
+    HIP_CHECK(hipMemcpyAsync(hp, totals, sizeof(double) * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    double t[12];
+    std::memcpy(t, hp, sizeof(t));
+    const IqCoef coef = iq_coefficients(t, img->x_size, img->y_size, n_total, quirks, res);
+    const int width = (quirks & ASARF_Q1_IQ_PADDED_WIDTH) ? img->x_stride : img->x_size;
+    HIP_CHECK(launch_iq_apply_cf32(f2, img->x_stride, img->x_size, img->y_size, img->x_stride, img->y_stride, width,
+                                   coef, c->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_doppler_centroid(asarf_ctx* c, const asarf_image* img, double prf, int quirks, double* poly3,
+                                      double* block_freq, double* block_x, int max_blocks, int* n_blocks) {
+    if (!c || !poly3) return set_error(ASARF_EINVAL, "asarf_doppler_centroid: null argument");
+    ST_CHECK(check_cf32(img, "asarf_doppler_centroid"));
+    if (!(prf > 0.0)) return set_error(ASARF_EINVAL, "asarf_doppler_centroid: prf must be > 0");
+    const int xs = img->x_size, ys = img->y_size;
+    const int step = xs / 32;
+    if (step <= 0) return set_error(ASARF_ESHAPE, "asarf_doppler_centroid: range size %d < 32", xs);
+    int nb = 0;
+    for (int xb = 0; xb < xs; xb += step) {
+        if (!(quirks & ASARF_Q4_DC_EXTRA_BLOCK) && xb + step > xs) break;
+        ++nb;
+    }
+    if (nb > kMaxDcBlocks) return set_error(ASARF_EUNSUPPORTED, "asarf_doppler_centroid: %d blocks", nb);
+    if (ys < 2) return set_error(ASARF_ESHAPE, "asarf_doppler_centroid: need at least 2 lines");
+    ST_CHECK(ctx_init(c));
+    const int nstrips = std::max(1, std::min(ys - 1, 2048 / nb));
+    const size_t nvals = (size_t)nstrips * nb * 2;
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(double) * nvals, &scratch));
+    HIP_CHECK(launch_dc_cf32(static_cast<const float2*>(img->data), img->x_stride, ys, 0, step, nb, img->x_stride,
+                             nstrips, static_cast<double*>(scratch), c->stream));
+    void* hp = nullptr;
+    ST_CHECK(ctx_pinned(c, sizeof(double) * nvals, &hp));
+    HIP_CHECK(hipMemcpyAsync(hp, scratch, sizeof(double) * nvals, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    const double* p = static_cast<const double*>(hp);
+    std::vector<double> fx(nb), fy(nb);
+    for (int b = 0; b < nb; ++b) {
+        double re = 0.0, im = 0.0;
+        for (int s = 0; s < nstrips; ++s) {
+            re += p[((size_t)s * nb + b) * 2];
+            im += p[((size_t)s * nb + b) * 2 + 1];
+        }
+        const int xb = b * step;
+        fx[b] = (double)((xb + xb + step) / 2);
+        fy[b] = prf * std::atan2(im, re) / (2.0 * M_PI);
+        if (block_freq && b < max_blocks) block_freq[b] = fy[b];
+        if (block_x && b < max_blocks) block_x[b] = fx[b];
+    }
+    if (n_blocks) *n_blocks = nb;
+    if (polyfit_qr(fx.data(), fy.data(), nb, 2, poly3) != 0)
+        return set_error(ASARF_EINVAL, "asarf_doppler_centroid: polynomial fit failed");
+    return clear_error();
+}
+
+// conj(FFT(chirp)) / n into d_h (device, n complex). Uses d_h as the transform buffer.
+static int chirp_spectrum(asarf_ctx* c, const float* chirp_iq, int n, float2* d_h) {
+    int st = ASARF_OK;
+    const FftPlanHost* p = get_plan(c, n, kRowThreads, &st);
+    if (!p) return st;
+    std::vector<float2> h(n);
+    for (int i = 0; i < n; ++i) h[i] = make_float2(chirp_iq[2 * i], -chirp_iq[2 * i + 1]);
+    // conj(FFT(x)) = conj(sum x W^nk) = inverse-direction transform of conj(x), i.e. sum conj(x) W^-nk
+    HIP_CHECK(hipMemcpyAsync(d_h, h.data(), sizeof(float2) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    RowFftArgs a{};
+    a.in = d_h;
+    a.in_stride = n;
+    a.out = d_h;
+    a.out_stride = n;
+    a.rows = 1;
+    a.scale = 1.0f / (float)n;
+    a.plan = p->dev();
+    HIP_CHECK(launch_row_fft(a, true, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // h (host) must outlive the copy
+    return ASARF_OK;
+}
+
+extern "C" int asarf_range_compression(asarf_ctx* c, asarf_image* img, const float* chirp_iq, int n_chirp, int nc,
+                                       asarf_workspace* ws) {
+    if (!c || !chirp_iq) return set_error(ASARF_EINVAL, "asarf_range_compression: null argument");
+    ST_CHECK(check_cf32(img, "asarf_range_compression"));
+    if (n_chirp != img->x_stride)
+        return set_error(ASARF_ESHAPE, "asarf_range_compression: Range padding and chirp size mismatch (%d vs %d)",
+                         n_chirp, img->x_stride);
+    if (nc <= 0 || nc >= img->x_size)
+        return set_error(ASARF_EINVAL, "asarf_range_compression: chirp samples %d vs range size %d", nc, img->x_size);
+    ST_CHECK(ctx_init(c));
+    int st = ASARF_OK;
+    const FftPlanHost* p = get_plan(c, n_chirp, kRowThreads, &st);
+    if (!p) return st;
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(float2) * (size_t)n_chirp, &scratch));
+    float2* d_h = static_cast<float2*>(scratch);
+    ST_CHECK(chirp_spectrum(c, chirp_iq, n_chirp, d_h));
+    RcArgs a{};
+    a.in = img->data;
+    a.in_stride = img->x_stride;
+    a.nr_in = img->x_stride;
+    a.out = static_cast<float2*>(img->data);
+    a.out_stride = img->x_stride;
+    a.rows = img->y_size;
+    a.nrv = img->x_size - nc;
+    a.zero_to = img->x_stride;
+    a.coef = nullptr;
+    a.hconj = d_h;
+    a.plan = p->dev();
+    HIP_CHECK(launch_range_compress(a, RAW_CF32, c->stream));
+    img->x_size = a.nrv;
+    return clear_error();
+}
+
+namespace {
+ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stride, int ncols, int groups, long in_g,
+                    long in_m, int in_rows, long out_g, long out_m, int out_rows, const float2* twf, float scale,
+                    const FftPlanHost* sub) {
+    ColFftArgs a{};
+    a.in = in;
+    a.in_stride = in_stride;
+    a.out = out;
+    a.out_stride = out_stride;
+    a.ncols = ncols;
+    a.ntiles = (ncols + kColTile - 1) / kColTile;
+    a.groups = groups;
+    a.in_g = in_g;
+    a.in_m = in_m;
+    a.in_rows = in_rows;
+    a.out_g = out_g;
+    a.out_m = out_m;
+    a.out_rows = out_rows;
+    a.tw_full = twf;
+    a.scale = scale;
+    a.sub = sub->dev();
+    return a;
+}
+}  // namespace
+
+namespace asarf {
+// Forward column transform into permuted storage: row k1*n2 + k2 holds bin k1 + n1*k2. pass 0 = A (strided
+// length-n1 transforms with the inter-pass twiddle), pass 1 = B (contiguous length-n2 transforms).
+int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
+                     float2* img, long stride, int ncols) {
+    const int n = cs.n;
+    if (cs.n2 == 1) {
+        if (pass == 0)
+            HIP_CHECK(launch_col_fft(col_args(in, in_stride, img, stride, ncols, 1, 0, 1, in_rows, 0, 1, n, nullptr,
+                                              1.f, cs.p1),
+                                     false, c->stream));
+        return ASARF_OK;
+    }
+    if (pass == 0)
+        HIP_CHECK(launch_col_fft(col_args(in, in_stride, img, stride, ncols, cs.n2, 1, cs.n2, in_rows, 1, cs.n2, n,
+                                          cs.full->d_tw, 1.f, cs.p1),
+                                 false, c->stream));
+    else
+        HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr,
+                                          1.f, cs.p2),
+                                 false, c->stream));
+    return ASARF_OK;
+}
+
+// Inverse column transform from permuted storage; writes rows < out_rows scaled by `scale`.
+// pass 0 = B' (contiguous, inverse inter-pass twiddle), pass 1 = A' (strided, scale and row limit).
+int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,
+                     float scale) {
+    const int n = cs.n;
+    if (cs.n2 == 1) {
+        if (pass == 0)
+            HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, 1, 0, 1, n, 0, 1, out_rows, nullptr,
+                                              scale, cs.p1),
+                                     true, c->stream));
+        return ASARF_OK;
+    }
+    if (pass == 0)
+        HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n,
+                                          cs.full->d_tw, 1.f, cs.p2),
+                                 true, c->stream));
+    else
+        HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows,
+                                          nullptr, scale, cs.p1),
+                                 true, c->stream));
+    return ASARF_OK;
+}
+}  // namespace asarf
+
+extern "C" int asarf_range_doppler(asarf_ctx* c, const asarf_rda_params* p, asarf_image* src, asarf_image* out,
+                                   asarf_workspace* ws) {
+    if (!c || !p || !out) return set_error(ASARF_EINVAL, "asarf_range_doppler: null argument");
+    ST_CHECK(check_cf32(src, "asarf_range_doppler"));
+    if (!(p->prf > 0) || !(p->wavelength > 0) || !(p->range_spacing > 0) || !(p->slant_range_first > 0) ||
+        !(p->az_bandwidth_fraction > 0))
+        return set_error(ASARF_EINVAL, "asarf_range_doppler: non-positive radar parameter");
+    for (int i = 0; i < 3; ++i)
+        if (!std::isfinite(p->dc_poly[i]) || !std::isfinite(p->vr_poly[i]))
+            return set_error(ASARF_EINVAL, "asarf_range_doppler: non-finite polynomial");
+    const double vr0 = polyval(p->vr_poly, 3, 0.0), vr1 = polyval(p->vr_poly, 3, (double)src->x_size);
+    if (!(vr0 > 0) || !(vr1 > 0)) return set_error(ASARF_EINVAL, "asarf_range_doppler: Vr must be > 0");
+    const bool adopt = (out->data == nullptr);
+    if (adopt) {
+        if (!ws) return set_error(ASARF_EINVAL, "asarf_range_doppler: no workspace to adopt");
+        const size_t need = asarf_image_bytes(src);
+        if (!ws->data || ws->bytes < need)
+            return set_error(ASARF_EWORKSPACE, "asarf_range_doppler: workspace %zu < %zu bytes", ws ? ws->bytes : 0,
+                             need);
+    } else {
+        ST_CHECK(check_cf32(out, "asarf_range_doppler(out)"));
+        if (out->x_stride < src->x_size || out->y_stride < src->y_stride)
+            return set_error(ASARF_ESHAPE, "asarf_range_doppler: output smaller than the padded source");
+    }
+    if ((size_t)src->x_size * sizeof(float2) > 0 && src->x_size > 8192)
+        return set_error(ASARF_EUNSUPPORTED, "asarf_range_doppler: range size %d", src->x_size);
+    ST_CHECK(ctx_init(c));
+    if ((size_t)src->x_size * sizeof(float2) > c->max_lds)
+        return set_error(ASARF_EUNSUPPORTED, "asarf_range_doppler: row of %d samples exceeds LDS", src->x_size);
+    ColSplit cs;
+    ST_CHECK(get_col_split(c, src->y_stride, &cs));
+    if (adopt) ST_CHECK(asarf_image_adopt(out, src->x_size, src->y_size, src->x_stride, src->y_stride, ASARF_CF32, ws));
+    const int nrv = src->x_size, na = src->y_size, nap = src->y_stride;
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(ColConst) * (size_t)nrv + 64, &scratch));
+    auto* col = static_cast<ColConst*>(scratch);
+    auto* d_dc = reinterpret_cast<double*>(static_cast<char*>(scratch) + sizeof(ColConst) * (size_t)nrv);
+    HIP_CHECK(hipMemcpyAsync(d_dc, p->dc_poly, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
+    ColConstArgs ca{};
+    ca.col = col;
+    ca.nrv = nrv;
+    ca.dc_poly = d_dc;
+    for (int i = 0; i < 3; ++i) ca.vr_poly[i] = p->vr_poly[i];
+    ca.wavelength = p->wavelength;
+    ca.range_spacing = p->range_spacing;
+    ca.r_first = p->slant_range_first;
+    ca.prf = p->prf;
+    ca.nbins = nap;
+    HIP_CHECK(launch_col_const(ca, c->stream));
+    auto* s = static_cast<float2*>(src->data);
+    auto* o = static_cast<float2*>(out->data);
+    const long so = out->x_stride;
+    ST_CHECK(col_forward_pass(c, cs, 0, s, src->x_stride, na, o, so, nrv));
+    ST_CHECK(col_forward_pass(c, cs, 1, s, src->x_stride, na, o, so, nrv));
+    RcmcArgs ra{};
+    ra.img = o;
+    ra.stride = so;
+    ra.nrv = nrv;
+    ra.nbins = nap;
+    ra.n1 = cs.n1;
+    ra.n2 = cs.n2;
+    ra.col = col;
+    ra.prf = p->prf;
+    ra.beta = p->az_bandwidth_fraction;
+    ra.mode = 0;
+    HIP_CHECK(launch_rcmc_mf(ra, c->stream));
+    ST_CHECK(col_inverse_pass(c, cs, 0, o, so, nrv, na, 1.0f / (float)nap));
+    ST_CHECK(col_inverse_pass(c, cs, 1, o, so, nrv, na, 1.0f / (float)nap));
+    HIP_CHECK(launch_zero_cols(o, so, out->y_stride, nrv, out->x_stride, c->stream));
+    out->x_size = nrv;
+    out->y_size = na;
+    if (adopt) {
+        // the source buffer becomes the new workspace (range_doppler_algorithm.cu:378)
+        ws->data = src->data;
+        ws->bytes = asarf_image_bytes(src);
+        *src = asarf_image{nullptr, 0, 0, 0, 0, ASARF_CF32};
+    }
+    return clear_error();
+}
+
+extern "C" int asarf_az_multilook(asarf_ctx* c, const asarf_image* slc, double az_spacing, asarf_workspace* out,
+                                  int* out_rows, double* new_spacing) {
+    if (!c || !out || !out_rows) return set_error(ASARF_EINVAL, "asarf_az_multilook: null argument");
+    ST_CHECK(check_cf32(slc, "asarf_az_multilook"));
+    if (!(az_spacing > 0)) return set_error(ASARF_EINVAL, "asarf_az_multilook: azimuth spacing must be > 0");
+    const float ratio = (float)(12.5 / az_spacing);
+    const int rows = (int)std::round((float)slc->y_size / ratio);
+    const size_t need = (size_t)rows * slc->x_size * sizeof(float);
+    if (!out->data || out->bytes < need)
+        return set_error(ASARF_EWORKSPACE, "asarf_az_multilook: output %zu < %zu bytes", out->bytes, need);
+    ST_CHECK(ctx_init(c));
+    MultilookArgs a{};
+    a.slc = static_cast<const float2*>(slc->data);
+    a.stride = slc->x_stride;
+    a.in_rows = slc->y_size;
+    a.nr = slc->x_size;
+    a.out_rows = rows;
+    a.ratio = ratio;
+    a.out = static_cast<float*>(out->data);
+    HIP_CHECK(launch_multilook(a, c->stream));
+    *out_rows = rows;
+    if (new_spacing) *new_spacing = az_spacing * (double)ratio;
+    return clear_error();
+}
+
+extern "C" int asarf_range_spreading_loss(asarf_ctx* c, float* img, int nr, int rows, double r_first, double dr,
+                                          double r_ref) {
+    if (!c || !img) return set_error(ASARF_EINVAL, "asarf_range_spreading_loss: null argument");
+    if (nr <= 0 || rows < 0 || !(r_first > 0) || !(dr > 0) || !(r_ref > 0))
+        return set_error(ASARF_EINVAL, "asarf_range_spreading_loss: bad parameters");
+    ST_CHECK(ctx_init(c));
+    std::vector<float> tab(nr);
+    for (int i = 0; i < nr; ++i) {
+        const double r = r_first + (double)i * dr;
+        tab[i] = (float)(1.0 / std::sqrt(std::pow(r_ref / r, 3.0)));
+    }
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(float) * (size_t)nr, &scratch));
+    HIP_CHECK(hipMemcpyAsync(scratch, tab.data(), sizeof(float) * (size_t)nr, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(launch_rsl(img, nr, rows, static_cast<float*>(scratch), c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // tab (host) must outlive the copy
+    return clear_error();
+}
+
+extern "C" int asarf_srgr(asarf_ctx* c, const float* in, int in_nr, int rows, const double* grsr5, double dr, double gs,
+                          int out_nr, float* out) {
+    if (!c || !in || !out || !grsr5) return set_error(ASARF_EINVAL, "asarf_srgr: null argument");
+    if (in_nr <= 1 || rows < 0 || out_nr <= 0 || !(dr > 0) || !(gs > 0))
+        return set_error(ASARF_EINVAL, "asarf_srgr: bad parameters");
+    ST_CHECK(ctx_init(c));
+    SrgrArgs a{};
+    a.in = in;
+    a.in_nr = in_nr;
+    a.rows = rows;
+    for (int i = 0; i < 5; ++i) a.grsr[i] = grsr5[i];
+    a.dr = dr;
+    a.gs = gs;
+    a.out_nr = out_nr;
+    a.out = out;
+    HIP_CHECK(launch_srgr(a, c->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_fft_c2c(asarf_ctx* c, void* data, int n, int batch, long dist, int layout, int direction,
+                             float scale) {
+    if (!c || !data) return set_error(ASARF_EINVAL, "asarf_fft_c2c: null argument");
+    if (n < 2 || batch <= 0) return set_error(ASARF_EINVAL, "asarf_fft_c2c: n=%d batch=%d", n, batch);
+    if (direction != ASARF_FORWARD && direction != ASARF_INVERSE)
+        return set_error(ASARF_EINVAL, "asarf_fft_c2c: direction %d", direction);
+    const bool inv = direction == ASARF_INVERSE;
+    auto* d = static_cast<float2*>(data);
+    if (layout == ASARF_FFT_ROWS) {
+        if (dist < n) return set_error(ASARF_ESHAPE, "asarf_fft_c2c: dist %ld < n %d", dist, n);
+        ST_CHECK(ctx_init(c));
+        int st = ASARF_OK;
+        const FftPlanHost* p = get_plan(c, n, kRowThreads, &st);
+        if (!p) return st;
+        RowFftArgs a{};
+        a.in = d;
+        a.in_stride = dist;
+        a.out = d;
+        a.out_stride = dist;
+        a.rows = batch;
+        a.scale = scale;
+        a.plan = p->dev();
+        HIP_CHECK(launch_row_fft(a, inv, c->stream));
+        return clear_error();
+    }
+    if (layout != ASARF_FFT_COLUMNS) return set_error(ASARF_EINVAL, "asarf_fft_c2c: layout %d", layout);
+    if (dist < batch) return set_error(ASARF_ESHAPE, "asarf_fft_c2c: row stride %ld < columns %d", dist, batch);
+    ST_CHECK(ctx_init(c));
+    ColSplit cs;
+    ST_CHECK(get_col_split(c, n, &cs));
+    if (cs.n2 == 1) {
+        HIP_CHECK(launch_col_fft(col_args(d, dist, d, dist, batch, 1, 0, 1, n, 0, 1, n, nullptr, scale, cs.p1), inv,
+                                 c->stream));
+        return clear_error();
+    }
+    // natural order through a temporary: forward A -> tmp, B -> data (bin k1 + n1 k2 at its own row)
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(float2) * (size_t)n * batch, &scratch));
+    auto* t = static_cast<float2*>(scratch);
+    const long ts = batch;
+    if (!inv) {
+        HIP_CHECK(launch_col_fft(col_args(d, dist, t, ts, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, cs.full->d_tw, 1.f,
+                                          cs.p1),
+                                 false, c->stream));
+        HIP_CHECK(launch_col_fft(col_args(t, ts, d, dist, batch, cs.n1, cs.n2, 1, n, 1, cs.n1, n, nullptr, scale,
+                                          cs.p2),
+                                 false, c->stream));
+    } else {
+        HIP_CHECK(launch_col_fft(col_args(d, dist, t, ts, batch, cs.n1, 1, cs.n1, n, cs.n2, 1, n, cs.full->d_tw, 1.f,
+                                          cs.p2),
+                                 true, c->stream));
+        HIP_CHECK(launch_col_fft(col_args(t, ts, d, dist, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, nullptr, scale,
+                                          cs.p1),
+                                 true, c->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));  // the scratch may be reused by the next call
+    return clear_error();
+}

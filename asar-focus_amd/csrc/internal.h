@@ -1,0 +1,73 @@
+// Internal declarations shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "asarf.h"
+#include "kernels.h"
+
+namespace asarf {
+
+int set_error(int status, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int clear_error();
+
+int aperture_pixels_max(double wavelength, double prf, double beta, const double* vr_poly, double r_first, double dr,
+                        int range_size, int chirp_samples);
+
+// Radix plan + device twiddle table for one transform length.
+struct FftPlanHost {
+    int n = 0;
+    std::vector<int> radix;
+    float2* d_tw = nullptr;
+    FftPlanDev dev() const {
+        FftPlanDev d{};
+        d.n = n;
+        d.npass = (int)radix.size();
+        d.radices = 0;
+        for (size_t i = 0; i < radix.size() && i < (size_t)kMaxPasses; ++i)
+            d.radices |= (unsigned long long)radix[i] << (5 * i);
+        d.tw = d_tw;
+        return d;
+    }
+};
+
+// Four-step split of a column transform of length n = n1 * n2 (n2 == 1: single pass).
+struct ColSplit {
+    int n = 0, n1 = 0, n2 = 0;
+    const FftPlanHost* p1 = nullptr;
+    const FftPlanHost* p2 = nullptr;
+    const FftPlanHost* full = nullptr;  // twiddle table of length n (inter-pass twiddle)
+};
+
+}  // namespace asarf
+
+struct asarf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool initialized = false;
+    size_t max_lds = 65536;
+    // plan caches keyed by (length, lanes per transform)
+    std::map<std::pair<int, int>, std::unique_ptr<asarf::FftPlanHost>> plans;
+    std::map<int, float2*> tw_tables;
+    // grow-only scratch
+    void* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void* h_pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+namespace asarf {
+int ctx_init(asarf_ctx* ctx);
+int ctx_scratch(asarf_ctx* ctx, size_t bytes, void** out);
+int ctx_pinned(asarf_ctx* ctx, size_t bytes, void** out);
+const FftPlanHost* get_plan(asarf_ctx* ctx, int n, int lanes, int* status);
+float2* get_twiddles(asarf_ctx* ctx, int n, int* status);
+int get_col_split(asarf_ctx* ctx, int n, ColSplit* out);
+bool radix_plan(int n, int lanes, std::vector<int>& out);
+}  // namespace asarf

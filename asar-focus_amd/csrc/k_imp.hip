@@ -1,0 +1,106 @@
+// IMP tail: azimuth multilook (sar/az_multilook.cu:25-80) with the range-spreading-loss factor fused into its
+// store (sar/range_spreading_loss.cu:29-40), the standalone RSL pass, and slant-to-ground resampling
+// (sar/srgr.cu:49-71, polynomial evaluated in FP64).
+#include "kernels.h"
+#include "poly.h"
+
+namespace asarf {
+
+__global__ void k_multilook(MultilookArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int out_rows = a.out_rows_dev ? *a.out_rows_dev : a.out_rows;
+    if (x >= a.nr || y >= out_rows || y >= a.out_rows) return;
+    const int row0 = a.row0_dev ? *a.row0_dev : a.row0;
+    const int in_rows = a.in_rows_dev ? *a.in_rows_dev : a.in_rows;
+    // resampling position in float, exactly as the reference computes it
+    const float in_idx = (float)y * a.ratio;
+    const float mu = in_idx - (float)(int)in_idx;
+    int s0 = (int)(in_idx - 2.0f), e0 = (int)(in_idx + 2.0f);
+    int s1 = (int)(in_idx - 1.0f), e1 = (int)(in_idx + 3.0f);
+    s0 = max(s0, 0);
+    e0 = min(e0, in_rows);
+    s1 = max(s1, 0);
+    e1 = min(e1, in_rows);
+    const float2* col = a.slc + (long)row0 * a.stride + a.col0 + x;
+    float sum0 = 0.f, sum1 = 0.f;
+    int c0 = 0, c1 = 0;
+    for (int i = s0; i < e0; ++i) {
+        const float2 v = col[(long)i * a.stride];
+        sum0 += v.x * v.x + v.y * v.y;
+        ++c0;
+    }
+    for (int i = s1; i < e1; ++i) {
+        const float2 v = col[(long)i * a.stride];
+        sum1 += v.x * v.x + v.y * v.y;
+        ++c1;
+    }
+    if (c0) sum0 /= (float)c0;
+    if (c1) sum1 /= (float)c1;
+    float val = (c0 && c1) ? sum0 + (sum1 - sum0) * mu : 0.f;
+    if (a.rsl) val *= a.rsl[x];
+    a.out[(long)y * a.nr + x] = val;
+}
+
+hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s) {
+    if (a.out_rows <= 0 || a.nr <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_multilook, dim3((a.nr + 255) / 256, a.out_rows), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void k_rsl(float* img, int nr, int rows, const float* tab) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= nr || y >= rows) return;
+    img[(long)y * nr + x] *= tab[x];
+}
+
+hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_t s) {
+    if (rows <= 0 || nr <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rsl, dim3((nr + 255) / 256, rows), dim3(256), 0, s, img, nr, rows, tab);
+    return hipGetLastError();
+}
+
+__global__ void k_srgr(SrgrArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int rows = a.rows_dev ? *a.rows_dev : a.rows;
+    if (x >= a.out_nr || y >= rows || y >= a.rows) return;
+    const double rd = polyval(a.grsr, 5, (double)x * a.gs);
+    const double ri = rd / a.dr;
+    int r0 = (int)ri;
+    float fx = (float)(ri - (double)r0);
+    if (r0 < 0) {
+        r0 = 0;
+        fx = 0.f;
+    }
+    if (r0 > a.in_nr - 1) {
+        r0 = a.in_nr - 1;
+        fx = 0.f;
+    }
+    const int r1 = min(r0 + 1, a.in_nr - 1);
+    const float* row = a.in + (long)y * a.in_nr;
+    const float v0 = row[r0], v1 = row[r1];
+    a.out[(long)y * a.out_nr + x] = v0 + fx * (v1 - v0);
+}
+
+hipError_t launch_srgr(const SrgrArgs& a, hipStream_t s) {
+    if (a.rows <= 0 || a.out_nr <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_srgr, dim3((a.out_nr + 255) / 256, a.rows), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void k_zero_cols(float2* img, long stride, int rows, int c0, int c1) {
+    const int x = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= c1 || y >= rows) return;
+    img[(long)y * stride + x] = make_float2(0.f, 0.f);
+}
+
+hipError_t launch_zero_cols(float2* img, long stride, int rows, int c0, int c1, hipStream_t s) {
+    if (c1 <= c0 || rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_cols, dim3((c1 - c0 + 255) / 256, rows), dim3(256), 0, s, img, stride, rows, c0, c1);
+    return hipGetLastError();
+}
+
+}  // namespace asarf

@@ -1,0 +1,138 @@
+// Range-Doppler domain kernels: per-column constants and the fused 16-tap sinc RCMC + azimuth matched filter.
+//
+// RCMC follows sar/range_doppler_algorithm.cu:107-195: shift = lambda^2 f^2 R0 / (8 Vr^2 dr), integer part by
+// round-half-away, 16-tap Blackman-windowed sinc normalised by the sum of its weights, zero when a tap leaves
+// [0, Nrv). The weights use sinc(d + 8 - i) = (-1)^i sin(pi d) / (pi (d + 8 - i)); the common factor
+// sin(pi d)/pi cancels in the normalisation, so no transcendental is evaluated per tap.
+// The matched filter follows range_doppler_algorithm.cu:197-309 with FP64 geometry and phase (reduced to turns
+// before the single-precision sincospi).
+#include "kernels.h"
+#include "poly.h"
+
+namespace asarf {
+
+// numpy.blackman(16) rounded to float (the table at sar/range_doppler_algorithm.cu:49-52 holds the same values).
+__constant__ float c_window[16];
+static bool g_window_ready = false;
+
+static void ensure_window(hipStream_t s) {
+    if (g_window_ready) return;
+    float w[16];
+    for (int i = 0; i < 16; ++i) {
+        const double a = 2.0 * kPi * (double)i / 15.0;
+        w[i] = (float)(0.42 - 0.5 * std::cos(a) + 0.08 * std::cos(2.0 * a));
+    }
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_window), w, sizeof(w), 0, hipMemcpyHostToDevice);
+    g_window_ready = true;
+}
+
+__global__ void k_col_const(ColConstArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.nrv) return;
+    const double step = a.prf / (double)a.nbins;
+    const double dc = polyval(a.dc_poly, 3, (double)x);
+    const double r0 = a.r_first + (double)x * a.range_spacing;
+    const double vr = polyval(a.vr_poly, 3, (double)x);
+    ColConst c;
+    c.dc = dc;
+    c.dc_off = round(dc / step) * step;
+    c.shift_k = a.wavelength * a.wavelength * r0 / (8.0 * vr * vr * a.range_spacing);
+    c.inv_ka = a.wavelength * r0 / (2.0 * vr * vr);
+    a.col[x] = c;
+}
+
+hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_col_const, dim3((a.nrv + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ double alias_about_dc(double f, double dc, double prf) {
+    const double half = 0.5 * prf;
+    if (dc < 0.0) {
+        if (f > half + dc) f -= prf;
+    } else if (dc > 0.0) {
+        if (f < -half + dc) f += prf;
+    }
+    return f;
+}
+
+// One workgroup per frequency row (in place). The row is staged in LDS so every output can read its 16 taps.
+__global__ __launch_bounds__(kThreads) void k_rcmc_mf(RcmcArgs a) {
+    extern __shared__ float2 row[];
+    const int q = blockIdx.x;
+    float2* img = a.img + (long)q * a.stride;
+    const int nrv = a.nrv;
+    for (int x = threadIdx.x; x < nrv; x += kThreads) row[x] = img[x];
+    __syncthreads();
+    const int k = (a.n2 == 1) ? q : (q / a.n2 + a.n1 * (q % a.n2));
+    const double step = a.prf / (double)a.nbins;
+    const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
+    const double half = 0.5 * a.prf;
+    float w[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
+    for (int x = threadIdx.x; x < nrv; x += kThreads) {
+        const ColConst cc = a.col[x];
+        const double fn = alias_about_dc(fb, cc.dc, a.prf);
+        float2 val;
+        if (a.mode != 2) {
+            const double shift = cc.shift_k * fn * fn;
+            const double p = round(shift);
+            const int cx = x + (int)p;
+            if (cx - 8 < 0 || cx + 8 >= nrv) {
+                val = make_float2(0.f, 0.f);
+            } else {
+                const float d = (float)(shift - p);
+                if (d == 0.0f) {
+                    val = row[cx];
+                } else {
+                    float2 acc = make_float2(0.f, 0.f);
+                    float den = 0.f;
+                    const float2* tap = row + cx - 8;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const float wi = w[i] / (d + (float)(8 - i));
+                        const float2 t = tap[i];
+                        acc.x += wi * t.x;
+                        acc.y += wi * t.y;
+                        den += wi;
+                    }
+                    val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x / den, acc.y / den);
+                }
+            }
+        } else {
+            val = row[x];
+        }
+        if (a.mode != 1) {
+            const double hi = half * a.beta + cc.dc_off;
+            const double lo = -half * a.beta + cc.dc_off;
+            if (fn < lo || fn > hi) {
+                val = make_float2(0.f, 0.f);
+            } else {
+                double fc = fb - cc.dc_off;
+                if (fc > half)
+                    fc -= a.prf;
+                else if (fc < -half)
+                    fc += a.prf;
+                // phase / (2 pi) = inv_ka * (fc^2 / 2 + dc_off * f_bin)
+                const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
+                const float fr = (float)(turns - rint(turns));
+                float sn, cs;
+                sincospif(2.0f * fr, &sn, &cs);
+                // H = exp(-i phase)
+                val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
+            }
+        }
+        img[x] = val;
+    }
+}
+
+hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s) {
+    ensure_window(s);
+    const size_t lds = sizeof(float2) * (size_t)a.nrv;
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rcmc_mf, dim3(a.nbins), dim3(kThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace asarf

@@ -1,0 +1,330 @@
+// I/Q statistics, I/Q correction and Doppler-centroid (ACCC) kernels.
+//
+// Two families:
+//  * cf32 image kernels (the per-stage entry points): NaN-aware FP64 moments of the logical region and of the
+//    padding columns, the affine correction applied over the padded width, and direct lag-1 products per range
+//    block for the Doppler estimate. They follow sar/iq_correction.cu:140-273, cuda_util/device_padded_image.cu
+//    :156-257,322-414 and sar/fractional_doppler_centroid.cu:33-96.
+//  * raw-byte kernels (the fused plan): one pass over the uint8/int8 samples collects, per Doppler block, exact
+//    integer sums of I, Q, I^2, Q^2, IQ and of the lag-1 cross products between consecutive lines. The I/Q
+//    correction is affine, so the corrected lag-1 sum of every block is a bilinear form in these sums; a single
+//    FP64 finalisation thread turns them into the DC bias, gain, phase correction, block Doppler estimates and the
+//    degree-2 Doppler polynomial without a second read of the raw data.
+#include "kernels.h"
+#include "poly.h"
+
+namespace asarf {
+
+// ---------------------------------------------------------------------------------- cf32 moments ----
+// partials[wg][12]: logical {n, SI, SQ, SII, SQQ, SIQ}, padding {n, SI, SQ, SII, SQQ, SIQ}
+__global__ __launch_bounds__(kThreads) void k_moments_cf32(const float2* img, long stride, int xs, int ys,
+                                                          int xstr, double* partials) {
+    double acc[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) acc[i] = 0.0;
+    for (int y = blockIdx.x; y < ys; y += gridDim.x) {
+        const float2* row = img + (long)y * stride;
+        for (int x = threadIdx.x; x < xstr; x += kThreads) {
+            const float2 v = row[x];
+            if (isnan(v.x) || isnan(v.y)) continue;
+            const int o = (x < xs) ? 0 : 6;
+            const double i_ = v.x, q_ = v.y;
+            acc[o + 0] += 1.0;
+            acc[o + 1] += i_;
+            acc[o + 2] += q_;
+            acc[o + 3] += i_ * i_;
+            acc[o + 4] += q_ * q_;
+            acc[o + 5] += i_ * q_;
+        }
+    }
+    __shared__ double sh[12][kThreads];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) sh[i][threadIdx.x] = acc[i];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int i = 0; i < 12; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) partials[(long)blockIdx.x * 12 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+hipError_t launch_moments_cf32(const float2* img, long stride, int xs, int ys, int xstr, double* partials, int nwg,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_moments_cf32, dim3(nwg), dim3(kThreads), 0, s, img, stride, xs, ys, xstr, partials);
+    return hipGetLastError();
+}
+
+// Apply the correction over x < width for rows < ys; NaN -> 0 component-wise over the whole padded image.
+__global__ void k_iq_apply_cf32(float2* img, long stride, int xs, int ys, int xstr, int ystr, int width, IqCoef c) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= xstr || y >= ystr) return;
+    float2 v = img[(long)y * stride + x];
+    if (y < ys && x < width) {
+        const float ic = (v.x - c.dc_i) * c.inv_gain;
+        const float qc = (v.y - c.dc_q - ic * c.sin_corr) * c.inv_cos;
+        v = make_float2(ic, qc);
+    }
+    if (isnan(v.x)) v.x = 0.f;
+    if (isnan(v.y)) v.y = 0.f;
+    img[(long)y * stride + x] = v;
+}
+
+hipError_t launch_iq_apply_cf32(float2* img, long stride, int xs, int ys, int xstr, int ystr, int width, IqCoef c,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_iq_apply_cf32, dim3((xstr + 255) / 256, ystr), dim3(256), 0, s, img, stride, xs, ys, xstr,
+                       ystr, width, c);
+    return hipGetLastError();
+}
+
+// Lag-1 products per Doppler block: grid (nblocks, nstrips); partials[(strip*nblocks + b)*2 + {re, im}].
+__global__ __launch_bounds__(kThreads) void k_dc_cf32(const float2* img, long stride, int ys, int step, int nblocks,
+                                                     int xstr, int nstrips, double* partials) {
+    const int b = blockIdx.x;
+    const int strip = blockIdx.y;
+    const int rows_per = (ys + nstrips - 1) / nstrips;
+    const int y0 = max(1, strip * rows_per), y1 = min(ys, (strip + 1) * rows_per);
+    const int xb = b * step;
+    const int xe = min(xb + step, xstr);
+    double re = 0.0, im = 0.0;
+    for (int x = xb + threadIdx.x; x < xe; x += kThreads) {
+        if (y0 >= y1) break;
+        float2 prev = img[(long)(y0 - 1) * stride + x];
+        for (int y = y0; y < y1; ++y) {
+            const float2 cur = img[(long)y * stride + x];
+            re += (double)prev.x * cur.x + (double)prev.y * cur.y;
+            im += (double)prev.x * cur.y - (double)prev.y * cur.x;
+            prev = cur;
+        }
+    }
+    __shared__ double sh[2][kThreads];
+    sh[0][threadIdx.x] = re;
+    sh[1][threadIdx.x] = im;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[((long)strip * nblocks + b) * 2 + 0] = sh[0][0];
+        partials[((long)strip * nblocks + b) * 2 + 1] = sh[1][0];
+    }
+}
+
+hipError_t launch_dc_cf32(const float2* img, long stride, int ys, int x_begin0, int step, int nblocks, int xstr,
+                          int nstrips, double* partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_dc_cf32, dim3(nblocks, nstrips), dim3(kThreads), 0, s, img, stride, ys, step, nblocks, xstr,
+                       nstrips, partials);
+    return hipGetLastError();
+}
+
+// totals[v] = sum_p partials[p * nvals + v] (fixed order, deterministic).
+__global__ __launch_bounds__(kThreads) void k_reduce_f64(const double* partials, int nparts, int nvals,
+                                                        double* totals) {
+    const int v = blockIdx.x;
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += kThreads) acc += partials[(long)p * nvals + v];
+    __shared__ double sh[kThreads];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[v] = sh[0];
+}
+
+hipError_t launch_reduce_f64(const double* partials, int nparts, int nvals, double* totals, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_f64, dim3(nvals), dim3(kThreads), 0, s, partials, nparts, nvals, totals);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kThreads) void k_reduce_i64(const long long* partials, int nparts, int nvals,
+                                                        long long* totals) {
+    const int v = blockIdx.x;
+    long long acc = 0;
+    for (int p = threadIdx.x; p < nparts; p += kThreads) acc += partials[(long)p * nvals + v];
+    __shared__ long long sh[kThreads];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[v] = sh[0];
+}
+
+hipError_t launch_reduce_i64(const long long* partials, int nparts, int nvals, long long* totals, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_i64, dim3(nvals), dim3(kThreads), 0, s, partials, nparts, nvals, totals);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------- raw moments ----
+// Moment slots: 0 SI, 1 SQ, 2 SII, 3 SQQ, 4 SIQ, 5 L_II, 6 L_QQ, 7 L_IQ (I_prev Q_cur), 8 L_QI (Q_prev I_cur),
+// 9 L_I1 (sum I_prev), 10 L_Q1, 11 L_I2 (sum I_cur), 12 L_Q2, 13 N, 14 M (lag pairs).
+template <typename P>
+__global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
+    const int tile = blockIdx.x;
+    const int strip = blockIdx.y;
+    const int x0 = tile * a.step;
+    const int x1 = (tile == a.ntiles - 1) ? a.nr : min(x0 + a.step, a.nr);
+    const int y0 = strip * kStripRows;
+    const int y1 = min(y0 + kStripRows, a.na);
+    const P* raw = static_cast<const P*>(a.raw);
+    int acc[13];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) acc[i] = 0;
+    int n = 0, m = 0;
+    for (int x = x0 + threadIdx.x; x < x1; x += kThreads) {
+        int pi = 0, pq = 0;
+        bool have_prev = false;
+        if (y0 > 0) {
+            const P p = raw[(long)(y0 - 1) * a.nr + x];
+            pi = p.x;
+            pq = p.y;
+            have_prev = true;
+        }
+        for (int y = y0; y < y1; ++y) {
+            const P c = raw[(long)y * a.nr + x];
+            const int ci = c.x, cq = c.y;
+            acc[0] += ci;
+            acc[1] += cq;
+            acc[2] += ci * ci;
+            acc[3] += cq * cq;
+            acc[4] += ci * cq;
+            ++n;
+            if (have_prev) {
+                acc[5] += pi * ci;
+                acc[6] += pq * cq;
+                acc[7] += pi * cq;
+                acc[8] += pq * ci;
+                acc[9] += pi;
+                acc[10] += pq;
+                acc[11] += ci;
+                acc[12] += cq;
+                ++m;
+            }
+            pi = ci;
+            pq = cq;
+            have_prev = true;
+        }
+    }
+    __shared__ long long sh[kRawMoments][kThreads];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) sh[i][threadIdx.x] = acc[i];
+    sh[13][threadIdx.x] = n;
+    sh[14][threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int i = 0; i < kRawMoments; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kRawMoments)
+        a.partials[((long)strip * a.ntiles + tile) * kRawMoments + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+hipError_t launch_raw_moments(const RawMomentsArgs& a, int fmt, hipStream_t s) {
+    const dim3 grid(a.ntiles, a.nstrips);
+    if (fmt == RAW_U8)
+        hipLaunchKernelGGL(k_raw_moments<uchar2>, grid, dim3(kThreads), 0, s, a);
+    else if (fmt == RAW_I8)
+        hipLaunchKernelGGL(k_raw_moments<char2>, grid, dim3(kThreads), 0, s, a);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// Single-thread FP64 finalisation of the fused statistics. See the derivation in the file header: with the
+// corrected sample u = a I + b, v = c I + d Q + e, the block lag sum is
+//   sum conj(s_prev) s_cur = sum(u1 u2 + v1 v2) + i sum(u1 v2 - v1 u2),
+// each term expanded over the raw lag moments. Padding columns of the last block (Q4) hold the corrected image
+// of raw 0 when the correction runs over the padded width (Q1), i.e. the constant (b, e).
+__global__ void k_finalize(FinalizeArgs f) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const long long* T = f.totals;
+    double s[kRawMoments];
+    for (int i = 0; i < kRawMoments; ++i) s[i] = 0.0;
+    for (int t = 0; t < f.ntiles; ++t)
+        for (int i = 0; i < 5; ++i) s[i] += (double)T[t * kRawMoments + i];
+    double nvalid = 0.0;
+    for (int t = 0; t < f.ntiles; ++t) nvalid += (double)T[t * kRawMoments + 13];
+    const double SI = s[0], SQ = s[1], SII = s[2], SQQ = s[3], SIQ = s[4];
+    const double ntot = f.n_total;
+    const double dci = SI / ntot, dcq = SQ / ntot;
+    DevResults* r = f.res;
+    r->dc_i = dci;
+    r->dc_q = dcq;
+    r->stddev_i = sqrt((SII - 2.0 * dci * SI + nvalid * dci * dci) / ntot);
+    r->stddev_q = sqrt((SQQ - 2.0 * dcq * SQ + nvalid * dcq * dcq) / ntot);
+    const bool q1 = (f.quirks & 1) != 0;
+    const double P = q1 ? (double)f.na * (double)(f.x_stride - f.nr) : 0.0;
+    const double sii = SII - 2.0 * dci * SI + nvalid * dci * dci + P * dci * dci;
+    const double sqq = SQQ - 2.0 * dcq * SQ + nvalid * dcq * dcq + P * dcq * dcq;
+    const double g = sqrt(sii / sqq);
+    const double siq = (SIQ - dcq * SI - dci * SQ + nvalid * dci * dcq + P * dci * dcq) / g;
+    const double delta = siq / (sii / (g * g));
+    const double cosc = (f.quirks & 2) ? (1.0 - delta * delta) : sqrt(1.0 - delta * delta);
+    r->iq_gain = g;
+    r->phase_error = asin(delta) * 180.0 / kPi;
+    IqCoef c;
+    c.dc_i = (float)dci;
+    c.dc_q = (float)dcq;
+    c.inv_gain = (float)(1.0 / g);
+    c.sin_corr = (float)delta;
+    c.inv_cos = (float)(1.0 / cosc);
+    c.identity = 0;
+    r->coef = c;
+    // affine coefficients in FP64
+    const double ca = 1.0 / g, cb = -dci / g;
+    const double cc = -delta / (g * cosc), cd = 1.0 / cosc, ce = (-dcq + delta * dci / g) / cosc;
+    double bx[kMaxDcBlocks], by[kMaxDcBlocks];
+    for (int blk = 0; blk < f.nblocks; ++blk) {
+        const long long* t = T + blk * kRawMoments;
+        const double LII = (double)t[5], LQQ = (double)t[6], LIQ = (double)t[7], LQI = (double)t[8];
+        const double I1 = (double)t[9], Q1 = (double)t[10], I2 = (double)t[11], Q2 = (double)t[12];
+        const double M = (double)t[14];
+        const double uu = ca * ca * LII + ca * cb * (I1 + I2) + cb * cb * M;
+        const double vv = cc * cc * LII + cc * cd * (LIQ + LQI) + cd * cd * LQQ + cc * ce * (I1 + I2) +
+                          cd * ce * (Q1 + Q2) + ce * ce * M;
+        const double uv = ca * cc * LII + ca * cd * LIQ + ca * ce * I1 + cb * cc * I2 + cb * cd * Q2 + cb * ce * M;
+        const double vu = cc * ca * LII + cc * cb * I1 + cd * ca * LQI + cd * cb * Q1 + ce * ca * I2 + ce * cb * M;
+        double re = uu + vv, im = uv - vu;
+        // padding columns of this block (only the last one can reach past nr)
+        const int xb = blk * f.step, xe = xb + f.step;
+        const int pad0 = max(xb, f.nr), pad1 = min(xe, f.x_stride);
+        if (q1 && pad1 > pad0 && f.na > 1) re += (double)(f.na - 1) * (double)(pad1 - pad0) * (cb * cb + ce * ce);
+        const double fr = f.prf * atan2(im, re) / (2.0 * kPi);
+        bx[blk] = (double)((xb + xe) / 2);
+        by[blk] = fr;
+        r->block_freq[blk] = fr;
+        r->block_x[blk] = bx[blk];
+    }
+    r->nblocks = f.nblocks;
+    double poly[3] = {0.0, 0.0, 0.0};
+    if (polyfit_qr(bx, by, f.nblocks, 2, poly) != 0) poly[0] = poly[1] = poly[2] = 0.0;
+    r->dc_poly[0] = poly[0];
+    r->dc_poly[1] = poly[1];
+    r->dc_poly[2] = poly[2];
+    // results window: lines cut at the start when the constant Doppler term is positive, else at the end
+    const bool pos = poly[2] > 0.0;
+    r->crop_front = pos ? f.crop_dc_pos : f.crop_dc_neg;
+    const int in_rows = pos ? f.imp_in_rows_pos : f.imp_in_rows_neg;
+    r->imp_rows = (f.ml_ratio > 0.f) ? (int)roundf((float)in_rows / f.ml_ratio) : 0;
+    r->pad = in_rows;
+}
+
+hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace asarf

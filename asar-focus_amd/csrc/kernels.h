@@ -1,0 +1,186 @@
+// Kernel argument blocks and host-side launchers (internal to libasarf.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "fft_core.h"
+
+namespace asarf {
+
+constexpr int kThreads = 256;      // generic kernels and column transforms
+constexpr int kRowThreads = 512;   // row transforms: one row per workgroup
+constexpr int kColTile = 16;   // columns per workgroup in column transforms (16 x 8 B = one 128 B line)
+constexpr int kMaxE = 16;      // complex elements per lane in a Stockham pass
+constexpr int kStripRows = 256; // rows per workgroup in the raw statistics pass
+constexpr int kRawMoments = 15; // values per (strip, tile) in the raw statistics pass
+constexpr int kMaxDcBlocks = 64;
+
+enum RawFormat { RAW_CF32 = 0, RAW_U8 = 1, RAW_I8 = 2 };
+
+// Affine I/Q correction: I' = (I - dc_i) * inv_gain, Q' = (Q - dc_q - I' * sin_corr) * inv_cos.
+struct IqCoef {
+    float dc_i, dc_q, inv_gain, sin_corr, inv_cos;
+    int identity;
+};
+
+// Device-resident results of the fused statistics/Doppler stage.
+struct DevResults {
+    double dc_i, dc_q, iq_gain, phase_error, stddev_i, stddev_q;
+    double dc_poly[3];
+    double block_freq[kMaxDcBlocks];
+    double block_x[kMaxDcBlocks];
+    int nblocks;
+    int crop_front;  // lines removed at the start of the SLC for the IMP tail
+    int imp_rows;    // rows of the multilooked product
+    int pad;
+    IqCoef coef;
+};
+
+// Per range column constants of the range-Doppler stage (FP64).
+struct ColConst {
+    double dc;        // Doppler centroid at this column [Hz]
+    double dc_off;    // bin-quantised centroid [Hz]
+    double shift_k;   // RCMC shift in pixels per Hz^2: lambda^2 R0 / (8 Vr^2 dr)
+    double inv_ka;    // 1 / azimuth FM rate: lambda R0 / (2 Vr^2)
+};
+
+struct RowFftArgs {
+    const float2* in;
+    long in_stride;
+    float2* out;
+    long out_stride;
+    int rows;
+    float scale;
+    FftPlanDev plan;
+};
+
+struct RcArgs {
+    const void* in;     // raw I/Q bytes (row pitch nr_in samples) or cf32 rows (row pitch in_stride)
+    long in_stride;     // elements
+    int nr_in;          // valid input samples per row (raw); columns beyond are zero
+    float2* out;
+    long out_stride;
+    int rows;
+    int nrv;            // valid output columns
+    int zero_to;        // columns [nrv, zero_to) are written as zero
+    const IqCoef* coef; // device, nullptr = identity
+    const float2* hconj; // conj(FFT(chirp)) / n
+    FftPlanDev plan;
+};
+
+struct ColFftArgs {
+    const float2* in;
+    long in_stride;
+    float2* out;
+    long out_stride;
+    int ncols;
+    int ntiles;
+    int groups;            // number of sub-transforms per column
+    long in_g, in_m;       // input row of element m of group g: g*in_g + m*in_m
+    int in_rows;           // rows >= in_rows read as zero
+    long out_g, out_m;     // output row of element k of group g: g*out_g + k*out_m
+    int out_rows;          // rows >= out_rows are not written
+    const float2* tw_full; // post-twiddle W_full^{g*k} (conjugated for the inverse); nullptr = none
+    float scale;
+    FftPlanDev sub;
+};
+
+struct RcmcArgs {
+    float2* img;
+    long stride;
+    int nrv;
+    int nbins;        // azimuth FFT length
+    int n1, n2;       // storage row q holds frequency bin (q / n2) + n1 * (q % n2)
+    const ColConst* col;
+    double prf;
+    double beta;      // azimuth bandwidth fraction
+    int mode;         // 0 = RCMC + matched filter, 1 = RCMC only, 2 = matched filter only
+};
+
+struct RawMomentsArgs {
+    const void* raw;
+    int nr, na;
+    int step;         // DC block width
+    int ntiles;       // DC blocks (+1 remainder tile when blocks do not cover the row)
+    int nstrips;
+    long long* partials; // [strip][tile][kRawMoments]
+};
+
+struct FinalizeArgs {
+    const long long* totals; // [tile][kRawMoments]
+    int ntiles, nblocks, step, nr, na, x_stride;
+    double n_total;
+    int quirks;
+    double prf;
+    // crop decision for the IMP tail
+    int crop_dc_pos, crop_dc_neg;
+    int imp_in_rows_pos, imp_in_rows_neg;
+    float ml_ratio;
+    DevResults* res;
+};
+
+struct ColConstArgs {
+    ColConst* col;
+    int nrv;
+    const double* dc_poly;  // device
+    double vr_poly[3];
+    double wavelength, range_spacing, r_first, prf;
+    int nbins;
+};
+
+struct MultilookArgs {
+    const float2* slc;
+    long stride;
+    int row0;               // first input row (crop)
+    const int* row0_dev;    // device override of row0 (fused plan), nullptr = use row0
+    int col0;
+    int in_rows;            // input rows after the crop
+    const int* in_rows_dev; // device override
+    int nr;                 // range samples
+    int out_rows;           // rows to produce (upper bound when device override is used)
+    const int* out_rows_dev;
+    float ratio;
+    const float* rsl;       // optional per-column factor
+    float* out;             // out_rows x nr
+};
+
+struct SrgrArgs {
+    const float* in;
+    int in_nr;
+    int rows;
+    const int* rows_dev;
+    double grsr[5];
+    double dr, gs;
+    int out_nr;
+    float* out;
+};
+
+// ---- launchers (k_*.hip) ----
+hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s);
+hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s);
+hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s);
+hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
+hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s);
+hipError_t launch_moments_cf32(const float2* img, long stride, int xs, int ys, int xstr, double* partials,
+                               int nwg, hipStream_t s);
+hipError_t launch_iq_apply_cf32(float2* img, long stride, int xs, int ys, int xstr, int ystr, int width,
+                                IqCoef c, hipStream_t s);
+hipError_t launch_dc_cf32(const float2* img, long stride, int ys, int x_begin0, int step, int nblocks, int xstr,
+                          int nstrips, double* partials, hipStream_t s);
+hipError_t launch_reduce_f64(const double* partials, int nparts, int nvals, double* totals, hipStream_t s);
+hipError_t launch_raw_moments(const RawMomentsArgs& a, int raw_format, hipStream_t s);
+hipError_t launch_reduce_i64(const long long* partials, int nparts, int nvals, long long* totals, hipStream_t s);
+hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
+hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s);
+hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_t s);
+hipError_t launch_srgr(const SrgrArgs& a, hipStream_t s);
+hipError_t launch_zero_cols(float2* img, long stride, int rows, int c0, int c1, hipStream_t s);
+
+// Max dynamic LDS bytes per workgroup the launchers may request (set by the context at initialisation).
+void set_max_lds_bytes(size_t bytes);
+size_t max_lds_bytes();
+
+}  // namespace asarf

@@ -1,0 +1,408 @@
+// Fused focusing plan: the GPU stage of main.cc:233-327 as one stream-ordered sequence with no host round trip.
+//
+//   raw bytes --(1 pass: I/Q + Doppler moments)--> finalize (FP64, 1 lane) --> column constants
+//   raw bytes --(row kernel: correct, FFT, x conj(H)/n, IFFT)--> image rows < Na, Nrv columns
+//   image --(column pass A, B: azimuth FFT, permuted bins)--> RCMC + matched filter (row kernel, in place)
+//         --(column pass B', A': azimuth IFFT, 1/Nap, rows < Na)--> SLC
+//   [IMP] SLC --(crop, multilook x RSL)--> --(SRGR)--> detected ground-range image
+//
+// Device memory: one Nap x stride cf32 image (stride = Nrv rounded up to 16 for 128 B rows) plus small tables.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "asarf.h"
+#include "internal.h"
+#include "poly.h"
+
+using namespace asarf;
+
+namespace asarf {
+int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
+                     float2* img, long stride, int ncols);
+int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,
+                     float scale);
+}  // namespace asarf
+
+#define HIP_CHECK(expr)                                                                                   \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return set_error(e_ == hipErrorOutOfMemory ? ASARF_ENOMEM : ASARF_EHIP, "%s: %s (%s:%d)", #expr, \
+                             hipGetErrorString(e_), __FILE__, __LINE__);                                  \
+    } while (0)
+#define ST_CHECK(expr)                 \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != ASARF_OK) return s_; \
+    } while (0)
+
+struct asarf_plan {
+    asarf_ctx* ctx = nullptr;
+    asarf_focus_params p{};
+    int nrp = 0, nap = 0, nrv = 0;
+    long stride = 0;
+    int ntiles = 0, nblocks = 0, step = 0, nstrips = 0;
+    const FftPlanHost* rplan = nullptr;
+    ColSplit cs;
+    // device buffers
+    float2* img = nullptr;
+    float2* hconj = nullptr;
+    ColConst* col = nullptr;
+    long long* partials = nullptr;
+    long long* totals = nullptr;
+    DevResults* res = nullptr;
+    float* ml = nullptr;
+    float* rsl = nullptr;
+    size_t bytes = 0;
+    // results window
+    int crop_pos[2] = {0, 0}, crop_neg[2] = {0, 0};  // {front, back}
+    int near = 8, far = 8, nr_s = 0;
+    float ml_ratio = 0.f;
+    int imp_cap = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<const char*> ev_names;
+    std::vector<float> last_ms;
+};
+
+static int plan_alloc(asarf_plan* pl, void** p, size_t bytes) {
+    HIP_CHECK(hipMalloc(p, bytes));
+    pl->bytes += bytes;
+    return ASARF_OK;
+}
+
+static void plan_free(asarf_plan* pl) {
+    void* bufs[] = {pl->img, pl->hconj, pl->col, pl->partials, pl->totals, pl->res, pl->ml, pl->rsl};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (hipEvent_t e : pl->ev) (void)hipEventDestroy(e);
+}
+
+extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, const float* chirp_iq, asarf_plan** out) {
+    if (!c || !p || !chirp_iq || !out) return set_error(ASARF_EINVAL, "asarf_plan_create: null argument");
+    if (p->na < 2 || p->nr < 32 || p->nc <= 0 || p->nc >= p->nr)
+        return set_error(ASARF_ESHAPE, "asarf_plan_create: na=%d nr=%d nc=%d", p->na, p->nr, p->nc);
+    if (p->raw_format != ASARF_RAW_U8 && p->raw_format != ASARF_RAW_I8)
+        return set_error(ASARF_EINVAL, "asarf_plan_create: raw_format %d (u8 or i8)", p->raw_format);
+    if (!(p->prf > 0) || !(p->wavelength > 0) || !(p->range_spacing > 0) || !(p->slant_range_first > 0) ||
+        !(p->az_bandwidth_fraction > 0))
+        return set_error(ASARF_EINVAL, "asarf_plan_create: non-positive radar parameter");
+    if (p->quirks & ~ASARF_QUIRKS_REFERENCE) return set_error(ASARF_EINVAL, "asarf_plan_create: unknown quirk bits");
+    const int nrp = p->nrp > 0 ? p->nrp : asarf_fft_padding_size(p->nr + p->nc);
+    const int nap = p->nap > 0 ? p->nap : asarf_fft_padding_size(p->na);
+    if (nrp < p->nr + p->nc || nap < p->na)
+        return set_error(ASARF_ESHAPE, "asarf_plan_create: FFT sizes %d x %d too small", nrp, nap);
+    const int nrv = p->nr - p->nc;
+    const double vr0 = polyval(p->vr_poly, 3, 0.0), vr1 = polyval(p->vr_poly, 3, (double)nrv);
+    if (!(vr0 > 0) || !(vr1 > 0)) return set_error(ASARF_EINVAL, "asarf_plan_create: Vr must be > 0");
+    const int step = p->nr / 32;
+    int nblocks = 0;
+    for (int xb = 0; xb < p->nr; xb += step) {
+        if (!(p->quirks & ASARF_Q4_DC_EXTRA_BLOCK) && xb + step > p->nr) break;
+        ++nblocks;
+    }
+    if (nblocks > kMaxDcBlocks) return set_error(ASARF_EUNSUPPORTED, "asarf_plan_create: %d Doppler blocks", nblocks);
+    if (p->imp) {
+        if (!(p->azimuth_spacing > 0) || !(p->ground_spacing > 0) || p->gr_range_size <= 0 || !(p->reference_range > 0))
+            return set_error(ASARF_EINVAL, "asarf_plan_create: IMP parameters");
+        if (nrv <= 16) return set_error(ASARF_ESHAPE, "asarf_plan_create: too few range samples for IMP");
+    }
+    ST_CHECK(ctx_init(c));
+    if ((size_t)nrv * sizeof(float2) > c->max_lds)
+        return set_error(ASARF_EUNSUPPORTED, "asarf_plan_create: %d range samples exceed LDS", nrv);
+
+    auto* pl = new asarf_plan();
+    pl->ctx = c;
+    pl->p = *p;
+    pl->nrp = nrp;
+    pl->nap = nap;
+    pl->nrv = nrv;
+    pl->stride = (nrv + kColTile - 1) / kColTile * kColTile;
+    pl->step = step;
+    pl->nblocks = nblocks;
+    pl->ntiles = nblocks + ((nblocks * step < p->nr) ? 1 : 0);
+    pl->nstrips = (p->na + kStripRows - 1) / kStripRows;
+    auto fail = [&](int st) {
+        plan_free(pl);
+        delete pl;
+        return st;
+    };
+    int st = ASARF_OK;
+    pl->rplan = get_plan(c, nrp, kRowThreads, &st);
+    if (!pl->rplan) return fail(st);
+    st = get_col_split(c, nap, &pl->cs);
+    if (st != ASARF_OK) return fail(st);
+    if ((st = plan_alloc(pl, (void**)&pl->img, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
+    if ((st = plan_alloc(pl, (void**)&pl->hconj, sizeof(float2) * (size_t)nrp))) return fail(st);
+    if ((st = plan_alloc(pl, (void**)&pl->col, sizeof(ColConst) * (size_t)nrv))) return fail(st);
+    if ((st = plan_alloc(pl, (void**)&pl->partials,
+                         sizeof(long long) * (size_t)pl->nstrips * pl->ntiles * kRawMoments)))
+        return fail(st);
+    if ((st = plan_alloc(pl, (void**)&pl->totals, sizeof(long long) * (size_t)pl->ntiles * kRawMoments)))
+        return fail(st);
+    if ((st = plan_alloc(pl, (void**)&pl->res, sizeof(DevResults)))) return fail(st);
+
+    // chirp spectrum conj(FFT(chirp))/n: inverse-direction transform of the conjugated chirp
+    {
+        std::vector<float2> h(nrp);
+        for (int i = 0; i < nrp; ++i) h[i] = make_float2(chirp_iq[2 * i], -chirp_iq[2 * i + 1]);
+        if (hipMemcpy(pl->hconj, h.data(), sizeof(float2) * (size_t)nrp, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(ASARF_EHIP, "asarf_plan_create: chirp upload"));
+        RowFftArgs a{};
+        a.in = pl->hconj;
+        a.in_stride = nrp;
+        a.out = pl->hconj;
+        a.out_stride = nrp;
+        a.rows = 1;
+        a.scale = 1.0f / (float)nrp;
+        a.plan = pl->rplan->dev();
+        if (launch_row_fft(a, true, c->stream) != hipSuccess)
+            return fail(set_error(ASARF_EHIP, "asarf_plan_create: chirp transform"));
+    }
+    // results window (main_flow.cc:155-201) for both signs of the Doppler constant term
+    const int max_ap = aperture_pixels_max(p->wavelength, p->prf, p->az_bandwidth_fraction, p->vr_poly,
+                                           p->slant_range_first, p->range_spacing, p->nr, p->nc);
+    const int lb = std::max(0, p->lines_before), la = std::max(0, p->lines_after);
+    pl->crop_pos[0] = lb + std::max(0, max_ap - lb);
+    pl->crop_pos[1] = la;
+    pl->crop_neg[0] = lb;
+    pl->crop_neg[1] = la + std::max(0, max_ap - la);
+    pl->near = asarf_rcmc_window_size() / 2;
+    pl->far = asarf_rcmc_window_size() / 2;
+    pl->nr_s = nrv - pl->near - pl->far;
+    if (p->imp) {
+        pl->ml_ratio = (float)(12.5 / p->azimuth_spacing);
+        const int rows_pos = p->na - pl->crop_pos[0] - pl->crop_pos[1];
+        const int rows_neg = p->na - pl->crop_neg[0] - pl->crop_neg[1];
+        if (rows_pos <= 4 || rows_neg <= 4) return fail(set_error(ASARF_ESHAPE, "asarf_plan_create: crop leaves no lines"));
+        pl->imp_cap = std::max((int)std::round((float)rows_pos / pl->ml_ratio),
+                               (int)std::round((float)rows_neg / pl->ml_ratio));
+        if ((st = plan_alloc(pl, (void**)&pl->ml, sizeof(float) * (size_t)pl->imp_cap * pl->nr_s))) return fail(st);
+        if ((st = plan_alloc(pl, (void**)&pl->rsl, sizeof(float) * (size_t)pl->nr_s))) return fail(st);
+        // range spreading loss table over the cropped range axis (range_spreading_loss.cu:46-52)
+        std::vector<float> tab(pl->nr_s);
+        const double r0 = p->slant_range_first + (double)pl->near * p->range_spacing;
+        for (int i = 0; i < pl->nr_s; ++i) {
+            const double r = r0 + (double)i * p->range_spacing;
+            tab[i] = (float)(1.0 / std::sqrt(std::pow(p->reference_range / r, 3.0)));
+        }
+        if (hipMemcpy(pl->rsl, tab.data(), sizeof(float) * (size_t)pl->nr_s, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_error(ASARF_EHIP, "asarf_plan_create: RSL table upload"));
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(ASARF_EHIP, "asarf_plan_create: sync"));
+    *out = pl;
+    return clear_error();
+}
+
+extern "C" int asarf_plan_destroy(asarf_plan* pl) {
+    if (!pl) return set_error(ASARF_EINVAL, "asarf_plan_destroy: null");
+    (void)hipStreamSynchronize(pl->ctx->stream);
+    plan_free(pl);
+    delete pl;
+    return clear_error();
+}
+
+extern "C" size_t asarf_plan_device_bytes(const asarf_plan* pl) { return pl ? pl->bytes : 0; }
+
+extern "C" int asarf_plan_imp_capacity(const asarf_plan* pl, int* rows, int* cols) {
+    if (!pl || !rows || !cols) return set_error(ASARF_EINVAL, "asarf_plan_imp_capacity: null argument");
+    *rows = pl->imp_cap;
+    *cols = pl->p.imp ? pl->p.gr_range_size : 0;
+    return clear_error();
+}
+
+extern "C" int asarf_plan_set_timing(asarf_plan* pl, int enable) {
+    if (!pl) return set_error(ASARF_EINVAL, "asarf_plan_set_timing: null");
+    pl->timing = enable != 0;
+    return clear_error();
+}
+
+static int mark(asarf_plan* pl, const char* name) {
+    if (!pl->timing) return ASARF_OK;
+    const size_t i = pl->ev_names.size();
+    if (i >= pl->ev.size()) {
+        hipEvent_t e;
+        HIP_CHECK(hipEventCreate(&e));
+        pl->ev.push_back(e);
+    }
+    pl->ev_names.push_back(name);
+    HIP_CHECK(hipEventRecord(pl->ev[i], pl->ctx->stream));
+    return ASARF_OK;
+}
+
+extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out) {
+    if (!pl || !raw || !slc) return set_error(ASARF_EINVAL, "asarf_plan_execute: null argument");
+    if (pl->p.imp && !imp_out) return set_error(ASARF_EINVAL, "asarf_plan_execute: IMP output required");
+    asarf_ctx* c = pl->ctx;
+    const asarf_focus_params& p = pl->p;
+    hipStream_t s = c->stream;
+    pl->ev_names.clear();
+    ST_CHECK(mark(pl, "start"));
+    // 1. one pass over the raw bytes: I/Q and Doppler-block moments
+    RawMomentsArgs rm{};
+    rm.raw = raw;
+    rm.nr = p.nr;
+    rm.na = p.na;
+    rm.step = pl->step;
+    rm.ntiles = pl->ntiles;
+    rm.nstrips = pl->nstrips;
+    rm.partials = pl->partials;
+    HIP_CHECK(launch_raw_moments(rm, p.raw_format, s));
+    ST_CHECK(mark(pl, "raw_moments"));
+    HIP_CHECK(launch_reduce_i64(pl->partials, pl->nstrips, pl->ntiles * kRawMoments, pl->totals, s));
+    ST_CHECK(mark(pl, "reduce"));
+    FinalizeArgs fa{};
+    fa.totals = pl->totals;
+    fa.ntiles = pl->ntiles;
+    fa.nblocks = pl->nblocks;
+    fa.step = pl->step;
+    fa.nr = p.nr;
+    fa.na = p.na;
+    fa.x_stride = pl->nrp;
+    fa.n_total = p.n_total_samples ? (double)p.n_total_samples : (double)p.na * (double)p.nr;
+    fa.quirks = p.quirks;
+    fa.prf = p.prf;
+    fa.crop_dc_pos = pl->crop_pos[0];
+    fa.crop_dc_neg = pl->crop_neg[0];
+    fa.imp_in_rows_pos = p.na - pl->crop_pos[0] - pl->crop_pos[1];
+    fa.imp_in_rows_neg = p.na - pl->crop_neg[0] - pl->crop_neg[1];
+    fa.ml_ratio = p.imp ? pl->ml_ratio : 0.f;
+    fa.res = pl->res;
+    HIP_CHECK(launch_finalize(fa, s));
+    ST_CHECK(mark(pl, "finalize"));
+    ColConstArgs ca{};
+    ca.col = pl->col;
+    ca.nrv = pl->nrv;
+    ca.dc_poly = pl->res->dc_poly;
+    for (int i = 0; i < 3; ++i) ca.vr_poly[i] = p.vr_poly[i];
+    ca.wavelength = p.wavelength;
+    ca.range_spacing = p.range_spacing;
+    ca.r_first = p.slant_range_first;
+    ca.prf = p.prf;
+    ca.nbins = pl->nap;
+    HIP_CHECK(launch_col_const(ca, s));
+    ST_CHECK(mark(pl, "column_constants"));
+    // 2. fused range compression
+    RcArgs ra{};
+    ra.in = raw;
+    ra.in_stride = p.nr;
+    ra.nr_in = p.nr;
+    ra.out = pl->img;
+    ra.out_stride = pl->stride;
+    ra.rows = p.na;
+    ra.nrv = pl->nrv;
+    ra.zero_to = pl->nrv;
+    ra.coef = &pl->res->coef;
+    ra.hconj = pl->hconj;
+    ra.plan = pl->rplan->dev();
+    HIP_CHECK(launch_range_compress(ra, p.raw_format, s));
+    ST_CHECK(mark(pl, "range_compression"));
+    // 3. azimuth FFT (permuted bins), rows >= Na read as zero
+    ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
+    ST_CHECK(mark(pl, "azimuth_fft_a"));
+    if (pl->cs.n2 > 1) {
+        ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
+        ST_CHECK(mark(pl, "azimuth_fft_b"));
+    }
+    // 4. RCMC + azimuth matched filter
+    RcmcArgs rc{};
+    rc.img = pl->img;
+    rc.stride = pl->stride;
+    rc.nrv = pl->nrv;
+    rc.nbins = pl->nap;
+    rc.n1 = pl->cs.n1;
+    rc.n2 = pl->cs.n2;
+    rc.col = pl->col;
+    rc.prf = p.prf;
+    rc.beta = p.az_bandwidth_fraction;
+    rc.mode = 0;
+    HIP_CHECK(launch_rcmc_mf(rc, s));
+    ST_CHECK(mark(pl, "rcmc_matched_filter"));
+    // 5. azimuth IFFT, 1/Nap, rows < Na
+    ST_CHECK(col_inverse_pass(c, pl->cs, 0, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap));
+    ST_CHECK(mark(pl, "azimuth_ifft_b"));
+    if (pl->cs.n2 > 1) {
+        ST_CHECK(col_inverse_pass(c, pl->cs, 1, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap));
+        ST_CHECK(mark(pl, "azimuth_ifft_a"));
+    }
+    *slc = asarf_image{pl->img, pl->nrv, p.na, (int)pl->stride, pl->nap, ASARF_CF32};
+    if (p.imp) {
+        // 6. crop (results window), multilook x RSL, SRGR
+        MultilookArgs ma{};
+        ma.slc = pl->img;
+        ma.stride = pl->stride;
+        ma.row0_dev = &pl->res->crop_front;
+        ma.col0 = pl->near;
+        ma.in_rows_dev = &pl->res->pad;
+        ma.nr = pl->nr_s;
+        ma.out_rows = pl->imp_cap;
+        ma.out_rows_dev = &pl->res->imp_rows;
+        ma.ratio = pl->ml_ratio;
+        ma.rsl = pl->rsl;
+        ma.out = pl->ml;
+        HIP_CHECK(launch_multilook(ma, s));
+        ST_CHECK(mark(pl, "multilook_rsl"));
+        SrgrArgs sa{};
+        sa.in = pl->ml;
+        sa.in_nr = pl->nr_s;
+        sa.rows = pl->imp_cap;
+        sa.rows_dev = &pl->res->imp_rows;
+        for (int i = 0; i < 5; ++i) sa.grsr[i] = p.grsr_poly[i];
+        sa.dr = p.range_spacing;
+        sa.gs = p.ground_spacing;
+        sa.out_nr = p.gr_range_size;
+        sa.out = imp_out;
+        HIP_CHECK(launch_srgr(sa, s));
+        ST_CHECK(mark(pl, "srgr"));
+    }
+    return clear_error();
+}
+
+extern "C" int asarf_plan_kernel_times(asarf_plan* pl, const char** names, float* ms, int cap, int* count) {
+    if (!pl || !count) return set_error(ASARF_EINVAL, "asarf_plan_kernel_times: null argument");
+    if (!pl->timing || pl->ev_names.size() < 2) {
+        *count = 0;
+        return clear_error();
+    }
+    HIP_CHECK(hipEventSynchronize(pl->ev[pl->ev_names.size() - 1]));
+    const int n = (int)pl->ev_names.size() - 1;
+    *count = n;
+    for (int i = 0; i < n && i < cap; ++i) {
+        float t = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&t, pl->ev[i], pl->ev[i + 1]));
+        if (ms) ms[i] = t;
+        if (names) names[i] = pl->ev_names[i + 1];
+    }
+    return clear_error();
+}
+
+extern "C" int asarf_plan_results(asarf_plan* pl, asarf_focus_results* r) {
+    if (!pl || !r) return set_error(ASARF_EINVAL, "asarf_plan_results: null argument");
+    DevResults d;
+    HIP_CHECK(hipMemcpyAsync(&d, pl->res, sizeof(d), hipMemcpyDeviceToHost, pl->ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(pl->ctx->stream));
+    std::memset(r, 0, sizeof(*r));
+    r->iq.dc_i = d.dc_i;
+    r->iq.dc_q = d.dc_q;
+    r->iq.iq_gain = d.iq_gain;
+    r->iq.phase_error = d.phase_error;
+    r->iq.stddev_i = d.stddev_i;
+    r->iq.stddev_q = d.stddev_q;
+    for (int i = 0; i < 3; ++i) r->dc_poly[i] = d.dc_poly[i];
+    r->n_blocks = d.nblocks;
+    for (int i = 0; i < d.nblocks && i < 64; ++i) {
+        r->block_freq[i] = d.block_freq[i];
+        r->block_x[i] = d.block_x[i];
+    }
+    r->nrv = pl->nrv;
+    const bool pos = d.dc_poly[2] > 0.0;
+    r->crop_front = pos ? pl->crop_pos[0] : pl->crop_neg[0];
+    r->crop_back = pos ? pl->crop_pos[1] : pl->crop_neg[1];
+    r->crop_near = pl->near;
+    r->crop_far = pl->far;
+    r->imp_rows = pl->p.imp ? d.imp_rows : 0;
+    return clear_error();
+}

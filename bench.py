@@ -1,0 +1,185 @@
+"""Throughput of the fused focusing plan on one MI355X.
+
+One step = one pass of the GPU focusing stage over one synthetic raw scene already resident in HBM: I/Q and
+Doppler statistics, range compression, azimuth FFT, RCMC + azimuth matched filter, azimuth IFFT, and the IMP tail
+(crop, multilook x range-spreading loss, slant-to-ground resampling). Default workload: the full ENVISAT ASAR IM
+scene, 30000 x 5616 complex int8 samples.
+
+Prints one JSON line. value = raw input samples focused per second (Msamples/s). The roofline object is for the
+kernel with the largest average launch time, whose duration is measured with HIP events on the plan's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "asar-focus_amd"))
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "focused SLC Msamples/s per GPU + % HBM roofline; learned/BF16 vs plain at equal PSLR"
+
+
+def kernel_bytes(name, cfg, plan, res):
+    """Compulsory HBM bytes of one launch (logical sizes; zero padding synthesised in registers counts 0)."""
+    na, nr = cfg.na, cfg.nr
+    nrv = cfg.nr - cfg.nc
+    nap = plan.params.nap
+    if name == "raw_moments":
+        return 2 * na * nr
+    if name == "range_compression":
+        return 2 * na * nr + 8 * na * nrv
+    if name in ("azimuth_fft_a",):
+        return 8 * na * nrv + 8 * nap * nrv
+    if name in ("azimuth_fft_b", "rcmc_matched_filter", "azimuth_ifft_b"):
+        return 16 * nap * nrv
+    if name == "azimuth_ifft_a":
+        return 8 * nap * nrv + 8 * na * nrv
+    if name == "multilook_rsl":
+        front, back, near, far = res["crop"]
+        return 8 * (na - front - back) * (nrv - near - far) + 4 * res["imp_rows"] * (nrv - near - far)
+    if name == "srgr":
+        front, back, near, far = res["crop"]
+        return 4 * res["imp_rows"] * (nrv - near - far) + 4 * res["imp_rows"] * plan.ng
+    return None
+
+
+def cpu_baseline(seconds_hint=True):
+    """FP64 oracle (the timed CPU baseline) on a bounded sample of the same workload's radar."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from asarf import configs, scene
+
+    cfg = configs.asar_scene().scaled(na=8192, name="asar_im_8192x5616_sample")
+    raw, _ = scene.generate(cfg)
+    x = scene.raw_to_complex(raw)
+    threads = po.get_threads()
+    t0 = time.perf_counter()
+    po.focus_slc(x, cfg)
+    dt = time.perf_counter() - t0
+    return {
+        "value": cfg.na * cfg.nr / dt / 1e6,
+        "unit": "Msamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"ASAR IM radar, {cfg.na} x {cfg.nr} int8 block, whole SLC chain (I/Q, DC, RC, RDA) in the FP64 "
+                  f"oracle, {dt:.2f} s on {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="asar_scene")
+    ap.add_argument("--no-imp", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by whole azimuth blocks per GPU, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from asarf import configs, focus, scene
+
+    cfg = configs.CONFIGS[args.config]()
+    imp = not args.no_imp
+    t0 = time.perf_counter()
+    raw, _ = scene.generate(cfg)
+    gen_s = time.perf_counter() - t0
+    traw = torch.from_numpy(raw).cuda()
+    ctx = focus.Context(0)
+    plan = focus.Plan(ctx, cfg, imp=imp)
+
+    for _ in range(args.warmup):
+        plan.execute(traw)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        plan.execute(traw)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / args.steps
+    res = plan.results()
+
+    # per-kernel launch durations (HIP events between launches on the plan's stream)
+    plan.set_timing(True)
+    acc = {}
+    for _ in range(args.timing_steps):
+        plan.execute(traw)
+        for k, v in plan.kernel_times().items():
+            acc.setdefault(k, []).append(v)
+    plan.set_timing(False)
+    stages = {k: float(np.mean(v)) for k, v in acc.items()}
+    gbs = {}
+    for k, v in stages.items():
+        b = kernel_bytes(k, cfg, plan, res)
+        if b:
+            gbs[k] = b / (v * 1e-3) / 1e9
+    dom = max((k for k in stages if kernel_bytes(k, cfg, plan, res)), key=lambda k: stages[k])
+    dom_bytes = kernel_bytes(dom, cfg, plan, res)
+    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(cfg.name, {}).get(dom)
+
+    value = cfg.na * cfg.nr / (ms * 1e-3) / 1e6
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 geometry and phase)",
+        "data": "synthetic (seeded Philox scene: point targets in gamma-textured clutter)",
+        "config": {
+            "workload": cfg.name,
+            "azimuth_lines": cfg.na,
+            "range_samples": cfg.nr,
+            "raw_format": cfg.raw_format,
+            "fft_sizes": [plan.params.nrp, plan.params.nap],
+            "imp": imp,
+            "form": "plain FP32 Stockham, 16-tap sinc RCMC",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": dom_bytes,
+            "ms": round(stages[dom], 4),
+        },
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "stages_gbs": {k: round(v, 1) for k, v in gbs.items()},
+        "slc_output_msamples_per_s": round(cfg.na * (cfg.nr - cfg.nc) / (ms * 1e-3) / 1e6, 2),
+        "plan_device_gb": round(plan.device_bytes / 1e9, 3),
+        "scene_generation_s": round(gen_s, 2),
+        "estimates": {"dc_poly": res["dc_poly"], "iq_gain": res["iq"]["iq_gain"],
+                      "phase_error_deg": res["iq"]["phase_error"], "imp_rows": res["imp_rows"]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    plan.close()
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+/*
+ * asarf — MI355X-native range-Doppler SAR focusser: C ABI.
+ *
+ * This is the boundary a host flow with the reference's interfaces (cgi-estonia-space/asar-focus 0.3.2) calls in
+ * place of its CUDA stage. Each entry point names the reference function it replaces. All device pointers are
+ * HIP device memory on the context's device; calls are ordered on the context's stream and are asynchronous
+ * unless documented as returning host results. Every call returns an asarf_status; the reason for a failure is
+ * available from asarf_last_error() on the calling thread. Arguments are validated before anything is launched.
+ * The library never frees memory it did not allocate.
+ */
+#ifndef ASARF_H
+#define ASARF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------ status codes */
+typedef enum {
+    ASARF_OK = 0,
+    ASARF_EINVAL = 1,       /* null pointer, bad flag or parameter value */
+    ASARF_ESHAPE = 2,       /* image size/stride inconsistent (reference: std::logic_error) */
+    ASARF_EWORKSPACE = 3,   /* workspace smaller than required (cufft_checks.h:92-98, device_padded_image.cuh:147-151) */
+    ASARF_EHIP = 4,         /* HIP runtime error */
+    ASARF_ENOMEM = 5,       /* device allocation failed */
+    ASARF_EUNSUPPORTED = 6  /* size outside what the kernels support (e.g. FFT length beyond LDS) */
+} asarf_status;
+
+/* Reference quirks reproduced by default (ASARF_QUIRKS_REFERENCE). */
+enum {
+    ASARF_Q1_IQ_PADDED_WIDTH = 1 << 0, /* I/Q correction over the padded row width, iq_correction.cu:197-198 */
+    ASARF_Q2_COS_NO_SQRT = 1 << 1,     /* cos_corr = 1 - d^2, iq_correction.cu:259-262 */
+    ASARF_Q4_DC_EXTRA_BLOCK = 1 << 2   /* 33rd Doppler block over the padding, fractional_doppler_centroid.cu:107-112 */
+};
+#define ASARF_QUIRKS_REFERENCE (ASARF_Q1_IQ_PADDED_WIDTH | ASARF_Q2_COS_NO_SQRT | ASARF_Q4_DC_EXTRA_BLOCK)
+
+enum { ASARF_CF32 = 0, ASARF_F32 = 1 };                     /* image element types */
+enum { ASARF_RAW_CF32 = 0, ASARF_RAW_U8 = 1, ASARF_RAW_I8 = 2 }; /* raw sample formats (I,Q interleaved) */
+enum { ASARF_FORWARD = 0, ASARF_INVERSE = 1 };
+enum { ASARF_FFT_ROWS = 0, ASARF_FFT_COLUMNS = 1 };
+
+/* ------------------------------------------------------------------------------------------ memory model */
+/* Padded image: logical (x_size, y_size) inside (x_stride, y_stride); row-major, range fastest.
+ * Replaces DevicePaddedImage (cuda_util/device_padded_image.cuh:28-158). */
+typedef struct {
+    void* data;
+    int x_size, y_size;
+    int x_stride, y_stride;
+    int dtype; /* ASARF_CF32 (8 B) or ASARF_F32 (4 B) */
+} asarf_image;
+
+/* Owning device buffer with its byte size. Replaces CudaWorkspace (cuda_util/cuda_workplace.h:28-90). */
+typedef struct {
+    void* data;
+    size_t bytes;
+} asarf_workspace;
+
+typedef struct asarf_ctx asarf_ctx;
+
+/* Context: device, stream and caches (FFT twiddles, scratch). One context per host thread.
+ * stream: the hipStream_t to run on (NULL = the device's default stream). Device initialisation is deferred
+ * to the first call that launches work, so argument validation works on hosts without a GPU.
+ * Replaces CudaInit (cuda_util/cuda_device_init.h:26-43). */
+int asarf_ctx_create(int device, void* stream, asarf_ctx** out);
+int asarf_ctx_destroy(asarf_ctx* ctx);
+int asarf_sync(asarf_ctx* ctx);
+void* asarf_ctx_stream(asarf_ctx* ctx);
+const char* asarf_last_error(void);
+const char* asarf_version(void);
+
+/* InitPadded + ZeroMemory (device_padded_image.cuh:30-41,116). */
+int asarf_image_alloc(asarf_ctx* ctx, int x_size, int y_size, int x_stride, int y_stride, int dtype, asarf_image* out);
+int asarf_image_free(asarf_ctx* ctx, asarf_image* img);
+int asarf_image_zero(asarf_ctx* ctx, asarf_image* img);
+size_t asarf_image_bytes(const asarf_image* img); /* TotalByteSize (device_padded_image.cuh:72-74) */
+/* CudaWorkspace(byte_size) / destructor (cuda_workplace.h:35-38,89). */
+int asarf_workspace_alloc(asarf_ctx* ctx, size_t bytes, asarf_workspace* out);
+int asarf_workspace_free(asarf_ctx* ctx, asarf_workspace* ws);
+/* InitExtPtr (device_padded_image.cuh:49-61): the image adopts the workspace memory; ws becomes empty. */
+int asarf_image_adopt(asarf_image* img, int x_size, int y_size, int x_stride, int y_stride, int dtype,
+                      asarf_workspace* ws);
+/* ReleaseMemory (device_padded_image.cuh:104-111): the image gives its memory to ws (which must be empty). */
+int asarf_image_release(asarf_image* img, asarf_workspace* ws);
+
+/* Strided copy of the sub-rectangle (x0, y0, width, height) of img's logical region to dst (device or host
+ * memory, dst_pitch bytes per row); asynchronous on the context stream (synchronises when dst is pageable host
+ * memory). Replaces the cudaMemcpy2D subset of SubsetResultsAndReassembleMeta (main_flow.cc:615-620) and
+ * CopyToHostLogicalSize (device_padded_image.cuh:81-87). */
+int asarf_image_subset_copy(asarf_ctx* ctx, const asarf_image* img, int x0, int y0, int width, int height, void* dst,
+                            size_t dst_pitch);
+
+/* ------------------------------------------------------------------------------------------ host helpers */
+/* Smallest 2^a 3^b 5^c 7^d strictly greater than n (<= 131072), or -1.
+ * Replaces GetOptimalFFTSizes().upper_bound(n) (cuda_util/cufft_plan.cc:147-181, main.cc:217,223). */
+int asarf_fft_padding_size(int n);
+
+/* ChirpInfo (sar/sar_metadata.h:34-44). */
+typedef struct {
+    double range_sampling_rate; /* Hz */
+    double pulse_duration;      /* s */
+    int n_samples;
+    double pulse_bandwidth;     /* Hz */
+    double kr;                  /* Hz/s */
+    int apply_window;           /* periodic Hamming */
+} asarf_chirp_info;
+/* GenerateChirpData (sar/sar_chirp.cc:70-104), FP64 on the host: writes `padding` complex floats (I,Q). */
+int asarf_generate_chirp(const asarf_chirp_info* chirp, int padding, float* out_iq);
+
+/* Polyfit / Polyval (util/src/math_utils.cc:23-44, util/include/math_utils.h:55-63); highest power first. */
+int asarf_polyfit(const double* x, const double* y, int n, int degree, double* coeffs);
+double asarf_polyval(const double* p, int n, double x);
+
+/* ------------------------------------------------------------------------------------------ stages */
+/* SARResults subset written by the I/Q correction (sar/sar_metadata.h:46-55). */
+typedef struct {
+    double dc_i, dc_q;
+    double iq_gain;
+    double phase_error; /* degrees */
+    double stddev_i, stddev_q;
+} asarf_iq_results;
+
+/* RawDataCorrection + ZeroNaNs (sar/iq_correction.cu:188-273, main.cc:244-245), in place on a cf32 image.
+ * n_total_samples: divisor of the mean and deviation (0 -> x_size*y_size). Returns host results (synchronises). */
+int asarf_iq_correction(asarf_ctx* ctx, asarf_image* img, size_t n_total_samples, int quirks,
+                        asarf_iq_results* results);
+
+/* CalculateDopplerCentroid (sar/fractional_doppler_centroid.cu:98-162): poly[3] highest power first, in range
+ * index. Optional block_freq/block_x (capacity max_blocks) receive the per-block estimates. Synchronises. */
+int asarf_doppler_centroid(asarf_ctx* ctx, const asarf_image* img, double prf, int quirks, double* poly3,
+                           double* block_freq, double* block_x, int max_blocks, int* n_blocks);
+
+/* RangeCompression (sar/range_compression.cu:57-111), in place on a cf32 image whose x_stride equals the chirp
+ * length. chirp_iq: host, n_chirp complex floats (the padded, windowed chirp). On return x_size = x_size - nc.
+ * ws may be NULL (the fused row kernel needs no FFT work area). */
+int asarf_range_compression(asarf_ctx* ctx, asarf_image* img, const float* chirp_iq, int n_chirp, int chirp_samples,
+                            asarf_workspace* ws);
+
+/* Parameters of RangeDopplerAlgorithm taken from SARMetadata (range_doppler_algorithm.cu:337-353). */
+typedef struct {
+    double wavelength;
+    double range_spacing;
+    double slant_range_first;
+    double prf;
+    double az_bandwidth_fraction;
+    double dc_poly[3]; /* highest power first */
+    double vr_poly[3]; /* highest power first */
+} asarf_rda_params;
+
+/* RangeDopplerAlgorithm (sar/range_doppler_algorithm.cu:312-407): azimuth FFT, 16-tap sinc RCMC, azimuth
+ * matched filter, azimuth IFFT, 1/Nap. src is the range-compressed image. If out->data is NULL, out adopts ws
+ * (as the reference does) and afterwards ws holds src's memory; otherwise out is used as given and src is left
+ * as scratch. out receives the SLC in its logical region (x < src.x_size, y < src.y_size). */
+int asarf_range_doppler(asarf_ctx* ctx, const asarf_rda_params* params, asarf_image* src, asarf_image* out,
+                        asarf_workspace* ws);
+
+/* GetRcmcParameters (sar/focussing_details.h:24-28): window size of the RCMC interpolator (16). */
+int asarf_rcmc_window_size(void);
+
+/* AzimuthMultiLook (sar/az_multilook.cu:82-101): slc cf32 (x_size == x_stride not required), output f32 rows x
+ * slc.x_size into out (capacity checked). Writes the output row count and the new azimuth spacing. */
+int asarf_az_multilook(asarf_ctx* ctx, const asarf_image* slc, double azimuth_spacing, asarf_workspace* out,
+                       int* out_rows, double* new_azimuth_spacing);
+
+/* RangeSpreadingLossCorrection (sar/range_spreading_loss.cu:42-69), in place on f32 rows. */
+int asarf_range_spreading_loss(asarf_ctx* ctx, float* img, int range_size, int azimuth_size, double slant_range_first,
+                               double range_spacing, double reference_range);
+
+/* Kernel part of SlantRangeToGroundRangeInterpolation (sar/srgr.cu:49-71,278-296): grsr[5] maps ground distance
+ * [m] to slant distance from the first sample [m] (highest power first), evaluated in FP64. */
+int asarf_srgr(asarf_ctx* ctx, const float* in, int in_range_size, int azimuth_size, const double* grsr5,
+               double slant_range_spacing, double ground_spacing, int out_range_size, float* out);
+
+/* Host part of SlantRangeToGroundRangeInterpolation (sar/srgr.cu:73-121) for a spherical earth: samples 100 slant
+ * ranges R_i = R_first + i*(range_size/100)*dr, ground distance along the sphere from the first one, degree-4
+ * fit. Writes grsr[5] and the output range size floor(g_99 / ground_spacing). */
+int asarf_srgr_fit_spherical(double slant_range_first, double range_spacing, int range_size, double orbit_radius,
+                             double earth_radius, double ground_spacing, double* grsr5, int* out_range_size);
+
+/* Batched complex FFT (test hook; replaces the cuFFT plans of cufft_plan.cc:34-91). ROWS: `batch` transforms of
+ * length n at data + i*dist, contiguous. COLUMNS: transforms of length n down `batch` columns of a row-major
+ * array with row stride `dist` elements. Unnormalised unless scale != 1. In place. */
+int asarf_fft_c2c(asarf_ctx* ctx, void* data, int n, int batch, long dist, int layout, int direction, float scale);
+
+/* ------------------------------------------------------------------------------------------ fused plan */
+/* The whole GPU stage of main.cc:233-327 as one stream-ordered sequence with no host round trip:
+ * raw bytes -> statistics + Doppler (one pass) -> fused range compression -> azimuth FFT -> RCMC + matched filter
+ * -> azimuth IFFT -> [IMP: crop, multilook x RSL, SRGR]. */
+typedef struct {
+    int na, nr, nc;          /* azimuth lines, range samples, chirp samples */
+    int nrp, nap;            /* FFT sizes, 0 -> reference padding rule */
+    int raw_format;          /* ASARF_RAW_U8 or ASARF_RAW_I8 */
+    double prf, wavelength, range_spacing, slant_range_first, az_bandwidth_fraction;
+    double vr_poly[3];
+    size_t n_total_samples;  /* 0 -> na*nr */
+    int quirks;
+    int imp;                 /* also produce the detected ground-range product */
+    double azimuth_spacing;  /* m, for the multilook ratio 12.5/azimuth_spacing */
+    double reference_range;  /* RSL reference range [m] (asar_constants.h:37: 800 km) */
+    double grsr_poly[5];     /* ground -> slant polynomial (see asarf_srgr_fit_spherical) */
+    int gr_range_size;       /* ground-range output samples */
+    double ground_spacing;   /* 12.5 m */
+    int lines_before, lines_after; /* extra lines around the requested sensing window (main.cc:137-142) */
+} asarf_focus_params;
+
+typedef struct {
+    asarf_iq_results iq;
+    double dc_poly[3];
+    int n_blocks;
+    double block_freq[64];
+    double block_x[64];
+    int nrv;                 /* SLC range samples */
+    int crop_front, crop_back, crop_near, crop_far; /* results window (main_flow.cc:155-201) */
+    int imp_rows;
+} asarf_focus_results;
+
+typedef struct asarf_plan asarf_plan;
+
+/* chirp_iq: host, nrp complex floats. Allocates the plan's device buffers (image, tables, scratch). */
+int asarf_plan_create(asarf_ctx* ctx, const asarf_focus_params* params, const float* chirp_iq, asarf_plan** out);
+int asarf_plan_destroy(asarf_plan* plan);
+/* Device bytes held by the plan. */
+size_t asarf_plan_device_bytes(const asarf_plan* plan);
+/* Runs the stage. raw: device, na*nr I/Q byte pairs. On return (asynchronously) *slc views the plan-owned SLC
+ * (x_size = nrv, y_size = na); imp_out (device, may be NULL when params.imp == 0) receives up to
+ * asarf_plan_imp_capacity rows x gr_range_size floats. */
+int asarf_plan_execute(asarf_plan* plan, const void* raw, asarf_image* slc, float* imp_out);
+int asarf_plan_imp_capacity(const asarf_plan* plan, int* rows, int* cols);
+/* Copies the estimator outputs to the host (synchronises). */
+int asarf_plan_results(asarf_plan* plan, asarf_focus_results* results);
+/* Per-kernel timing of the last execute (HIP events on the plan's stream; enable before executing). */
+int asarf_plan_set_timing(asarf_plan* plan, int enable);
+int asarf_plan_kernel_times(asarf_plan* plan, const char** names, float* ms, int capacity, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,254 @@
+"""ctypes binding of the FP64 oracle (build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
+product path. Arrays are numpy complex128 images with an explicit row stride (the padded image model).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+Q1_IQ_PADDED_WIDTH = 1
+Q2_COS_NO_SQRT = 2
+Q4_DC_EXTRA_BLOCK = 4
+QUIRKS_REFERENCE = Q1_IQ_PADDED_WIDTH | Q2_COS_NO_SQRT | Q4_DC_EXTRA_BLOCK
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class IqResults(ctypes.Structure):
+    _fields_ = [("dc_i", ctypes.c_double), ("dc_q", ctypes.c_double), ("iq_gain", ctypes.c_double),
+                ("phase_error", ctypes.c_double), ("stddev_i", ctypes.c_double), ("stddev_q", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RdaParams(ctypes.Structure):
+    _fields_ = [("wavelength", ctypes.c_double), ("range_spacing", ctypes.c_double),
+                ("slant_range_first", ctypes.c_double), ("prf", ctypes.c_double),
+                ("az_bandwidth_fraction", ctypes.c_double), ("dc_poly", ctypes.c_double * 3),
+                ("vr_poly", ctypes.c_double * 3)]
+
+
+class FocusParams(ctypes.Structure):
+    _fields_ = [("na", ctypes.c_int), ("nr", ctypes.c_int), ("nc", ctypes.c_int), ("nrp", ctypes.c_int),
+                ("nap", ctypes.c_int), ("prf", ctypes.c_double), ("wavelength", ctypes.c_double),
+                ("range_spacing", ctypes.c_double), ("slant_range_first", ctypes.c_double),
+                ("az_bandwidth_fraction", ctypes.c_double), ("vr_poly", ctypes.c_double * 3),
+                ("chirp_kr", ctypes.c_double), ("range_sampling_rate", ctypes.c_double),
+                ("apply_range_window", ctypes.c_int), ("quirks", ctypes.c_int),
+                ("n_total_samples", ctypes.c_size_t)]
+
+
+class FocusResults(ctypes.Structure):
+    _fields_ = [("iq", IqResults), ("dc_poly", ctypes.c_double * 3), ("nrv", ctypes.c_int)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: build the oracle (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ci, cd = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        sig = {
+            "or_fft_rows": (None, [vp, ci, ci, ci, ci]),
+            "or_fft_cols": (None, [vp, ci, ci, ci, ci]),
+            "or_dft": (None, [vp, vp, ci, ci]),
+            "or_next_fft_size": (ci, [ci]),
+            "or_generate_chirp": (None, [cd, cd, ci, ci, ci, vp]),
+            "or_polyfit": (ci, [vp, vp, ci, ci, vp]),
+            "or_iq_correction": (None, [vp, ci, ci, ci, ci, ctypes.c_size_t, ci, ctypes.POINTER(IqResults)]),
+            "or_doppler_centroid": (ci, [vp, ci, ci, ci, cd, ci, vp, vp, ci, vp]),
+            "or_range_compression": (ci, [vp, ci, ci, ci, vp, ci]),
+            "or_range_doppler": (None, [vp, vp, ci, ci, ci, ci, ctypes.POINTER(RdaParams)]),
+            "or_rcmc": (None, [vp, vp, ci, ci, ci, ctypes.POINTER(RdaParams)]),
+            "or_az_matched_filter": (None, [vp, ci, ci, ci, ctypes.POINTER(RdaParams)]),
+            "or_rcmc_window": (None, [vp]),
+            "or_az_multilook_rows": (ci, [ci, cd]),
+            "or_az_multilook": (None, [vp, ci, ci, ci, cd, vp]),
+            "or_rsl": (None, [vp, ci, ci, cd, cd, cd]),
+            "or_srgr": (None, [vp, ci, ci, vp, cd, cd, ci, vp]),
+            "or_aperture_pixels": (ci, [cd, cd, cd, vp, vp, ci, cd, cd, ci]),
+            "or_focus_slc": (ci, [vp, ctypes.POINTER(FocusParams), vp, ctypes.POINTER(FocusResults)]),
+            "or_set_threads": (None, [ci]),
+            "or_get_threads": (ci, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _c128(a):
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+def set_threads(n: int):
+    lib().or_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().or_get_threads()
+
+
+def fft_rows(a: np.ndarray, inverse=False, n=None):
+    """In place on a 2-D complex128 array: FFT of length n (default a.shape[1]) over every row."""
+    n = a.shape[1] if n is None else n
+    lib().or_fft_rows(_c128(a), n, a.shape[0], a.shape[1], int(inverse))
+    return a
+
+
+def fft_cols(a: np.ndarray, inverse=False, cols=None):
+    cols = a.shape[1] if cols is None else cols
+    lib().or_fft_cols(_c128(a), a.shape[0], cols, a.shape[1], int(inverse))
+    return a
+
+
+def dft(x: np.ndarray, inverse=False):
+    x = np.ascontiguousarray(x, np.complex128)
+    out = np.empty_like(x)
+    lib().or_dft(x.ctypes.data, out.ctypes.data, x.size, int(inverse))
+    return out
+
+
+def next_fft_size(n: int) -> int:
+    return lib().or_next_fft_size(int(n))
+
+
+def generate_chirp(kr, fs, n_samples, apply_window, padding) -> np.ndarray:
+    out = np.zeros(padding, np.complex64)
+    lib().or_generate_chirp(kr, fs, n_samples, int(apply_window), padding, out.ctypes.data)
+    return out
+
+
+def polyfit(x, y, degree):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    c = np.zeros(degree + 1)
+    rc = lib().or_polyfit(x.ctypes.data, y.ctypes.data, x.size, degree, c.ctypes.data)
+    if rc != 0:
+        raise ValueError("polyfit failed")
+    return c
+
+
+def iq_correction(img, x_size, y_size, n_total=0, quirks=QUIRKS_REFERENCE):
+    r = IqResults()
+    lib().or_iq_correction(_c128(img), x_size, y_size, img.shape[1], img.shape[0], n_total, quirks, ctypes.byref(r))
+    return r.as_dict()
+
+
+def doppler_centroid(img, x_size, y_size, prf, quirks=QUIRKS_REFERENCE):
+    bf = np.zeros(64)
+    bx = np.zeros(64)
+    poly = np.zeros(3)
+    nb = lib().or_doppler_centroid(_c128(img), x_size, y_size, img.shape[1], prf, quirks, bf.ctypes.data,
+                                   bx.ctypes.data, 64, poly.ctypes.data)
+    if nb < 0:
+        raise ValueError("doppler centroid failed")
+    return poly, bx[:nb].copy(), bf[:nb].copy()
+
+
+def range_compression(img, x_size, y_size, chirp: np.ndarray, nc):
+    ch = np.ascontiguousarray(chirp, np.complex64)
+    assert ch.size == img.shape[1]
+    return lib().or_range_compression(_c128(img), x_size, y_size, img.shape[1], ch.ctypes.data, nc)
+
+
+def rda_params(wavelength, range_spacing, r_first, prf, beta, dc_poly, vr_poly) -> RdaParams:
+    p = RdaParams()
+    p.wavelength, p.range_spacing, p.slant_range_first, p.prf, p.az_bandwidth_fraction = (
+        wavelength, range_spacing, r_first, prf, beta)
+    for i in range(3):
+        p.dc_poly[i] = dc_poly[i]
+        p.vr_poly[i] = vr_poly[i]
+    return p
+
+
+def range_doppler(src, x_size, y_size, p: RdaParams):
+    out = np.zeros_like(src)
+    lib().or_range_doppler(_c128(src), _c128(out), x_size, y_size, src.shape[1], src.shape[0], ctypes.byref(p))
+    return out
+
+
+def rcmc(freq_img, x_size, p: RdaParams):
+    out = np.zeros_like(freq_img)
+    lib().or_rcmc(_c128(freq_img), _c128(out), x_size, freq_img.shape[0], freq_img.shape[1], ctypes.byref(p))
+    return out
+
+
+def az_matched_filter(freq_img, x_size, p: RdaParams):
+    lib().or_az_matched_filter(_c128(freq_img), x_size, freq_img.shape[0], freq_img.shape[1], ctypes.byref(p))
+    return freq_img
+
+
+def rcmc_window() -> np.ndarray:
+    w = np.zeros(16, np.float32)
+    lib().or_rcmc_window(w.ctypes.data)
+    return w
+
+
+def az_multilook(slc: np.ndarray, az_spacing: float) -> np.ndarray:
+    slc = np.ascontiguousarray(slc, np.complex128)
+    rows = lib().or_az_multilook_rows(slc.shape[0], az_spacing)
+    out = np.zeros((rows, slc.shape[1]))
+    lib().or_az_multilook(slc.ctypes.data, slc.shape[1], slc.shape[0], slc.shape[1], az_spacing, out.ctypes.data)
+    return out
+
+
+def rsl(img: np.ndarray, r_first, dr, r_ref=800000.0):
+    assert img.dtype == np.float64 and img.flags.c_contiguous
+    lib().or_rsl(img.ctypes.data, img.shape[1], img.shape[0], r_first, dr, r_ref)
+    return img
+
+
+def srgr(img: np.ndarray, grsr5, dr, gs, out_nr):
+    img = np.ascontiguousarray(img, np.float64)
+    g = np.ascontiguousarray(grsr5, np.float64)
+    out = np.zeros((img.shape[0], out_nr))
+    lib().or_srgr(img.ctypes.data, img.shape[1], img.shape[0], g.ctypes.data, dr, gs, out_nr, out.ctypes.data)
+    return out
+
+
+def aperture_pixels(wavelength, prf, beta, vr_poly, dc_poly, r_first, dr, x):
+    v = np.ascontiguousarray(vr_poly, np.float64)
+    d = np.ascontiguousarray(dc_poly if len(dc_poly) else [0.0], np.float64)
+    return lib().or_aperture_pixels(wavelength, prf, beta, v.ctypes.data, d.ctypes.data, len(dc_poly), r_first, dr,
+                                    int(x))
+
+
+def focus_slc(raw_c64: np.ndarray, cfg, quirks=QUIRKS_REFERENCE, nrp=None, nap=None):
+    """Whole path on a (na, nr) complex64 raw block. Returns (slc complex128 (na, nrv), results dict)."""
+    raw = np.ascontiguousarray(raw_c64, np.complex64)
+    na, nr = raw.shape
+    fp = FocusParams()
+    fp.na, fp.nr, fp.nc = na, nr, cfg.nc
+    fp.nrp = cfg.nrp if nrp is None else nrp
+    fp.nap = cfg.nap if nap is None else nap
+    fp.prf, fp.wavelength, fp.range_spacing = cfg.prf, cfg.wavelength, cfg.range_spacing
+    fp.slant_range_first, fp.az_bandwidth_fraction = cfg.r_first, cfg.az_bandwidth_fraction
+    for i, v in enumerate(cfg.vr_poly):
+        fp.vr_poly[i] = v
+    fp.chirp_kr, fp.range_sampling_rate = cfg.kr, cfg.fs
+    fp.apply_range_window = int(cfg.apply_range_window)
+    fp.quirks = quirks
+    fp.n_total_samples = 0
+    nrv = nr - cfg.nc
+    slc = np.zeros((na, nrv), np.complex128)
+    res = FocusResults()
+    rc = lib().or_focus_slc(raw.ctypes.data, ctypes.byref(fp), slc.ctypes.data, ctypes.byref(res))
+    if rc != 0:
+        raise RuntimeError(f"oracle focus failed ({rc})")
+    return slc, {"iq": res.iq.as_dict(), "dc_poly": list(res.dc_poly), "nrv": res.nrv}

@@ -1,0 +1,88 @@
+"""GPU parity of the fused focusing plan (raw bytes -> SLC [-> IMP]) against the FP64 oracle, and point-target
+quality parity (PSLR/ISLR within 0.05 dB, IRW within 1 %)."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+from asarf import analysis, configs, focus, scene
+
+pytestmark = pytest.mark.gpu
+
+SLC_TOL = 1e-4  # relative RMS, FP32 GPU vs FP64 oracle
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def run_plan(ctx, cfg, quirks=po.QUIRKS_REFERENCE, imp=False):
+    raw, _ = scene.generate(cfg)
+    traw = torch.from_numpy(raw).cuda()
+    plan = focus.Plan(ctx, cfg, quirks=quirks, imp=imp)
+    plan.execute(traw)
+    slc = plan.slc_tensor().cpu().numpy()
+    res = plan.results()
+    imp_img = plan.imp_out.cpu().numpy() if imp else None
+    plan.close()
+    return raw, slc, res, imp_img
+
+
+@pytest.mark.parametrize("quirks", [po.QUIRKS_REFERENCE, 0])
+@pytest.mark.parametrize("which", ["ers", "asar"])
+def test_plan_matches_oracle_small(ctx, which, quirks):
+    if which == "ers":
+        cfg = configs.ers_block().scaled(na=700, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6))
+    else:
+        cfg = configs.asar_scene().scaled(na=640, nr=1400, n_targets_random=6)
+    raw, slc, res, _ = run_plan(ctx, cfg, quirks)
+    ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg, quirks=quirks)
+    err = rel_rms(slc, ref)
+    print(f"{which} quirks={quirks} slc rel rms {err:.3e}")
+    assert err < SLC_TOL
+    for k, v in rres["iq"].items():
+        assert abs(res["iq"][k] - v) <= 1e-9 * max(1.0, abs(v)), k
+    xs = np.arange(0, cfg.nr - cfg.nc)
+    assert np.abs(np.polyval(res["dc_poly"], xs) - np.polyval(rres["dc_poly"], xs)).max() < 1e-6
+
+
+def test_plan_ers_block_full_size_parity_and_quality(ctx):
+    """4096 x 5616 ERS block: SLC within 1e-4 of the oracle; point-target PSLR/ISLR within 0.05 dB, IRW 1 %."""
+    cfg = configs.ers_block()
+    raw, slc, res, _ = run_plan(ctx, cfg)
+    ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    err = rel_rms(slc, ref)
+    print(f"ers block slc rel rms {err:.3e}")
+    assert err < SLC_TOL
+    tx, ty = scene.targets(cfg)
+    worst = {"pslr": 0.0, "islr": 0.0, "irw": 0.0}
+    for x, y in zip(tx, ty):
+        mg = analysis.point_target(slc, x, y)
+        mr = analysis.point_target(ref, x, y)
+        for ax in ("range", "azimuth"):
+            worst["pslr"] = max(worst["pslr"], abs(mg[ax]["pslr"] - mr[ax]["pslr"]))
+            worst["islr"] = max(worst["islr"], abs(mg[ax]["islr"] - mr[ax]["islr"]))
+            worst["irw"] = max(worst["irw"], abs(mg[ax]["irw"] / mr[ax]["irw"] - 1.0))
+    print("worst target metric deltas", worst)
+    assert worst["pslr"] < 0.05 and worst["islr"] < 0.05 and worst["irw"] < 0.01
+
+
+def test_plan_imp_matches_oracle_chain(ctx):
+    cfg = configs.asar_scene().scaled(na=1200, nr=1400, n_targets_random=8)
+    raw, slc, res, imp = run_plan(ctx, cfg, imp=True)
+    ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    front, back, near, far = res["crop"]
+    sub = ref[front:cfg.na - back, near:ref.shape[1] - far]
+    ml = po.az_multilook(sub, cfg.az_spacing)
+    po.rsl(ml, cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, 800e3)
+    nr_s = ref.shape[1] - near - far
+    grsr, ng = focus.srgr_fit_spherical(cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, nr_s,
+                                        cfg.earth_radius + cfg.orbit_height, cfg.earth_radius, 12.5)
+    gr = po.srgr(ml, grsr, cfg.range_spacing, 12.5, ng)
+    assert res["imp_rows"] == gr.shape[0]
+    got = imp[: res["imp_rows"], :ng]
+    err = float(np.sqrt(np.sum((got - gr) ** 2) / np.sum(gr ** 2)))
+    print(f"imp rel rms {err:.3e}")
+    assert err < 2e-4
