@@ -55,17 +55,18 @@ def test_column_fft_matches_oracle(ctx, n, cols):
 # ------------------------------------------------------------------------------------------ I/Q correction ----
 @pytest.mark.parametrize("quirks", [po.QUIRKS_REFERENCE, 0])
 def test_iq_correction_matches_oracle(ctx, quirks):
-    cfg = configs.ers_block().scaled(na=300, nr=700, n_targets_az=1, n_targets_rg=1)
+    cfg = configs.ers_block().scaled(na=300, nr=1100, n_targets_az=1, n_targets_rg=1)
     raw, _ = scene.generate(cfg)
     x = scene.raw_to_complex(raw)
-    nrp, nap = po.next_fft_size(700 + cfg.nc), po.next_fft_size(300)
+    nrp, nap = po.next_fft_size(1100 + cfg.nc), po.next_fft_size(300)
     img = np.zeros((nap, nrp), np.complex64)
-    img[:300, :700] = x
+    img[:300, :1100] = x
     img[5, 7] = np.nan + 1j * np.nan
+    img[9, 3] = np.nan + 2.0j
     ref = img.astype(np.complex128)
-    rres = po.iq_correction(ref, 700, 300, n_total=300 * 700, quirks=quirks)
+    rres = po.iq_correction(ref, 1100, 300, n_total=300 * 1100, quirks=quirks)
     t = cuda(img)
-    gres = focus.iq_correction(ctx, t, 700, 300, n_total=300 * 700, quirks=quirks)
+    gres = focus.iq_correction(ctx, t, 1100, 300, n_total=300 * 1100, quirks=quirks)
     for k in rres:
         assert abs(gres[k] - rres[k]) <= 1e-9 * max(1.0, abs(rres[k])), k
     assert rel_rms(t.cpu().numpy(), ref) < 1e-6
@@ -87,7 +88,7 @@ def test_doppler_centroid_matches_oracle(ctx):
 
 
 # ------------------------------------------------------------------------------------- range compression ----
-@pytest.mark.parametrize("nr,na", [(5616, 48), (700, 64)])
+@pytest.mark.parametrize("nr,na", [(5616, 48), (1300, 64)])
 def test_range_compression_matches_oracle(ctx, nr, na):
     cfg = configs.ers_block()
     nrp = po.next_fft_size(nr + cfg.nc)
