@@ -204,6 +204,38 @@ struct Dft<R, INV, typename std::enable_if<(R > 4) && !is_prime(R)>::type> {
 };
 
 // ----------------------------------------------------------------------------------------- pass engine ----
+// Two-level twiddle table in LDS: W_n^e = hi[e >> bits] * lo[e & (2^bits - 1)], about 2 sqrt(n) entries, so every
+// twiddle is two LDS reads and one complex multiply instead of a global-memory read.
+struct TwLds {
+    const float2* hi;
+    const float2* lo;
+    int bits;
+    int mask;
+    __device__ __forceinline__ float2 operator()(int e) const { return cmul(hi[e >> bits], lo[e & mask]); }
+};
+
+__host__ __device__ inline int tw_bits(int n) {
+    int b = 0;
+    while ((1 << (2 * b)) < n) ++b;
+    return b;
+}
+__host__ __device__ inline int tw_lds_elems(int n) {
+    const int b = tw_bits(n);
+    return ((n - 1) >> b) + 1 + (1 << b);
+}
+
+// Fills the two-level table from the global table W_n^m (m < n) and returns its accessor. Caller syncs.
+__device__ __forceinline__ TwLds tw_fill(float2* dst, const float2* tw_global, int n, int tid, int nthreads) {
+    const int b = tw_bits(n);
+    const int nhi = ((n - 1) >> b) + 1;
+    const int nlo = 1 << b;
+    for (int i = tid; i < nhi + nlo; i += nthreads) {
+        const int m = i < nhi ? (i << b) : (i - nhi);
+        dst[i] = tw_global[m < n ? m : 0];
+    }
+    return TwLds{dst, dst + nhi, b, nlo - 1};
+}
+
 // Radices compiled into the runtime dispatch. The host planner only emits these.
 #ifndef ASARF_FOR_EACH_RADIX
 #define ASARF_FOR_EACH_RADIX(X) X(2) X(3) X(4) X(5) X(7) X(8) X(16)
@@ -237,25 +269,50 @@ __device__ __forceinline__ void pass_load(float2 (&v)[MAXE], int s, int S, int n
     }
 }
 
+// j / ns and j % ns without integer division: q = trunc((j + 0.5) / ns) evaluated in float is exact while
+// n * 2^-23 < 0.5 (n < 4M), since (j + 0.5)/ns stays at least 0.5/ns away from an integer.
+__device__ __forceinline__ int fast_div(int j, int ns, float inv_ns) {
+    return __float2int_rz(((float)j + 0.5f) * inv_ns);
+}
+
 // Twiddle + DFT of every butterfly of the thread (pass with prefix product ns); clamped lanes compute a copy.
-template <int R, int MAXE, bool INV>
-__device__ __forceinline__ void pass_compute(float2 (&v)[MAXE], int s, int S, int n, int ns, const float2* tw) {
+// TW(e) returns W_n^e for 0 <= e < n.
+template <int R, int MAXE, bool INV, class TwAt>
+__device__ __forceinline__ void pass_compute(float2 (&v)[MAXE], int s, int S, int n, int ns, TwAt&& tw_at) {
     constexpr int NB = MAXE / R;
     const int nr = n / R;
-    const int tstride = n / (ns * R);
+    if (ns > 1) {
+        const int tstride = n / (ns * R);
+        const float inv_ns = 1.0f / (float)ns;
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-        const int j = min(s + i * S, nr - 1);
-        const int base = (j % ns) * tstride;  // 0 when ns == 1
-        int idx = base;
+        for (int i = 0; i < NB; ++i) {
+            const int j = min(s + i * S, nr - 1);
+            const int k = j - fast_div(j, ns, inv_ns) * ns;
+            const int base = k * tstride;
+            // w^r from one table lookup per power of two and products of at most two factors: w^r = w^(2^a) w^b
+            // with b < 2^a, so every twiddle carries at most a few ulp more than the table.
+            float2 pw[R];
+            pw[1] = tw_at(base);
 #pragma unroll
-        for (int r = 1; r < R; ++r) {
-            const float2 w = tw[idx];
-            v[i * R + r] = INV ? cmulc(v[i * R + r], w) : cmul(v[i * R + r], w);
-            idx += base;
+            for (int r = 2; r < R; ++r) {
+                if ((r & (r - 1)) == 0) {
+                    pw[r] = tw_at(base * r);
+                } else {
+                    int hb = 1;
+                    while (hb * 2 <= r) hb *= 2;
+                    pw[r] = cmul(pw[hb], pw[r - hb]);
+                }
+            }
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[i * R + r] = INV ? cmulc(v[i * R + r], pw[r]) : cmul(v[i * R + r], pw[r]);
+            Dft<R, INV>::run(&v[i * R]);
+            // keep butterflies in sequence: hoisting every butterfly's twiddles at once costs registers, not time
+            __builtin_amdgcn_sched_barrier(0);
         }
-        Dft<R, INV>::run(&v[i * R]);
+        return;
     }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) Dft<R, INV>::run(&v[i * R]);
 }
 
 // Store phase: dst((j/ns)*ns*R + j%ns + r*ns, value).
@@ -263,11 +320,12 @@ template <int R, int MAXE, class Dst>
 __device__ __forceinline__ void pass_store(float2 (&v)[MAXE], int s, int S, int n, int ns, Dst&& dst) {
     constexpr int NB = MAXE / R;
     const int nr = n / R;
+    const float inv_ns = 1.0f / (float)ns;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
         const int j = s + i * S;
         if (j < nr) {
-            const int k = j % ns;
+            const int k = j - fast_div(j, ns, inv_ns) * ns;
             const int base = (j - k) * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst(base + r * ns, v[i * R + r]);
@@ -297,7 +355,8 @@ __device__ __forceinline__ void pass_visit(float2 (&v)[MAXE], int s, int S, int 
 // to scratch memory.
 //
 // Kernels using these macros have a template parameter RSET and name their state: `v` (float2[kMaxE]), `lane`,
-// `nlanes`, `fft_n`, `fft_tw`, and the LDS accessors `lds_store(int, float2)` / `lds_load(int)`.
+// `nlanes`, `fft_n`, the twiddle accessor `tw_at(int e)` = W_n^e, and the LDS accessors `lds_store(int, float2)`
+// / `lds_load(int)`.
 // Kernels are instantiated per radix set (a bit mask, see radix_bit); cases outside the set compile to nothing,
 // so each instantiation only carries the register pressure of the radices it can run.
 constexpr int radix_bit(int r) {
@@ -319,10 +378,10 @@ constexpr int kRadixAll = 127;
 
 #define ASARF_STORE_LDS_(RR) pass_store<RR>(v, lane, nlanes, fft_n, prev_ns_, lds_store)
 #define ASARF_LOAD_LDS_(RR) pass_load<RR>(v, lane, nlanes, fft_n, lds_load)
-#define ASARF_COMPUTE_FWD_(RR) pass_compute<RR, kMaxE, false>(v, lane, nlanes, fft_n, ns_, fft_tw)
-#define ASARF_COMPUTE_INV_(RR) pass_compute<RR, kMaxE, true>(v, lane, nlanes, fft_n, ns_, fft_tw)
-#define ASARF_COMPUTE1_FWD(RR) pass_compute<RR, kMaxE, false>(v, lane, nlanes, fft_n, 1, fft_tw)
-#define ASARF_COMPUTE1_INV(RR) pass_compute<RR, kMaxE, true>(v, lane, nlanes, fft_n, 1, fft_tw)
+#define ASARF_COMPUTE_FWD_(RR) pass_compute<RR, kMaxE, false>(v, lane, nlanes, fft_n, ns_, tw_at)
+#define ASARF_COMPUTE_INV_(RR) pass_compute<RR, kMaxE, true>(v, lane, nlanes, fft_n, ns_, tw_at)
+#define ASARF_COMPUTE1_FWD(RR) pass_compute<RR, kMaxE, false>(v, lane, nlanes, fft_n, 1, tw_at)
+#define ASARF_COMPUTE1_INV(RR) pass_compute<RR, kMaxE, true>(v, lane, nlanes, fft_n, 1, tw_at)
 
 // Passes 1..NPASS-1 of one transform direction (pass 0 already loaded and computed). RADIX_AT(p) gives the
 // radix of pass p in this direction. On exit the registers hold the final outputs.

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_fft(RowFftArgs a) {
     const int row = blockIdx.x;
     const int lane = threadIdx.x, nlanes = kRowThreads;
     const int fft_n = a.plan.n;
-    const float2* fft_tw = a.plan.tw;
+    const TwLds tw_at = tw_fill(lds + lds_row_elems(fft_n), a.plan.tw, fft_n, lane, nlanes);
     const float2* in = a.in + (long)row * a.in_stride;
     float2* out = a.out + (long)row * a.out_stride;
     float2 v[kMaxE];
@@ -68,7 +68,7 @@ static int plan_mask(const FftPlanDev& p) {
 }
 
 hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s) {
-    const size_t lds = sizeof(float2) * (size_t)lds_row_elems(a.plan.n);
+    const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
     if (lds > g_max_lds) return hipErrorInvalidValue;
     const int m = plan_mask(a.plan);
 #define X_(SET)                                                                                          \
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kRowThreads) void k_range_compress(RcArgs a) {
     const int row = blockIdx.x;
     const int lane = threadIdx.x, nlanes = kRowThreads;
     const int fft_n = a.plan.n;
-    const float2* fft_tw = a.plan.tw;
+    const TwLds tw_at = tw_fill(lds + lds_row_elems(fft_n), a.plan.tw, fft_n, lane, nlanes);
     float2 v[kMaxE];
     IqCoef c;
     if (a.coef) {
@@ -174,9 +174,11 @@ __global__ __launch_bounds__(kRowThreads) void k_range_compress(RcArgs a) {
 }
 
 hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
-    const size_t lds = sizeof(float2) * (size_t)lds_row_elems(a.plan.n);
+    const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
     if (lds > g_max_lds) return hipErrorInvalidValue;
     if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8) return hipErrorInvalidValue;
+    hipError_t serr = hipSuccess;
+    if (launch_range_compress_static(a, fmt, s, &serr)) return serr;
     const int m = plan_mask(a.plan);
 #define X_(SET)                                                                                                 \
     if ((m & ~(SET)) == 0) {                                                                                    \
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
     const int col = tile * kColTile + b;
     const bool col_ok = col < a.ncols;
     const int fft_n = a.sub.n;
-    const float2* fft_tw = a.sub.tw;
+    const TwLds tw_at = tw_fill(lds + fft_n * (kColTile + 1), a.sub.tw, fft_n, threadIdx.x, kThreads);
     float2 v[kMaxE];
     const float2* in = a.in + (col_ok ? col : 0);
     const long in_base = (long)g * a.in_g;
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
 }
 
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s) {
-    const size_t lds = sizeof(float2) * (size_t)a.sub.n * (kColTile + 1);
+    const size_t lds = sizeof(float2) * (size_t)(a.sub.n * (kColTile + 1) + tw_lds_elems(a.sub.n));
     if (lds > g_max_lds) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((long)a.ntiles * a.groups));
     const int m = plan_mask(a.sub);

@@ -1,0 +1,165 @@
+// Range compression with compile-time transform plans for the production row lengths.
+//
+// Same algorithm as k_range_compress (correct on load, forward Stockham FFT, x conj(H)/n, inverse FFT, store the
+// kept columns), but the radix sequence, the lane count and the LDS padding are template parameters chosen so
+// that every lane owns exactly N/T elements in every pass: no runtime radix dispatch, no idle lanes, compile-time
+// index arithmetic. Lengths without a specialisation use the generic kernel.
+//   6144 (ASAR IM, 5616 + 521 samples): 256 lanes x 24 elements, radices 8, 8, 8, 12
+//   6400 (ERS IM,  5616 + 704 samples): 320 lanes x 20 elements, radices 20, 20, 4, 4
+//   8192 (split-precision ERS form):    256 lanes x 32 elements, radices 8, 8, 8, 16
+#include "kernels.h"
+
+namespace asarf {
+
+template <int... Rs>
+struct RadixSeq {
+    static constexpr int count = sizeof...(Rs);
+    static constexpr int at(int p) {
+        constexpr int a[] = {Rs...};
+        return a[p];
+    }
+    static constexpr int rat(int p) { return at(count - 1 - p); }
+    static constexpr int prefix(int p) {
+        int x = 1;
+        for (int i = 0; i < p; ++i) x *= at(i);
+        return x;
+    }
+    static constexpr int rprefix(int p) {
+        int x = 1;
+        for (int i = 0; i < p; ++i) x *= rat(i);
+        return x;
+    }
+    static constexpr int product() { return prefix(count); }
+};
+
+template <typename T>
+struct RawPairS;
+template <>
+struct RawPairS<uint8_t> {
+    using type = uchar2;
+};
+template <>
+struct RawPairS<int8_t> {
+    using type = char2;
+};
+
+template <int N, int PADSHIFT>
+__host__ __device__ constexpr int static_lds_elems() {
+    return N + (N >> PADSHIFT) + 1;
+}
+
+template <int FMT, int N, int T, int PADSHIFT, int WAVES, class Seq>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_rc_static(RcArgs a) {
+    static_assert(Seq::product() == N, "radix sequence must factor N");
+    constexpr int E = N / T;
+    static_assert(E * T == N, "lanes must divide N");
+    extern __shared__ float2 lds[];
+    const int row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const TwLds tw_at = tw_fill(lds + static_lds_elems<N, PADSHIFT>(), a.plan.tw, N, lane, T);
+    float2 v[E];
+    IqCoef c;
+    if (a.coef) {
+        c = *a.coef;
+    } else {
+        c.dc_i = c.dc_q = c.sin_corr = 0.f;
+        c.inv_gain = c.inv_cos = 1.f;
+        c.identity = 1;
+    }
+    const int nr_in = a.nr_in;
+    auto correct = [&](float i, float q) {
+        if (c.identity) return make_float2(i, q);
+        const float ic = (i - c.dc_i) * c.inv_gain;
+        const float qc = (q - c.dc_q - ic * c.sin_corr) * c.inv_cos;
+        return make_float2(ic, qc);
+    };
+    auto gl_load = [&](int x) -> float2 {
+        if constexpr (FMT == RAW_CF32) {
+            const float2* in = static_cast<const float2*>(a.in) + (long)row * a.in_stride;
+            const float2 r = in[x];
+            return correct(r.x, r.y);
+        } else {
+            if (x >= nr_in) return make_float2(0.f, 0.f);
+            using P = typename RawPairS<typename std::conditional<FMT == RAW_U8, uint8_t, int8_t>::type>::type;
+            const P* in = static_cast<const P*>(a.in) + (long)row * a.in_stride;
+            const P r = in[x];
+            return correct((float)r.x, (float)r.y);
+        }
+    };
+    auto lds_store = [&](int i, float2 x) { lds[i + (i >> PADSHIFT)] = x; };
+    auto lds_load = [&](int i) { return lds[i + (i >> PADSHIFT)]; };
+
+    // forward transform
+    constexpr int R0 = Seq::at(0);
+    pass_load<R0>(v, lane, T, N, gl_load);
+    pass_compute<R0, E, false>(v, lane, T, N, 1, tw_at);
+    static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
+        constexpr int p = decltype(pc)::value;
+        constexpr int Rp = Seq::at(p - 1), R = Seq::at(p);
+        pass_store<Rp>(v, lane, T, N, Seq::prefix(p - 1), lds_store);
+        __syncthreads();
+        pass_load<R>(v, lane, T, N, lds_load);
+        __syncthreads();
+        pass_compute<R, E, false>(v, lane, T, N, Seq::prefix(p), tw_at);
+    });
+    // matched filter in the range frequency domain
+    constexpr int RL = Seq::at(Seq::count - 1);
+    const float2* h = a.hconj;
+    auto spec_mul = [&](int k, float2& x, bool) { x = cmul(x, h[k]); };
+    pass_visit<RL>(v, lane, T, N, spec_mul);
+    __syncthreads();  // all LDS reads of the forward transform precede the inverse stores
+    // inverse transform (radices in reverse order; its first pass consumes the registers as they are)
+    pass_compute<RL, E, true>(v, lane, T, N, 1, tw_at);
+    static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
+        constexpr int p = decltype(pc)::value;
+        constexpr int Rp = Seq::rat(p - 1), R = Seq::rat(p);
+        pass_store<Rp>(v, lane, T, N, Seq::rprefix(p - 1), lds_store);
+        __syncthreads();
+        pass_load<R>(v, lane, T, N, lds_load);
+        __syncthreads();
+        pass_compute<R, E, true>(v, lane, T, N, Seq::rprefix(p), tw_at);
+    });
+    float2* out = a.out + (long)row * a.out_stride;
+    const int nrv = a.nrv, zt = a.zero_to;
+    auto out_write = [&](int x, float2& val, bool ok) {
+        if (!ok) return;
+        if (x < nrv)
+            out[x] = val;
+        else if (x < zt)
+            out[x] = make_float2(0.f, 0.f);
+    };
+    constexpr int RIL = Seq::rat(Seq::count - 1);
+    pass_visit<RIL>(v, lane, T, N, out_write);
+}
+
+template <int N, int T, int PADSHIFT, int WAVES, class Seq>
+static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
+    const size_t lds = sizeof(float2) * (size_t)(static_lds_elems<N, PADSHIFT>() + tw_lds_elems(N));
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
+    if (fmt == RAW_CF32)
+        hipLaunchKernelGGL((k_rc_static<RAW_CF32, N, T, PADSHIFT, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
+    else if (fmt == RAW_U8)
+        hipLaunchKernelGGL((k_rc_static<RAW_U8, N, T, PADSHIFT, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_rc_static<RAW_I8, N, T, PADSHIFT, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
+    return hipGetLastError();
+}
+
+// Returns true (and the launch status in *err) when a specialisation exists for the plan's length.
+bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipError_t* err) {
+    switch (a.plan.n) {
+        case 6144:
+            *err = launch_static<6144, 256, 5, 2, RadixSeq<8, 8, 8, 12>>(a, fmt, s);
+            return true;
+        case 6400:
+            *err = launch_static<6400, 320, 7, 3, RadixSeq<20, 20, 4, 4>>(a, fmt, s);
+            return true;
+        case 8192:
+            *err = launch_static<8192, 256, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s);
+            return true;
+        default:
+            return false;
+    }
+}
+
+}  // namespace asarf

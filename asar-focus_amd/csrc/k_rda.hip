@@ -91,13 +91,16 @@ __global__ __launch_bounds__(kThreads) void k_rcmc_mf(RcmcArgs a) {
                     const float2* tap = row + cx - 8;
 #pragma unroll
                     for (int i = 0; i < 16; ++i) {
-                        const float wi = w[i] / (d + (float)(8 - i));
+                        // d + (8 - i) is exact to one rounding; the 1-ulp reciprocal error cancels in the
+                        // normalisation to first order
+                        const float wi = w[i] * __builtin_amdgcn_rcpf(d + (float)(8 - i));
                         const float2 t = tap[i];
                         acc.x += wi * t.x;
                         acc.y += wi * t.y;
                         den += wi;
                     }
-                    val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x / den, acc.y / den);
+                    const float inv = __builtin_amdgcn_rcpf(den);
+                    val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
                 }
             }
         } else {

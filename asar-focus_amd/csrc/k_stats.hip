@@ -7,8 +7,8 @@
 //    :156-257,322-414 and sar/fractional_doppler_centroid.cu:33-96.
 //  * raw-byte kernels (the fused plan): one pass over the uint8/int8 samples collects, per Doppler block, exact
 //    integer sums of I, Q, I^2, Q^2, IQ and of the lag-1 cross products between consecutive lines. The I/Q
-//    correction is affine, so the corrected lag-1 sum of every block is a bilinear form in these sums; a single
-//    FP64 finalisation thread turns them into the DC bias, gain, phase correction, block Doppler estimates and the
+//    correction is affine, so the corrected lag-1 sum of every block is a bilinear form in these sums; one FP64
+//    finalisation wave turns them into the DC bias, gain, phase correction, block Doppler estimates and the
 //    degree-2 Doppler polynomial without a second read of the raw data.
 #include "kernels.h"
 #include "poly.h"
@@ -168,6 +168,13 @@ hipError_t launch_reduce_i64(const long long* partials, int nparts, int nvals, l
 // ---------------------------------------------------------------------------------- raw moments ----
 // Moment slots: 0 SI, 1 SQ, 2 SII, 3 SQQ, 4 SIQ, 5 L_II, 6 L_QQ, 7 L_IQ (I_prev Q_cur), 8 L_QI (Q_prev I_cur),
 // 9 L_I1 (sum I_prev), 10 L_Q1, 11 L_I2 (sum I_cur), 12 L_Q2, 13 N, 14 M (lag pairs).
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
 template <typename P>
 __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
     const int tile = blockIdx.x;
@@ -181,6 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
 #pragma unroll
     for (int i = 0; i < 13; ++i) acc[i] = 0;
     int n = 0, m = 0;
+    constexpr int U = 8;  // rows loaded per iteration (independent loads in flight)
     for (int x = x0 + threadIdx.x; x < x1; x += kThreads) {
         int pi = 0, pq = 0;
         bool have_prev = false;
@@ -190,46 +198,51 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
             pq = p.y;
             have_prev = true;
         }
-        for (int y = y0; y < y1; ++y) {
-            const P c = raw[(long)y * a.nr + x];
-            const int ci = c.x, cq = c.y;
-            acc[0] += ci;
-            acc[1] += cq;
-            acc[2] += ci * ci;
-            acc[3] += cq * cq;
-            acc[4] += ci * cq;
-            ++n;
-            if (have_prev) {
-                acc[5] += pi * ci;
-                acc[6] += pq * cq;
-                acc[7] += pi * cq;
-                acc[8] += pq * ci;
-                acc[9] += pi;
-                acc[10] += pq;
-                acc[11] += ci;
-                acc[12] += cq;
-                ++m;
+        for (int y = y0; y < y1; y += U) {
+            P c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) c[u] = (y + u < y1) ? raw[(long)(y + u) * a.nr + x] : P{0, 0};
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (y + u >= y1) break;
+                const int ci = c[u].x, cq = c[u].y;
+                acc[0] += ci;
+                acc[1] += cq;
+                acc[2] += ci * ci;
+                acc[3] += cq * cq;
+                acc[4] += ci * cq;
+                ++n;
+                if (have_prev) {
+                    acc[5] += pi * ci;
+                    acc[6] += pq * cq;
+                    acc[7] += pi * cq;
+                    acc[8] += pq * ci;
+                    acc[9] += pi;
+                    acc[10] += pq;
+                    acc[11] += ci;
+                    acc[12] += cq;
+                    ++m;
+                }
+                pi = ci;
+                pq = cq;
+                have_prev = true;
             }
-            pi = ci;
-            pq = cq;
-            have_prev = true;
         }
     }
-    __shared__ long long sh[kRawMoments][kThreads];
+    __shared__ long long sh[kThreads / 64][kRawMoments];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < 13; ++i) sh[i][threadIdx.x] = acc[i];
-    sh[13][threadIdx.x] = n;
-    sh[14][threadIdx.x] = m;
+    for (int i = 0; i < kRawMoments; ++i) {
+        const long long v = wave_sum<long long>(i < 13 ? (long long)acc[i] : (i == 13 ? (long long)n : (long long)m));
+        if (lane == 0) sh[wave][i] = v;
+    }
     __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
+    if (threadIdx.x < kRawMoments) {
+        long long t = 0;
 #pragma unroll
-            for (int i = 0; i < kRawMoments; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + w];
-        }
-        __syncthreads();
+        for (int w = 0; w < kThreads / 64; ++w) t += sh[w][threadIdx.x];
+        a.partials[((long)strip * a.ntiles + tile) * kRawMoments + threadIdx.x] = t;
     }
-    if (threadIdx.x < kRawMoments)
-        a.partials[((long)strip * a.ntiles + tile) * kRawMoments + threadIdx.x] = sh[threadIdx.x][0];
 }
 
 hipError_t launch_raw_moments(const RawMomentsArgs& a, int fmt, hipStream_t s) {
@@ -243,28 +256,76 @@ hipError_t launch_raw_moments(const RawMomentsArgs& a, int fmt, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Single-thread FP64 finalisation of the fused statistics. See the derivation in the file header: with the
-// corrected sample u = a I + b, v = c I + d Q + e, the block lag sum is
+// Degree-M-1 least squares on one wave (lane i owns point i, n <= 64): Householder QR with wave reductions,
+// the same steps as polyfit_qr (util/src/math_utils.cc:23-44), result highest power first.
+template <int M>
+__device__ bool polyfit_wave(double x, double y, int n, double* out) {
+    const int lane = threadIdx.x & 63;
+    const bool in = lane < n;
+    double a[M];
+    double v = 1.0;
+#pragma unroll
+    for (int c = 0; c < M; ++c) {
+        a[c] = in ? v : 0.0;
+        v *= x;
+    }
+    double b = in ? y : 0.0;
+    double rdiag[M], rrow[M][M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const double ak = lane >= k ? a[k] : 0.0;
+        const double nrm = sqrt(wave_sum(ak * ak));
+        if (nrm == 0.0) return false;
+        const double akk = __shfl(a[k], k, 64);
+        const double alpha = akk > 0.0 ? -nrm : nrm;
+        const double hv = (lane == k) ? ak - alpha : ak;  // Householder vector (zero above row k)
+        const double vv = wave_sum(hv * hv);
+#pragma unroll
+        for (int c = k + 1; c < M; ++c) {
+            const double d = wave_sum(hv * a[c]);
+            a[c] -= 2.0 * d / vv * hv;
+        }
+        const double d = wave_sum(hv * b);
+        b -= 2.0 * d / vv * hv;
+        rdiag[k] = alpha;
+#pragma unroll
+        for (int c = k + 1; c < M; ++c) rrow[k][c] = __shfl(a[c], k, 64);
+    }
+    double qb[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) qb[k] = __shfl(b, k, 64);
+    double coef[M];
+#pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
+        double t = qb[k];
+#pragma unroll
+        for (int c = k + 1; c < M; ++c) t -= rrow[k][c] * coef[c];
+        coef[k] = t / rdiag[k];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) out[k] = coef[M - 1 - k];
+    return true;
+}
+
+// One-wave FP64 finalisation of the fused statistics. With the corrected sample u = a I + b, v = c I + d Q + e,
+// the lag-1 sum of a Doppler block is
 //   sum conj(s_prev) s_cur = sum(u1 u2 + v1 v2) + i sum(u1 v2 - v1 u2),
 // each term expanded over the raw lag moments. Padding columns of the last block (Q4) hold the corrected image
-// of raw 0 when the correction runs over the padded width (Q1), i.e. the constant (b, e).
-__global__ void k_finalize(FinalizeArgs f) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// of raw 0 when the correction runs over the padded width (Q1), i.e. the constant (b, e). Lane b handles block b.
+__global__ __launch_bounds__(64) void k_finalize(FinalizeArgs f) {
+    const int lane = threadIdx.x;
     const long long* T = f.totals;
-    double s[kRawMoments];
-    for (int i = 0; i < kRawMoments; ++i) s[i] = 0.0;
-    for (int t = 0; t < f.ntiles; ++t)
-        for (int i = 0; i < 5; ++i) s[i] += (double)T[t * kRawMoments + i];
-    double nvalid = 0.0;
-    for (int t = 0; t < f.ntiles; ++t) nvalid += (double)T[t * kRawMoments + 13];
-    const double SI = s[0], SQ = s[1], SII = s[2], SQQ = s[3], SIQ = s[4];
+    double st[6] = {0, 0, 0, 0, 0, 0};  // SI SQ SII SQQ SIQ N
+    for (int t = lane; t < f.ntiles; t += 64) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) st[i] += (double)T[t * kRawMoments + i];
+        st[5] += (double)T[t * kRawMoments + 13];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st[i] = wave_sum(st[i]);
+    const double SI = st[0], SQ = st[1], SII = st[2], SQQ = st[3], SIQ = st[4], nvalid = st[5];
     const double ntot = f.n_total;
     const double dci = SI / ntot, dcq = SQ / ntot;
-    DevResults* r = f.res;
-    r->dc_i = dci;
-    r->dc_q = dcq;
-    r->stddev_i = sqrt((SII - 2.0 * dci * SI + nvalid * dci * dci) / ntot);
-    r->stddev_q = sqrt((SQQ - 2.0 * dcq * SQ + nvalid * dcq * dcq) / ntot);
     const bool q1 = (f.quirks & 1) != 0;
     const double P = q1 ? (double)f.na * (double)(f.x_stride - f.nr) : 0.0;
     const double sii = SII - 2.0 * dci * SI + nvalid * dci * dci + P * dci * dci;
@@ -273,22 +334,12 @@ __global__ void k_finalize(FinalizeArgs f) {
     const double siq = (SIQ - dcq * SI - dci * SQ + nvalid * dci * dcq + P * dci * dcq) / g;
     const double delta = siq / (sii / (g * g));
     const double cosc = (f.quirks & 2) ? (1.0 - delta * delta) : sqrt(1.0 - delta * delta);
-    r->iq_gain = g;
-    r->phase_error = asin(delta) * 180.0 / kPi;
-    IqCoef c;
-    c.dc_i = (float)dci;
-    c.dc_q = (float)dcq;
-    c.inv_gain = (float)(1.0 / g);
-    c.sin_corr = (float)delta;
-    c.inv_cos = (float)(1.0 / cosc);
-    c.identity = 0;
-    r->coef = c;
-    // affine coefficients in FP64
     const double ca = 1.0 / g, cb = -dci / g;
     const double cc = -delta / (g * cosc), cd = 1.0 / cosc, ce = (-dcq + delta * dci / g) / cosc;
-    double bx[kMaxDcBlocks], by[kMaxDcBlocks];
-    for (int blk = 0; blk < f.nblocks; ++blk) {
-        const long long* t = T + blk * kRawMoments;
+    DevResults* r = f.res;
+    double bx = 0.0, by = 0.0;
+    if (lane < f.nblocks) {
+        const long long* t = T + lane * kRawMoments;
         const double LII = (double)t[5], LQQ = (double)t[6], LIQ = (double)t[7], LQI = (double)t[8];
         const double I1 = (double)t[9], Q1 = (double)t[10], I2 = (double)t[11], Q2 = (double)t[12];
         const double M = (double)t[14];
@@ -297,29 +348,44 @@ __global__ void k_finalize(FinalizeArgs f) {
                           cd * ce * (Q1 + Q2) + ce * ce * M;
         const double uv = ca * cc * LII + ca * cd * LIQ + ca * ce * I1 + cb * cc * I2 + cb * cd * Q2 + cb * ce * M;
         const double vu = cc * ca * LII + cc * cb * I1 + cd * ca * LQI + cd * cb * Q1 + ce * ca * I2 + ce * cb * M;
-        double re = uu + vv, im = uv - vu;
-        // padding columns of this block (only the last one can reach past nr)
-        const int xb = blk * f.step, xe = xb + f.step;
+        double re = uu + vv;
+        const double im = uv - vu;
+        const int xb = lane * f.step, xe = xb + f.step;
         const int pad0 = max(xb, f.nr), pad1 = min(xe, f.x_stride);
         if (q1 && pad1 > pad0 && f.na > 1) re += (double)(f.na - 1) * (double)(pad1 - pad0) * (cb * cb + ce * ce);
-        const double fr = f.prf * atan2(im, re) / (2.0 * kPi);
-        bx[blk] = (double)((xb + xe) / 2);
-        by[blk] = fr;
-        r->block_freq[blk] = fr;
-        r->block_x[blk] = bx[blk];
+        bx = (double)((xb + xe) / 2);
+        by = f.prf * atan2(im, re) / (2.0 * kPi);
+        r->block_freq[lane] = by;
+        r->block_x[lane] = bx;
     }
-    r->nblocks = f.nblocks;
-    double poly[3] = {0.0, 0.0, 0.0};
-    if (polyfit_qr(bx, by, f.nblocks, 2, poly) != 0) poly[0] = poly[1] = poly[2] = 0.0;
-    r->dc_poly[0] = poly[0];
-    r->dc_poly[1] = poly[1];
-    r->dc_poly[2] = poly[2];
-    // results window: lines cut at the start when the constant Doppler term is positive, else at the end
-    const bool pos = poly[2] > 0.0;
-    r->crop_front = pos ? f.crop_dc_pos : f.crop_dc_neg;
-    const int in_rows = pos ? f.imp_in_rows_pos : f.imp_in_rows_neg;
-    r->imp_rows = (f.ml_ratio > 0.f) ? (int)roundf((float)in_rows / f.ml_ratio) : 0;
-    r->pad = in_rows;
+    double poly[3];
+    if (!polyfit_wave<3>(bx, by, f.nblocks, poly)) poly[0] = poly[1] = poly[2] = 0.0;
+    if (lane == 0) {
+        r->dc_i = dci;
+        r->dc_q = dcq;
+        r->stddev_i = sqrt((SII - 2.0 * dci * SI + nvalid * dci * dci) / ntot);
+        r->stddev_q = sqrt((SQQ - 2.0 * dcq * SQ + nvalid * dcq * dcq) / ntot);
+        r->iq_gain = g;
+        r->phase_error = asin(delta) * 180.0 / kPi;
+        IqCoef c;
+        c.dc_i = (float)dci;
+        c.dc_q = (float)dcq;
+        c.inv_gain = (float)(1.0 / g);
+        c.sin_corr = (float)delta;
+        c.inv_cos = (float)(1.0 / cosc);
+        c.identity = 0;
+        r->coef = c;
+        r->nblocks = f.nblocks;
+        r->dc_poly[0] = poly[0];
+        r->dc_poly[1] = poly[1];
+        r->dc_poly[2] = poly[2];
+        // results window: lines cut at the start when the constant Doppler term is positive, else at the end
+        const bool pos = poly[2] > 0.0;
+        r->crop_front = pos ? f.crop_dc_pos : f.crop_dc_neg;
+        const int in_rows = pos ? f.imp_in_rows_pos : f.imp_in_rows_neg;
+        r->imp_rows = (f.ml_ratio > 0.f) ? (int)roundf((float)in_rows / f.ml_ratio) : 0;
+        r->pad = in_rows;
+    }
 }
 
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s) {

@@ -14,7 +14,7 @@ constexpr int kThreads = 256;      // generic kernels and column transforms
 constexpr int kRowThreads = 512;   // row transforms: one row per workgroup
 constexpr int kColTile = 16;   // columns per workgroup in column transforms (16 x 8 B = one 128 B line)
 constexpr int kMaxE = 16;      // complex elements per lane in a Stockham pass
-constexpr int kStripRows = 256; // rows per workgroup in the raw statistics pass
+constexpr int kStripRows = 128; // rows per workgroup in the raw statistics pass
 constexpr int kRawMoments = 15; // values per (strip, tile) in the raw statistics pass
 constexpr int kMaxDcBlocks = 64;
 
@@ -161,6 +161,7 @@ struct SrgrArgs {
 // ---- launchers (k_*.hip) ----
 hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s);
+bool launch_range_compress_static(const RcArgs& a, int raw_format, hipStream_t s, hipError_t* err);
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s);
