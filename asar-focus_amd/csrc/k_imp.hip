@@ -6,45 +6,54 @@
 
 namespace asarf {
 
-__global__ void k_multilook(MultilookArgs a) {
+// Each lane owns one range column and walks a chunk of output rows; the |s|^2 of every input row is read once
+// and kept in a per-lane ring of 8 rows in LDS (a window spans at most 5 input rows).
+constexpr int kMlRows = 64;  // output rows per workgroup
+
+__global__ __launch_bounds__(256) void k_multilook(MultilookArgs a) {
+    __shared__ float ring[8][256];
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int out_rows = a.out_rows_dev ? *a.out_rows_dev : a.out_rows;
-    if (x >= a.nr || y >= out_rows || y >= a.out_rows) return;
+    const int out_rows = min(a.out_rows_dev ? *a.out_rows_dev : a.out_rows, a.out_rows);
+    const int y0 = blockIdx.y * kMlRows;
+    const int y1 = min(y0 + kMlRows, out_rows);
+    if (x >= a.nr || y0 >= y1) return;
     const int row0 = a.row0_dev ? *a.row0_dev : a.row0;
     const int in_rows = a.in_rows_dev ? *a.in_rows_dev : a.in_rows;
-    // resampling position in float, exactly as the reference computes it
-    const float in_idx = (float)y * a.ratio;
-    const float mu = in_idx - (float)(int)in_idx;
-    int s0 = (int)(in_idx - 2.0f), e0 = (int)(in_idx + 2.0f);
-    int s1 = (int)(in_idx - 1.0f), e1 = (int)(in_idx + 3.0f);
-    s0 = max(s0, 0);
-    e0 = min(e0, in_rows);
-    s1 = max(s1, 0);
-    e1 = min(e1, in_rows);
     const float2* col = a.slc + (long)row0 * a.stride + a.col0 + x;
-    float sum0 = 0.f, sum1 = 0.f;
-    int c0 = 0, c1 = 0;
-    for (int i = s0; i < e0; ++i) {
-        const float2 v = col[(long)i * a.stride];
-        sum0 += v.x * v.x + v.y * v.y;
-        ++c0;
+    float* rg = &ring[0][threadIdx.x];
+    int next = -1;
+    for (int y = y0; y < y1; ++y) {
+        // resampling position in float, exactly as the reference computes it
+        const float in_idx = (float)y * a.ratio;
+        const float mu = in_idx - (float)(int)in_idx;
+        int s0 = (int)(in_idx - 2.0f), e0 = (int)(in_idx + 2.0f);
+        int s1 = (int)(in_idx - 1.0f), e1 = (int)(in_idx + 3.0f);
+        s0 = max(s0, 0);
+        e0 = min(e0, in_rows);
+        s1 = max(s1, 0);
+        e1 = min(e1, in_rows);
+        if (next < s0) next = s0;
+        const int hi = max(e0, e1);
+        for (; next < hi; ++next) {
+            const float2 v = col[(long)next * a.stride];
+            rg[(next & 7) * 256] = v.x * v.x + v.y * v.y;
+        }
+        float sum0 = 0.f, sum1 = 0.f;
+        for (int i = s0; i < e0; ++i) sum0 += rg[(i & 7) * 256];
+        for (int i = s1; i < e1; ++i) sum1 += rg[(i & 7) * 256];
+        const int c0 = max(e0 - s0, 0), c1 = max(e1 - s1, 0);
+        if (c0) sum0 /= (float)c0;
+        if (c1) sum1 /= (float)c1;
+        float val = (c0 && c1) ? sum0 + (sum1 - sum0) * mu : 0.f;
+        if (a.rsl) val *= a.rsl[x];
+        a.out[(long)y * a.nr + x] = val;
     }
-    for (int i = s1; i < e1; ++i) {
-        const float2 v = col[(long)i * a.stride];
-        sum1 += v.x * v.x + v.y * v.y;
-        ++c1;
-    }
-    if (c0) sum0 /= (float)c0;
-    if (c1) sum1 /= (float)c1;
-    float val = (c0 && c1) ? sum0 + (sum1 - sum0) * mu : 0.f;
-    if (a.rsl) val *= a.rsl[x];
-    a.out[(long)y * a.nr + x] = val;
 }
 
 hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s) {
     if (a.out_rows <= 0 || a.nr <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_multilook, dim3((a.nr + 255) / 256, a.out_rows), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_multilook, dim3((a.nr + 255) / 256, (a.out_rows + kMlRows - 1) / kMlRows), dim3(256), 0, s,
+                       a);
     return hipGetLastError();
 }
 

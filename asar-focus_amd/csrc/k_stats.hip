@@ -145,29 +145,7 @@ hipError_t launch_reduce_f64(const double* partials, int nparts, int nvals, doub
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kThreads) void k_reduce_i64(const long long* partials, int nparts, int nvals,
-                                                        long long* totals) {
-    const int v = blockIdx.x;
-    long long acc = 0;
-    for (int p = threadIdx.x; p < nparts; p += kThreads) acc += partials[(long)p * nvals + v];
-    __shared__ long long sh[kThreads];
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) totals[v] = sh[0];
-}
-
-hipError_t launch_reduce_i64(const long long* partials, int nparts, int nvals, long long* totals, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_i64, dim3(nvals), dim3(kThreads), 0, s, partials, nparts, nvals, totals);
-    return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------------- raw moments ----
-// Moment slots: 0 SI, 1 SQ, 2 SII, 3 SQQ, 4 SIQ, 5 L_II, 6 L_QQ, 7 L_IQ (I_prev Q_cur), 8 L_QI (Q_prev I_cur),
-// 9 L_I1 (sum I_prev), 10 L_Q1, 11 L_I2 (sum I_cur), 12 L_Q2, 13 N, 14 M (lag pairs).
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -175,21 +153,22 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Workgroup (tile, strip): lane t owns column 256*tile + t over rows [128*strip, +128) and accumulates exact
+// integer sums; lag-1 products pair each row with the one above (the strip re-reads its preceding row).
+// Statistics are reduced over the workgroup; lag sums per Doppler block (slot = block - first block of the tile).
 template <typename P>
 __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
     const int tile = blockIdx.x;
     const int strip = blockIdx.y;
-    const int x0 = tile * a.step;
-    const int x1 = (tile == a.ntiles - 1) ? a.nr : min(x0 + a.step, a.nr);
+    const int x = tile * kThreads + threadIdx.x;
     const int y0 = strip * kStripRows;
     const int y1 = min(y0 + kStripRows, a.na);
     const P* raw = static_cast<const P*>(a.raw);
-    int acc[13];
-#pragma unroll
-    for (int i = 0; i < 13; ++i) acc[i] = 0;
+    int st[5] = {0, 0, 0, 0, 0};
+    int lg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int n = 0, m = 0;
-    constexpr int U = 8;  // rows loaded per iteration (independent loads in flight)
-    for (int x = x0 + threadIdx.x; x < x1; x += kThreads) {
+    if (x < a.nr) {
+        constexpr int U = 8;  // rows loaded per iteration (independent loads in flight)
         int pi = 0, pq = 0;
         bool have_prev = false;
         if (y0 > 0) {
@@ -206,21 +185,21 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
             for (int u = 0; u < U; ++u) {
                 if (y + u >= y1) break;
                 const int ci = c[u].x, cq = c[u].y;
-                acc[0] += ci;
-                acc[1] += cq;
-                acc[2] += ci * ci;
-                acc[3] += cq * cq;
-                acc[4] += ci * cq;
+                st[0] += ci;
+                st[1] += cq;
+                st[2] += ci * ci;
+                st[3] += cq * cq;
+                st[4] += ci * cq;
                 ++n;
                 if (have_prev) {
-                    acc[5] += pi * ci;
-                    acc[6] += pq * cq;
-                    acc[7] += pi * cq;
-                    acc[8] += pq * ci;
-                    acc[9] += pi;
-                    acc[10] += pq;
-                    acc[11] += ci;
-                    acc[12] += cq;
+                    lg[0] += pi * ci;
+                    lg[1] += pq * cq;
+                    lg[2] += pi * cq;
+                    lg[3] += pq * ci;
+                    lg[4] += pi;
+                    lg[5] += pq;
+                    lg[6] += ci;
+                    lg[7] += cq;
                     ++m;
                 }
                 pi = ci;
@@ -229,19 +208,40 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
             }
         }
     }
-    __shared__ long long sh[kThreads / 64][kRawMoments];
+    const int nval = kRawStats + a.nslots * kRawLag;
+    long long* out = a.partials + ((long)strip * a.ntiles + tile) * nval;
+    // statistics: wave sums, then across the 4 waves
+    __shared__ long long shs[kThreads / 64][kRawStats];
+    __shared__ long long shl[kRawLag][kThreads];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < kRawMoments; ++i) {
-        const long long v = wave_sum<long long>(i < 13 ? (long long)acc[i] : (i == 13 ? (long long)n : (long long)m));
-        if (lane == 0) sh[wave][i] = v;
+    for (int i = 0; i < kRawStats; ++i) {
+        const long long v = wave_sum<long long>(i < 5 ? (long long)st[i] : (long long)n);
+        if (lane == 0) shs[wave][i] = v;
     }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) shl[i][threadIdx.x] = lg[i];
+    shl[8][threadIdx.x] = m;
     __syncthreads();
-    if (threadIdx.x < kRawMoments) {
+    if (threadIdx.x < kRawStats) {
         long long t = 0;
 #pragma unroll
-        for (int w = 0; w < kThreads / 64; ++w) t += sh[w][threadIdx.x];
-        a.partials[((long)strip * a.ntiles + tile) * kRawMoments + threadIdx.x] = t;
+        for (int w = 0; w < kThreads / 64; ++w) t += shs[w][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+    // lag sums per block slot: lanes of one block form a contiguous range
+    const int c0 = tile * kThreads;
+    const int b0 = c0 / a.step;
+    for (int k = threadIdx.x; k < a.nslots * kRawLag; k += kThreads) {
+        const int slot = k / kRawLag, v = k % kRawLag;
+        const int b = b0 + slot;
+        long long t = 0;
+        if (b < a.nblocks) {
+            const int l0 = max(0, b * a.step - c0);
+            const int l1 = min(kThreads, (b + 1) * a.step - c0);
+            for (int l = l0; l < l1; ++l) t += shl[v][l];
+        }
+        out[kRawStats + k] = t;
     }
 }
 
@@ -253,6 +253,43 @@ hipError_t launch_raw_moments(const RawMomentsArgs& a, int fmt, hipStream_t s) {
         hipLaunchKernelGGL(k_raw_moments<char2>, grid, dim3(kThreads), 0, s, a);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// totals[v]: statistics summed over every (strip, tile); lag value k of block b summed over every strip and the
+// tiles whose slot range covers b. One workgroup per output value, fixed summation order.
+__global__ __launch_bounds__(kThreads) void k_reduce_raw(RawReduceArgs a) {
+    const int v = blockIdx.x;
+    const int nval = kRawStats + a.nslots * kRawLag;
+    long long acc = 0;
+    if (v < kRawStats) {
+        const long parts = (long)a.nstrips * a.ntiles;
+        for (long p = threadIdx.x; p < parts; p += kThreads) acc += a.partials[p * nval + v];
+    } else {
+        const int b = (v - kRawStats) / kRawLag, k = (v - kRawStats) % kRawLag;
+        // tiles t with b0(t) <= b < b0(t) + nslots
+        const int t_lo = max(0, (b - a.nslots + 1) * a.step / kThreads - 1);
+        const int t_hi = min(a.ntiles - 1, ((b + 1) * a.step) / kThreads + 1);
+        const int nt = t_hi - t_lo + 1;
+        const long parts = (long)a.nstrips * nt;
+        for (long p = threadIdx.x; p < parts; p += kThreads) {
+            const int strip = (int)(p / nt), t = t_lo + (int)(p % nt);
+            const int slot = b - (t * kThreads) / a.step;
+            if (slot >= 0 && slot < a.nslots) acc += a.partials[((long)strip * a.ntiles + t) * nval + kRawStats + slot * kRawLag + k];
+        }
+    }
+    __shared__ long long sh[kThreads];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.totals[v] = sh[0];
+}
+
+hipError_t launch_reduce_raw(const RawReduceArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_raw, dim3(kRawStats + a.nblocks * kRawLag), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -315,14 +352,9 @@ __device__ bool polyfit_wave(double x, double y, int n, double* out) {
 __global__ __launch_bounds__(64) void k_finalize(FinalizeArgs f) {
     const int lane = threadIdx.x;
     const long long* T = f.totals;
-    double st[6] = {0, 0, 0, 0, 0, 0};  // SI SQ SII SQQ SIQ N
-    for (int t = lane; t < f.ntiles; t += 64) {
+    double st[6];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) st[i] += (double)T[t * kRawMoments + i];
-        st[5] += (double)T[t * kRawMoments + 13];
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) st[i] = wave_sum(st[i]);
+    for (int i = 0; i < 6; ++i) st[i] = (double)T[i];
     const double SI = st[0], SQ = st[1], SII = st[2], SQQ = st[3], SIQ = st[4], nvalid = st[5];
     const double ntot = f.n_total;
     const double dci = SI / ntot, dcq = SQ / ntot;
@@ -339,10 +371,10 @@ __global__ __launch_bounds__(64) void k_finalize(FinalizeArgs f) {
     DevResults* r = f.res;
     double bx = 0.0, by = 0.0;
     if (lane < f.nblocks) {
-        const long long* t = T + lane * kRawMoments;
-        const double LII = (double)t[5], LQQ = (double)t[6], LIQ = (double)t[7], LQI = (double)t[8];
-        const double I1 = (double)t[9], Q1 = (double)t[10], I2 = (double)t[11], Q2 = (double)t[12];
-        const double M = (double)t[14];
+        const long long* t = T + kRawStats + lane * kRawLag;
+        const double LII = (double)t[0], LQQ = (double)t[1], LIQ = (double)t[2], LQI = (double)t[3];
+        const double I1 = (double)t[4], Q1 = (double)t[5], I2 = (double)t[6], Q2 = (double)t[7];
+        const double M = (double)t[8];
         const double uu = ca * ca * LII + ca * cb * (I1 + I2) + cb * cb * M;
         const double vv = cc * cc * LII + cc * cd * (LIQ + LQI) + cd * cd * LQQ + cc * ce * (I1 + I2) +
                           cd * ce * (Q1 + Q2) + ce * ce * M;

@@ -15,7 +15,8 @@ constexpr int kRowThreads = 512;   // row transforms: one row per workgroup
 constexpr int kColTile = 16;   // columns per workgroup in column transforms (16 x 8 B = one 128 B line)
 constexpr int kMaxE = 16;      // complex elements per lane in a Stockham pass
 constexpr int kStripRows = 128; // rows per workgroup in the raw statistics pass
-constexpr int kRawMoments = 15; // values per (strip, tile) in the raw statistics pass
+constexpr int kRawStats = 6;    // SI, SQ, SII, SQQ, SIQ, N
+constexpr int kRawLag = 9;      // LII, LQQ, LIQ, LQI, LI1, LQ1, LI2, LQ2, M per Doppler block
 constexpr int kMaxDcBlocks = 64;
 
 enum RawFormat { RAW_CF32 = 0, RAW_U8 = 1, RAW_I8 = 2 };
@@ -104,14 +105,22 @@ struct RawMomentsArgs {
     const void* raw;
     int nr, na;
     int step;         // DC block width
-    int ntiles;       // DC blocks (+1 remainder tile when blocks do not cover the row)
+    int nblocks;      // Doppler blocks (columns at or beyond nblocks*step only enter the statistics)
+    int ntiles;       // 256-column tiles
+    int nslots;       // Doppler blocks a tile can touch: 255/step + 2
     int nstrips;
-    long long* partials; // [strip][tile][kRawMoments]
+    long long* partials; // [strip][tile][kRawStats + nslots*kRawLag]
+};
+
+struct RawReduceArgs {
+    const long long* partials;
+    int nstrips, ntiles, nslots, step, nblocks;
+    long long* totals;  // [kRawStats + nblocks*kRawLag]
 };
 
 struct FinalizeArgs {
-    const long long* totals; // [tile][kRawMoments]
-    int ntiles, nblocks, step, nr, na, x_stride;
+    const long long* totals; // [kRawStats + nblocks*kRawLag]
+    int nblocks, step, nr, na, x_stride;
     double n_total;
     int quirks;
     double prf;
@@ -173,7 +182,7 @@ hipError_t launch_dc_cf32(const float2* img, long stride, int ys, int x_begin0, 
                           int nstrips, double* partials, hipStream_t s);
 hipError_t launch_reduce_f64(const double* partials, int nparts, int nvals, double* totals, hipStream_t s);
 hipError_t launch_raw_moments(const RawMomentsArgs& a, int raw_format, hipStream_t s);
-hipError_t launch_reduce_i64(const long long* partials, int nparts, int nvals, long long* totals, hipStream_t s);
+hipError_t launch_reduce_raw(const RawReduceArgs& a, hipStream_t s);
 hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
 hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s);
 hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_t s);

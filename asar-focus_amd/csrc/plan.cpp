@@ -43,7 +43,7 @@ struct asarf_plan {
     asarf_focus_params p{};
     int nrp = 0, nap = 0, nrv = 0;
     long stride = 0;
-    int ntiles = 0, nblocks = 0, step = 0, nstrips = 0;
+    int ntiles = 0, nblocks = 0, step = 0, nstrips = 0, nslots = 0;
     const FftPlanHost* rplan = nullptr;
     ColSplit cs;
     // device buffers
@@ -123,7 +123,8 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     pl->stride = (nrv + kColTile - 1) / kColTile * kColTile;
     pl->step = step;
     pl->nblocks = nblocks;
-    pl->ntiles = nblocks + ((nblocks * step < p->nr) ? 1 : 0);
+    pl->ntiles = (p->nr + kThreads - 1) / kThreads;
+    pl->nslots = (kThreads - 1) / step + 2;
     pl->nstrips = (p->na + kStripRows - 1) / kStripRows;
     auto fail = [&](int st) {
         plan_free(pl);
@@ -139,9 +140,9 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     if ((st = plan_alloc(pl, (void**)&pl->hconj, sizeof(float2) * (size_t)nrp))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->col, sizeof(ColConst) * (size_t)nrv))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->partials,
-                         sizeof(long long) * (size_t)pl->nstrips * pl->ntiles * kRawMoments)))
+                         sizeof(long long) * (size_t)pl->nstrips * pl->ntiles * (kRawStats + pl->nslots * kRawLag))))
         return fail(st);
-    if ((st = plan_alloc(pl, (void**)&pl->totals, sizeof(long long) * (size_t)pl->ntiles * kRawMoments)))
+    if ((st = plan_alloc(pl, (void**)&pl->totals, sizeof(long long) * (size_t)(kRawStats + nblocks * kRawLag))))
         return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->res, sizeof(DevResults)))) return fail(st);
 
@@ -247,16 +248,25 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
     rm.nr = p.nr;
     rm.na = p.na;
     rm.step = pl->step;
+    rm.nblocks = pl->nblocks;
     rm.ntiles = pl->ntiles;
+    rm.nslots = pl->nslots;
     rm.nstrips = pl->nstrips;
     rm.partials = pl->partials;
     HIP_CHECK(launch_raw_moments(rm, p.raw_format, s));
     ST_CHECK(mark(pl, "raw_moments"));
-    HIP_CHECK(launch_reduce_i64(pl->partials, pl->nstrips, pl->ntiles * kRawMoments, pl->totals, s));
+    RawReduceArgs rr{};
+    rr.partials = pl->partials;
+    rr.nstrips = pl->nstrips;
+    rr.ntiles = pl->ntiles;
+    rr.nslots = pl->nslots;
+    rr.step = pl->step;
+    rr.nblocks = pl->nblocks;
+    rr.totals = pl->totals;
+    HIP_CHECK(launch_reduce_raw(rr, s));
     ST_CHECK(mark(pl, "reduce"));
     FinalizeArgs fa{};
     fa.totals = pl->totals;
-    fa.ntiles = pl->ntiles;
     fa.nblocks = pl->nblocks;
     fa.step = pl->step;
     fa.nr = p.nr;
