@@ -72,7 +72,8 @@ class FocusResults(ctypes.Structure):
     _fields_ = [("iq", IqResults), ("dc_poly", ctypes.c_double * 3), ("n_blocks", ctypes.c_int),
                 ("block_freq", ctypes.c_double * 64), ("block_x", ctypes.c_double * 64), ("nrv", ctypes.c_int),
                 ("crop_front", ctypes.c_int), ("crop_back", ctypes.c_int), ("crop_near", ctypes.c_int),
-                ("crop_far", ctypes.c_int), ("imp_rows", ctypes.c_int)]
+                ("crop_far", ctypes.c_int), ("imp_rows", ctypes.c_int),
+                ("azimuth_core_fused", ctypes.c_int), ("rcmc_shift_bound", ctypes.c_int)]
 
 
 _vp, _ci, _cd, _cf, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t

@@ -240,7 +240,8 @@ class Plan:
         nb = r.n_blocks
         return {"iq": r.iq.as_dict(), "dc_poly": list(r.dc_poly), "n_blocks": nb,
                 "block_freq": list(r.block_freq[:nb]), "block_x": list(r.block_x[:nb]), "nrv": r.nrv,
-                "crop": (r.crop_front, r.crop_back, r.crop_near, r.crop_far), "imp_rows": r.imp_rows}
+                "crop": (r.crop_front, r.crop_back, r.crop_near, r.crop_far), "imp_rows": r.imp_rows,
+                "azimuth_core_fused": bool(r.azimuth_core_fused), "rcmc_shift_bound": r.rcmc_shift_bound}
 
     def set_timing(self, on: bool):
         L.check(L.lib().asarf_plan_set_timing(self.h, int(on)))

@@ -204,13 +204,13 @@ const FftPlanHost* get_plan(asarf_ctx* c, int n, int lanes, int* status) {
     return raw;
 }
 
-int get_col_split(asarf_ctx* c, int n, ColSplit* cs) {
+int get_col_split(asarf_ctx* c, int n, ColSplit* cs, int n2_max) {
     const int lanes = kThreads / kColTile;
     const size_t lds_cap = c->max_lds / (sizeof(float2) * (kColTile + 1));
     std::vector<int> tmp;
     cs->n = n;
     int st = ASARF_OK;
-    if ((size_t)n <= lds_cap && radix_plan_relaxed(n, lanes, tmp)) {
+    if (n2_max <= 0 && (size_t)n <= lds_cap && radix_plan_relaxed(n, lanes, tmp)) {
         cs->n1 = n;
         cs->n2 = 1;
         cs->p1 = get_plan(c, n, lanes, &st);
@@ -218,14 +218,23 @@ int get_col_split(asarf_ctx* c, int n, ColSplit* cs) {
         cs->full = nullptr;
         return cs->p1 ? ASARF_OK : st;
     }
+    // n2_max <= 0: the most balanced split (smallest max(n1, n2)); otherwise the largest n2 <= n2_max
     int best1 = 0, best2 = 0;
     for (int n1 = 2; n1 <= n / 2; ++n1) {
         if (n % n1) continue;
         const int n2 = n / n1;
         if ((size_t)n1 > lds_cap || (size_t)n2 > lds_cap) continue;
+        if (n2_max > 0 && n2 > n2_max) continue;
         if (!radix_plan_relaxed(n1, lanes, tmp) || !radix_plan_relaxed(n2, lanes, tmp)) continue;
-        if (!best1 || std::max(n1, n2) < std::max(best1, best2) ||
-            (std::max(n1, n2) == std::max(best1, best2) && n1 < best1)) {
+        bool better;
+        if (!best1)
+            better = true;
+        else if (n2_max > 0)
+            better = n2 > best2;
+        else
+            better = std::max(n1, n2) < std::max(best1, best2) ||
+                     (std::max(n1, n2) == std::max(best1, best2) && n1 < best1);
+        if (better) {
             best1 = n1;
             best2 = n2;
         }
@@ -237,15 +246,18 @@ int get_col_split(asarf_ctx* c, int n, ColSplit* cs) {
     if (!cs->p1) return st;
     cs->p2 = get_plan(c, best2, lanes, &st);
     if (!cs->p2) return st;
-    FftPlanHost* full = new FftPlanHost();
-    full->n = n;
-    full->d_tw = get_twiddles(c, n, &st);
-    if (!full->d_tw) {
-        delete full;
-        return st;
+    // the inter-pass twiddle table of length n, cached with the plans (entries are never replaced, so splits
+    // handed out earlier stay valid)
+    const auto key = std::make_pair(n, -1);
+    auto it = c->plans.find(key);
+    if (it == c->plans.end()) {
+        auto full = std::make_unique<FftPlanHost>();
+        full->n = n;
+        full->d_tw = get_twiddles(c, n, &st);
+        if (!full->d_tw) return st;
+        it = c->plans.emplace(key, std::move(full)).first;
     }
-    c->plans[std::make_pair(n, -1)] = std::unique_ptr<FftPlanHost>(full);
-    cs->full = full;
+    cs->full = it->second.get();
     return ASARF_OK;
 }
 }  // namespace asarf
@@ -553,7 +565,7 @@ extern "C" int asarf_range_compression(asarf_ctx* c, asarf_image* img, const flo
     return clear_error();
 }
 
-namespace {
+namespace asarf {
 ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stride, int ncols, int groups, long in_g,
                     long in_m, int in_rows, long out_g, long out_m, int out_rows, const float2* twf, float scale,
                     const FftPlanHost* sub) {
@@ -576,7 +588,7 @@ ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stri
     a.sub = sub->dev();
     return a;
 }
-}  // namespace
+}  // namespace asarf
 
 namespace asarf {
 // Forward column transform into permuted storage: row k1*n2 + k2 holds bin k1 + n1*k2. pass 0 = A (strided

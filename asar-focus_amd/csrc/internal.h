@@ -68,6 +68,11 @@ int ctx_scratch(asarf_ctx* ctx, size_t bytes, void** out);
 int ctx_pinned(asarf_ctx* ctx, size_t bytes, void** out);
 const FftPlanHost* get_plan(asarf_ctx* ctx, int n, int lanes, int* status);
 float2* get_twiddles(asarf_ctx* ctx, int n, int* status);
-int get_col_split(asarf_ctx* ctx, int n, ColSplit* out);
+int get_col_split(asarf_ctx* ctx, int n, ColSplit* out, int n2_max = 0);
+// Column-transform launch arguments: element m of group g is read from row g*in_g + m*in_m, element k written
+// to row g*out_g + k*out_m; twf (nullptr = none) is the inter-pass twiddle table.
+ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stride, int ncols, int groups, long in_g,
+                    long in_m, int in_rows, long out_g, long out_m, int out_rows, const float2* twf, float scale,
+                    const FftPlanHost* sub);
 bool radix_plan(int n, int lanes, std::vector<int>& out);
 }  // namespace asarf

@@ -201,6 +201,7 @@ hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
 template <bool INV, int RSET>
 __global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
     extern __shared__ float2 lds[];
+    if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
     const int tile = blockIdx.x % a.ntiles;
     const int g = blockIdx.x / a.ntiles;
     const int b = threadIdx.x & (kColTile - 1);

@@ -15,30 +15,47 @@ namespace asarf {
 __constant__ float c_window[16];
 static bool g_window_ready = false;
 
-static void ensure_window(hipStream_t s) {
-    if (g_window_ready) return;
-    float w[16];
+void rcmc_window(float w[16]) {
     for (int i = 0; i < 16; ++i) {
         const double a = 2.0 * kPi * (double)i / 15.0;
         w[i] = (float)(0.42 - 0.5 * std::cos(a) + 0.08 * std::cos(2.0 * a));
     }
+}
+
+static void ensure_window(hipStream_t s) {
+    if (g_window_ready) return;
+    float w[16];
+    rcmc_window(w);
     (void)hipMemcpyToSymbol(HIP_SYMBOL(c_window), w, sizeof(w), 0, hipMemcpyHostToDevice);
     g_window_ready = true;
 }
 
 __global__ void k_col_const(ColConstArgs a) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= a.nrv) return;
-    const double step = a.prf / (double)a.nbins;
-    const double dc = polyval(a.dc_poly, 3, (double)x);
-    const double r0 = a.r_first + (double)x * a.range_spacing;
-    const double vr = polyval(a.vr_poly, 3, (double)x);
-    ColConst c;
-    c.dc = dc;
-    c.dc_off = round(dc / step) * step;
-    c.shift_k = a.wavelength * a.wavelength * r0 / (8.0 * vr * vr * a.range_spacing);
-    c.inv_ka = a.wavelength * r0 / (2.0 * vr * vr);
-    a.col[x] = c;
+    const bool valid = x < a.nrv;
+    int r = 0;
+    if (valid) {
+        const double step = a.prf / (double)a.nbins;
+        const double dc = polyval(a.dc_poly, 3, (double)x);
+        const double r0 = a.r_first + (double)x * a.range_spacing;
+        const double vr = polyval(a.vr_poly, 3, (double)x);
+        ColConst c;
+        c.dc = dc;
+        c.dc_off = round(dc / step) * step;
+        c.shift_k = a.wavelength * a.wavelength * r0 / (8.0 * vr * vr * a.range_spacing);
+        c.inv_ka = a.wavelength * r0 / (2.0 * vr * vr);
+        a.col[x] = c;
+        // The aliased bin frequency stays within prf/2 of the centroid, so |f| <= prf/2 + |dc| bounds the
+        // shift of every bin (plus one pixel against rounding).
+        const double fmax = (0.5 * a.prf + fabs(dc)) * (1.0 + 1e-12);
+        const double smax = c.shift_k * fmax * fmax;
+        r = (smax >= 0.0 && smax < 1e6) ? (int)floor(smax + 0.5) + 1 : 1000000;
+    }
+    if (a.reach) {
+        // one atomic per wave; every lane of the wave takes part in the reduction
+        for (int o = 32; o > 0; o >>= 1) r = max(r, __shfl_xor(r, o));
+        if ((threadIdx.x & 63) == 0 && r > 0) atomicMax(a.reach, r);
+    }
 }
 
 hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s) {
@@ -59,6 +76,7 @@ __device__ __forceinline__ double alias_about_dc(double f, double dc, double prf
 // One workgroup per frequency row (in place). The row is staged in LDS so every output can read its 16 taps.
 __global__ __launch_bounds__(kThreads) void k_rcmc_mf(RcmcArgs a) {
     extern __shared__ float2 row[];
+    if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
     const int q = blockIdx.x;
     float2* img = a.img + (long)q * a.stride;
     const int nrv = a.nrv;

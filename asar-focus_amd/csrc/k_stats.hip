@@ -417,6 +417,7 @@ __global__ __launch_bounds__(64) void k_finalize(FinalizeArgs f) {
         const int in_rows = pos ? f.imp_in_rows_pos : f.imp_in_rows_neg;
         r->imp_rows = (f.ml_ratio > 0.f) ? (int)roundf((float)in_rows / f.ml_ratio) : 0;
         r->pad = in_rows;
+        r->rcmc_reach = 0;
     }
 }
 

@@ -18,6 +18,12 @@ constexpr int kStripRows = 128; // rows per workgroup in the raw statistics pass
 constexpr int kRawStats = 6;    // SI, SQ, SII, SQQ, SIQ, N
 constexpr int kRawLag = 9;      // LII, LQQ, LIQ, LQI, LI1, LQ1, LI2, LQ2, M per Doppler block
 constexpr int kMaxDcBlocks = 64;
+// Largest integer RCMC shift (pixels) the fused azimuth kernel's range halo covers; above it the plan runs the
+// separate RCMC/matched-filter and inverse-pass kernels instead.
+constexpr int kAzFusedReach = 8;
+// Default upper bound on the second-pass length of the column split when the fused azimuth kernel runs (its
+// LDS tile is n2 x 41 complex samples, so a shorter second pass fits more workgroups per CU); 0 = balanced.
+constexpr int kAzFusedN2Max = 0;
 
 enum RawFormat { RAW_CF32 = 0, RAW_U8 = 1, RAW_I8 = 2 };
 
@@ -37,6 +43,7 @@ struct DevResults {
     int crop_front;  // lines removed at the start of the SLC for the IMP tail
     int imp_rows;    // rows of the multilooked product
     int pad;
+    int rcmc_reach;  // bound on the integer RCMC shift over all columns and Doppler bins (pixels)
     IqCoef coef;
 };
 
@@ -87,6 +94,7 @@ struct ColFftArgs {
     const float2* tw_full; // post-twiddle W_full^{g*k} (conjugated for the inverse); nullptr = none
     float scale;
     FftPlanDev sub;
+    const int* fused_gate; // non-null: do nothing when *fused_gate <= kAzFusedReach (the fused kernel ran)
 };
 
 struct RcmcArgs {
@@ -99,6 +107,25 @@ struct RcmcArgs {
     double prf;
     double beta;      // azimuth bandwidth fraction
     int mode;         // 0 = RCMC + matched filter, 1 = RCMC only, 2 = matched filter only
+    const int* fused_gate;  // non-null: do nothing when *fused_gate <= kAzFusedReach (the fused kernel ran)
+};
+
+// Fused RCMC + matched filter + inverse pass B' (out of place; four-step split n1 x n2).
+struct AzFusedArgs {
+    const float2* in;     // pass-B output: row k1*n2 + k2 holds bin k1 + n1*k2
+    float2* out;          // inverse pass-B' output with the inverse inter-pass twiddle applied
+    long stride;
+    int nrv;
+    int nbins;            // n1 * n2
+    int n1;
+    int ntiles;           // ceil(nrv / 16)
+    int items;            // n1 * ntiles
+    const ColConst* col;
+    double prf, beta;
+    const float2* tw_full; // W_nbins^m
+    FftPlanDev sub;       // length-n2 plan
+    const int* gate;      // non-null: do nothing when *gate > kAzFusedReach
+    float window[16];     // Blackman window of the RCMC interpolator
 };
 
 struct RawMomentsArgs {
@@ -138,6 +165,7 @@ struct ColConstArgs {
     double vr_poly[3];
     double wavelength, range_spacing, r_first, prf;
     int nbins;
+    int* reach;             // non-null: atomic max of the per-column RCMC shift bound (cleared by finalize)
 };
 
 struct MultilookArgs {
@@ -173,6 +201,9 @@ hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s)
 bool launch_range_compress_static(const RcArgs& a, int raw_format, hipStream_t s, hipError_t* err);
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
+hipError_t launch_az_fused(const AzFusedArgs& a, hipStream_t s);
+size_t az_fused_lds_bytes(int n2);
+void rcmc_window(float w[16]);
 hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s);
 hipError_t launch_moments_cf32(const float2* img, long stride, int xs, int ys, int xstr, double* partials,
                                int nwg, hipStream_t s);

@@ -214,6 +214,10 @@ typedef struct {
     int nrv;                 /* SLC range samples */
     int crop_front, crop_back, crop_near, crop_far; /* results window (main_flow.cc:155-201) */
     int imp_rows;
+    /* Azimuth core of the last execute: 1 = fused forward pass / RCMC / matched filter / inverse pass kernel,
+     * 0 = separate kernels (no second column pass, tile beyond LDS, or RCMC shift bound above the halo). */
+    int azimuth_core_fused;
+    int rcmc_shift_bound; /* device bound on the integer RCMC shift (pixels); 0 when not computed */
 } asarf_focus_results;
 
 typedef struct asarf_plan asarf_plan;

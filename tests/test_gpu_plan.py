@@ -86,3 +86,47 @@ def test_plan_imp_matches_oracle_chain(ctx):
     err = float(np.sqrt(np.sum((got - gr) ** 2) / np.sum(gr ** 2)))
     print(f"imp rel rms {err:.3e}")
     assert err < 2e-4
+
+
+def run_plan_env(ctx, cfg, env):
+    """run_plan with environment overrides that the plan reads at creation."""
+    import os
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return run_plan(ctx, cfg)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("which", ["ers", "asar"])
+def test_plan_fused_azimuth_core_matches_separate_kernels(ctx, which):
+    """The fused pass-B/RCMC/matched-filter/pass-B' kernel against the separate kernels on the same raw data."""
+    if which == "ers":
+        cfg = configs.ers_block().scaled(na=700, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6))
+    else:
+        cfg = configs.asar_scene().scaled(na=640, nr=1400, n_targets_random=6)
+    _, slc_f, res_f, _ = run_plan_env(ctx, cfg, {"ASARF_AZ_FUSED": "1"})
+    _, slc_s, res_s, _ = run_plan_env(ctx, cfg, {"ASARF_AZ_FUSED": "0"})
+    assert res_f["azimuth_core_fused"] and not res_s["azimuth_core_fused"]
+    assert 0 < res_f["rcmc_shift_bound"] <= 8
+    err = rel_rms(slc_f, slc_s)
+    print(f"{which} fused vs separate rel rms {err:.3e}")
+    assert err < 1e-6
+
+
+def test_plan_large_migration_takes_separate_path(ctx):
+    """L-band wavelength: the RCMC shift bound exceeds the fused tile's halo, the device-side gate selects the
+    separate kernels, and the SLC still matches the oracle."""
+    cfg = configs.ers_block().scaled(na=700, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6),
+                                     wavelength=0.236)
+    raw, slc, res, _ = run_plan_env(ctx, cfg, {"ASARF_AZ_FUSED": "1"})
+    assert not res["azimuth_core_fused"] and res["rcmc_shift_bound"] > 8
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    err = rel_rms(slc, ref)
+    print(f"large migration slc rel rms {err:.3e} (shift bound {res['rcmc_shift_bound']})")
+    assert err < SLC_TOL
