@@ -116,6 +116,16 @@ SIGNATURES = {
     "asarf_plan_results": (_ci, [_vp, _P(FocusResults)]),
     "asarf_plan_set_timing": (_ci, [_vp, _ci]),
     "asarf_plan_kernel_times": (_ci, [_vp, _P(ctypes.c_char_p), _P(_cf), _ci, _P(_ci)]),
+    "asarf_rcmc_net_create": (_ci, [_vp, ctypes.c_ulonglong, _P(_vp)]),
+    "asarf_rcmc_net_destroy": (_ci, [_vp]),
+    "asarf_rcmc_net_num_params": (_ci, []),
+    "asarf_rcmc_net_get_params": (_ci, [_vp, _P(_cf), _ci]),
+    "asarf_rcmc_net_set_params": (_ci, [_vp, _P(_cf), _ci]),
+    "asarf_rcmc_net_grad": (_ci, [_vp, _vp, ctypes.c_long, _P(_cf), _P(_cd)]),
+    "asarf_rcmc_net_train": (_ci, [_vp, _vp, ctypes.c_long, _ci, _ci, _cf, _P(_cf)]),
+    "asarf_rcmc_net_build_table": (_ci, [_vp, _vp, ctypes.c_long, _ci, _ci, _ci, _cf, _P(_cf), _P(_cf), _P(_ci)]),
+    "asarf_plan_rcmc_pairs": (_ci, [_vp, _vp, ctypes.c_long, ctypes.c_ulonglong, _vp]),
+    "asarf_plan_set_rcmc": (_ci, [_vp, _ci, _vp]),
 }
 
 _LIB = None

@@ -227,6 +227,17 @@ class Plan:
                                            self.imp_out.data_ptr() if self.imp_out is not None else None))
         return self.slc
 
+    def rcmc_pairs(self, raw: torch.Tensor, n: int, seed: int) -> torch.Tensor:
+        """n training pairs (n x 20 float32, device) for the learned RCMC interpolator from this scene."""
+        assert raw.is_cuda and raw.is_contiguous() and raw.numel() == self.cfg.na * self.cfg.nr * 2
+        out = torch.empty((n, RCMC_PAIR_FLOATS), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        L.check(L.lib().asarf_plan_rcmc_pairs(self.h, raw.data_ptr(), n, seed, out.data_ptr()))
+        return out
+
+    def set_rcmc(self, interpolator: int, net: "RcmcNet | None" = None):
+        L.check(L.lib().asarf_plan_set_rcmc(self.h, interpolator, net.h if net is not None else None))
+        self._net = net  # keep the table alive while the plan uses it
+
     def slc_tensor(self) -> torch.Tensor:
         """Copy of the focused SLC (na x nrv complex64)."""
         out = torch.empty((self.slc.y_size, self.slc.x_size), dtype=torch.complex64,
@@ -256,6 +267,67 @@ class Plan:
     def close(self):
         if self.h:
             L.lib().asarf_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+RCMC_PAIR_FLOATS = 20
+RCMC_SINC16, RCMC_LEARNED, RCMC_SINC8 = 0, 1, 2
+
+
+class RcmcNet:
+    """Learned RCMC interpolator (network, trainer, tap table) on one context."""
+
+    def __init__(self, ctx: Context, seed: int = 1):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        L.check(L.lib().asarf_rcmc_net_create(ctx.h, seed, ctypes.byref(h)))
+        self.h = h
+        self.n_params = L.lib().asarf_rcmc_net_num_params()
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.n_params, np.float32)
+        L.check(L.lib().asarf_rcmc_net_get_params(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                  self.n_params))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        L.check(L.lib().asarf_rcmc_net_set_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                  self.n_params))
+
+    def grad(self, pairs: torch.Tensor):
+        g = np.zeros(self.n_params, np.float32)
+        loss = ctypes.c_double()
+        L.check(L.lib().asarf_rcmc_net_grad(self.h, pairs.data_ptr(), pairs.shape[0],
+                                            g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, pairs: torch.Tensor, batch: int, steps: int, lr: float) -> np.ndarray:
+        hist = np.zeros(steps, np.float32)
+        L.check(L.lib().asarf_rcmc_net_train(self.h, pairs.data_ptr(), pairs.shape[0], batch, steps, lr,
+                                             hist.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return hist
+
+    def build_table(self, val_pairs: torch.Tensor, nd=8192, ns=2, judge_nd=256, threshold=1e-3):
+        nj = judge_nd * ns
+        rms = np.zeros(nj, np.float32)
+        rms8 = np.zeros(nj, np.float32)
+        nok = ctypes.c_int()
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.check(L.lib().asarf_rcmc_net_build_table(self.h, val_pairs.data_ptr(), val_pairs.shape[0], nd, ns, judge_nd,
+                                                   threshold, rms.ctypes.data_as(fp), rms8.ctypes.data_as(fp),
+                                                   ctypes.byref(nok)))
+        return {"rms": rms.reshape(ns, judge_nd), "sinc8_rms": rms8.reshape(ns, judge_nd), "n_ok": nok.value}
+
+    def close(self):
+        if self.h:
+            L.lib().asarf_rcmc_net_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -76,3 +76,23 @@ ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stri
                     const FftPlanHost* sub);
 bool radix_plan(int n, int lanes, std::vector<int>& out);
 }  // namespace asarf
+
+// Learned RCMC interpolator state (device buffers; the host API is in rcmc_net.cpp).
+struct asarf_rcmc_net {
+    asarf_ctx* ctx = nullptr;
+    float* master = nullptr;  // FP32 parameters
+    float* m = nullptr;       // Adam moments
+    float* v = nullptr;
+    float* wq = nullptr;      // bf16-rounded copy used by the forward pass
+    float* partials = nullptr;
+    float* loss_part = nullptr;
+    float* grad = nullptr;
+    long groups_cap = 0;
+    int step = 0;
+    // tap table and judge
+    float2* lut = nullptr;
+    unsigned char* lut_ok = nullptr;
+    float* judge = nullptr;   // err_sum | count | sinc8_err_sum, each ns * judge_nd
+    int nd = 0, ns = 0, judge_nd = 0;
+    float window8[8] = {};
+};

@@ -8,6 +8,7 @@
 // before the single-precision sincospi).
 #include "kernels.h"
 #include "poly.h"
+#include "rcmc_net.h"
 
 namespace asarf {
 
@@ -101,24 +102,65 @@ __global__ __launch_bounds__(kThreads) void k_rcmc_mf(RcmcArgs a) {
                 val = make_float2(0.f, 0.f);
             } else {
                 const float d = (float)(shift - p);
-                if (d == 0.0f) {
-                    val = row[cx];
-                } else {
-                    float2 acc = make_float2(0.f, 0.f);
-                    float den = 0.f;
-                    const float2* tap = row + cx - 8;
+                bool sinc16 = a.interp == 0;
+                if (a.interp == 1) {
+                    // learned taps where the judge accepted the cell, else the 16-tap sinc
+                    const int cell = net_cell(d, net_slope(cc.shift_k, fn, a.prf), a.lut_nd, a.lut_ns);
+                    const int jc = (cell / a.lut_nd) * a.judge_nd + (cell % a.lut_nd) * a.judge_nd / a.lut_nd;
+                    if (a.lut_ok[jc]) {
+                        const float2* w8 = a.lut + (long)cell * kNetTaps;
+                        const float2* tap = row + cx - 4;
+                        float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        // d + (8 - i) is exact to one rounding; the 1-ulp reciprocal error cancels in the
-                        // normalisation to first order
-                        const float wi = w[i] * __builtin_amdgcn_rcpf(d + (float)(8 - i));
-                        const float2 t = tap[i];
-                        acc.x += wi * t.x;
-                        acc.y += wi * t.y;
-                        den += wi;
+                        for (int i = 0; i < kNetTaps; ++i) {
+                            const float2 wi = w8[i], t = tap[i];
+                            acc.x += wi.x * t.x - wi.y * t.y;
+                            acc.y += wi.x * t.y + wi.y * t.x;
+                        }
+                        val = acc;
+                    } else {
+                        sinc16 = true;
                     }
-                    const float inv = __builtin_amdgcn_rcpf(den);
-                    val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
+                } else if (a.interp == 2) {
+                    if (d == 0.0f) {
+                        val = row[cx];
+                    } else {
+                        float2 acc = make_float2(0.f, 0.f);
+                        float den = 0.f;
+                        const float2* tap = row + cx - 4;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const float wi =
+                                ((i & 1) ? -a.window8[i] : a.window8[i]) * __builtin_amdgcn_rcpf(d + (float)(4 - i));
+                            const float2 t = tap[i];
+                            acc.x += wi * t.x;
+                            acc.y += wi * t.y;
+                            den += wi;
+                        }
+                        const float inv = __builtin_amdgcn_rcpf(den);
+                        val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
+                    }
+                }
+                if (sinc16) {
+                    if (d == 0.0f) {
+                        val = row[cx];
+                    } else {
+                        float2 acc = make_float2(0.f, 0.f);
+                        float den = 0.f;
+                        const float2* tap = row + cx - 8;
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) {
+                            // d + (8 - i) is exact to one rounding; the 1-ulp reciprocal error cancels in the
+                            // normalisation to first order
+                            const float wi = w[i] * __builtin_amdgcn_rcpf(d + (float)(8 - i));
+                            const float2 t = tap[i];
+                            acc.x += wi * t.x;
+                            acc.y += wi * t.y;
+                            den += wi;
+                        }
+                        const float inv = __builtin_amdgcn_rcpf(den);
+                        val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
+                    }
                 }
             }
         } else {

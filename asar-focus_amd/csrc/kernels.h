@@ -108,6 +108,13 @@ struct RcmcArgs {
     double beta;      // azimuth bandwidth fraction
     int mode;         // 0 = RCMC + matched filter, 1 = RCMC only, 2 = matched filter only
     const int* fused_gate;  // non-null: do nothing when *fused_gate <= kAzFusedReach (the fused kernel ran)
+    // interpolator: 0 = 16-tap Blackman sinc (the reference's), 1 = learned 8-tap table where its judge cell
+    // passed (16-tap sinc elsewhere), 2 = plain 8-tap Blackman sinc
+    int interp;
+    const float2* lut;            // [ns][nd][8] complex taps
+    const unsigned char* lut_ok;  // [ns][judge_nd]
+    int lut_nd, lut_ns, judge_nd;
+    float window8[8];
 };
 
 // Fused RCMC + matched filter + inverse pass B' (out of place; four-step split n1 x n2).

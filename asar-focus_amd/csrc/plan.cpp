@@ -16,6 +16,7 @@
 #include "asarf.h"
 #include "internal.h"
 #include "poly.h"
+#include "rcmc_net.h"
 
 using namespace asarf;
 
@@ -63,6 +64,9 @@ struct asarf_plan {
     int near = 8, far = 8, nr_s = 0;
     float ml_ratio = 0.f;
     int imp_cap = 0;
+    // RCMC interpolator (0 = 16-tap sinc, 1 = learned table with sinc fallback, 2 = 8-tap sinc)
+    int rcmc_interp = 0;
+    const asarf_rcmc_net* net = nullptr;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -253,14 +257,12 @@ static int mark(asarf_plan* pl, const char* name) {
     return ASARF_OK;
 }
 
-extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out) {
-    if (!pl || !raw || !slc) return set_error(ASARF_EINVAL, "asarf_plan_execute: null argument");
-    if (pl->p.imp && !imp_out) return set_error(ASARF_EINVAL, "asarf_plan_execute: IMP output required");
+// Statistics, Doppler, column constants, range compression and the forward azimuth transform (both passes):
+// on return pl->img holds the range-Doppler image in permuted bin order.
+static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     asarf_ctx* c = pl->ctx;
     const asarf_focus_params& p = pl->p;
     hipStream_t s = c->stream;
-    pl->ev_names.clear();
-    ST_CHECK(mark(pl, "start"));
     // 1. one pass over the raw bytes: I/Q and Doppler-block moments
     RawMomentsArgs rm{};
     rm.raw = raw;
@@ -333,6 +335,22 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
     // 3. azimuth FFT (permuted bins), rows >= Na read as zero
     ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
     ST_CHECK(mark(pl, "azimuth_fft_a"));
+    if (pl->cs.n2 > 1) {
+        ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
+        ST_CHECK(mark(pl, "azimuth_fft_b"));
+    }
+    return ASARF_OK;
+}
+
+extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out) {
+    if (!pl || !raw || !slc) return set_error(ASARF_EINVAL, "asarf_plan_execute: null argument");
+    if (pl->p.imp && !imp_out) return set_error(ASARF_EINVAL, "asarf_plan_execute: IMP output required");
+    asarf_ctx* c = pl->ctx;
+    const asarf_focus_params& p = pl->p;
+    hipStream_t s = c->stream;
+    pl->ev_names.clear();
+    ST_CHECK(mark(pl, "start"));
+    ST_CHECK(enqueue_to_spectrum(pl, raw));
     RcmcArgs rc{};
     rc.img = pl->img;
     rc.stride = pl->stride;
@@ -344,11 +362,18 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
     rc.prf = p.prf;
     rc.beta = p.az_bandwidth_fraction;
     rc.mode = 0;
-    if (pl->img2) {
+    rc.interp = pl->rcmc_interp;
+    if (pl->net) {
+        rc.lut = pl->net->lut;
+        rc.lut_ok = pl->net->lut_ok;
+        rc.lut_nd = pl->net->nd;
+        rc.lut_ns = pl->net->ns;
+        rc.judge_nd = pl->net->judge_nd;
+    }
+    for (int i = 0; i < 8; ++i) rc.window8[i] = blackman_value(i, 8);
+    if (pl->img2 && pl->rcmc_interp == 0) {
         const ColSplit& cs = pl->cs;
         const int n = cs.n;
-        ST_CHECK(col_forward_pass(c, cs, 1, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
-        ST_CHECK(mark(pl, "azimuth_fft_b"));
         // 4. fused RCMC + matched filter + inverse pass B' (img -> img2) when the RCMC shift bound computed on
         // the device fits the tile halo; otherwise the same steps as separate kernels. Both variants are
         // enqueued and the device-side bound makes exactly one of them do the work.
@@ -384,10 +409,6 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
         HIP_CHECK(launch_col_fft(ia, true, s));
         ST_CHECK(mark(pl, "azimuth_ifft_a"));
     } else {
-        if (pl->cs.n2 > 1) {
-            ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
-            ST_CHECK(mark(pl, "azimuth_fft_b"));
-        }
         // 4. RCMC + azimuth matched filter
         HIP_CHECK(launch_rcmc_mf(rc, s));
         ST_CHECK(mark(pl, "rcmc_matched_filter"));
@@ -477,5 +498,40 @@ extern "C" int asarf_plan_results(asarf_plan* pl, asarf_focus_results* r) {
     r->imp_rows = pl->p.imp ? d.imp_rows : 0;
     r->rcmc_shift_bound = pl->img2 ? d.rcmc_reach : 0;
     r->azimuth_core_fused = (pl->img2 && d.rcmc_reach <= kAzFusedReach) ? 1 : 0;
+    return clear_error();
+}
+
+extern "C" int asarf_plan_set_rcmc(asarf_plan* pl, int interp, const asarf_rcmc_net* net) {
+    if (!pl) return set_error(ASARF_EINVAL, "asarf_plan_set_rcmc: null plan");
+    if (interp < ASARF_RCMC_SINC16 || interp > ASARF_RCMC_SINC8)
+        return set_error(ASARF_EINVAL, "asarf_plan_set_rcmc: interpolator %d", interp);
+    if (interp == ASARF_RCMC_LEARNED && (!net || !net->lut || !net->lut_ok))
+        return set_error(ASARF_EINVAL, "asarf_plan_set_rcmc: the learned interpolator needs a net with a built table");
+    if (net && net->ctx != pl->ctx) return set_error(ASARF_EINVAL, "asarf_plan_set_rcmc: net of another context");
+    pl->rcmc_interp = interp;
+    pl->net = interp == ASARF_RCMC_LEARNED ? net : nullptr;
+    return clear_error();
+}
+
+extern "C" int asarf_plan_rcmc_pairs(asarf_plan* pl, const void* raw, long n_pairs, unsigned long long seed,
+                                     float* pairs) {
+    if (!pl || !raw || !pairs || n_pairs <= 0) return set_error(ASARF_EINVAL, "asarf_plan_rcmc_pairs: argument");
+    if (pl->nrv < 32) return set_error(ASARF_ESHAPE, "asarf_plan_rcmc_pairs: too few range samples");
+    pl->ev_names.clear();
+    ST_CHECK(enqueue_to_spectrum(pl, raw));
+    PairArgs a{};
+    a.img = pl->img;
+    a.stride = pl->stride;
+    a.nrv = pl->nrv;
+    a.nbins = pl->nap;
+    a.n1 = pl->cs.n1;
+    a.n2 = pl->cs.n2;
+    a.col = pl->col;
+    a.prf = pl->p.prf;
+    rcmc_window(a.window);
+    a.n_pairs = n_pairs;
+    a.seed = seed;
+    a.pairs = pairs;
+    HIP_CHECK(launch_rcmc_pairs(a, pl->ctx->stream));
     return clear_error();
 }

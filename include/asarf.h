@@ -238,6 +238,46 @@ int asarf_plan_results(asarf_plan* plan, asarf_focus_results* results);
 int asarf_plan_set_timing(asarf_plan* plan, int enable);
 int asarf_plan_kernel_times(asarf_plan* plan, const char** names, float* ms, int capacity, int* count);
 
+/* ------------------------------------------------------------------------------ learned RCMC interpolator */
+/* No counterpart in the reference: an 8-tap interpolator whose complex taps are produced by a small network
+ * (MLP 2 -> 64 -> 64 -> 16, tanh, bfloat16 weights, FP32 sums) from the fractional RCMC shift d and the local
+ * migration slope, trained by this library's own kernels on pairs produced by the 16-tap sinc RCMC
+ * (range_doppler_algorithm.cu:107-195), with the 16-tap sinc kept as fallback and error judge.
+ * A training pair is ASARF_RCMC_PAIR_FLOATS floats: 2 d, slope, 8 complex samples at cx-4..cx+3 and the
+ * complex 16-tap target, samples and target divided by the samples' RMS. */
+#define ASARF_RCMC_PAIR_FLOATS 20
+enum { ASARF_RCMC_SINC16 = 0, ASARF_RCMC_LEARNED = 1, ASARF_RCMC_SINC8 = 2 };
+typedef struct asarf_rcmc_net asarf_rcmc_net;
+
+/* Glorot-uniform weights from the seed, zero biases, fresh Adam state. */
+int asarf_rcmc_net_create(asarf_ctx* ctx, unsigned long long seed, asarf_rcmc_net** out);
+int asarf_rcmc_net_destroy(asarf_rcmc_net* net);
+int asarf_rcmc_net_num_params(void);
+/* FP32 master parameters (host arrays of asarf_rcmc_net_num_params() floats). set resets the Adam state. */
+int asarf_rcmc_net_get_params(asarf_rcmc_net* net, float* out, int count);
+int asarf_rcmc_net_set_params(asarf_rcmc_net* net, const float* in, int count);
+/* Sum over the pairs (device) of the gradient of |prediction - target|^2 with respect to the parameters, the
+ * forward pass using the bfloat16-rounded parameters; *loss = sum of the squared errors. Synchronises. */
+int asarf_rcmc_net_grad(asarf_rcmc_net* net, const float* pairs, long n_pairs, float* grad_out, double* loss);
+/* Adam (beta 0.9/0.999), learning rate lr with cosine decay to lr/100 over `steps` steps of `batch` pairs
+ * (a multiple of 64) taken in order from the device pairs. loss_history (host, `steps` floats, may be NULL)
+ * receives each step's mean batch loss. Synchronises. */
+int asarf_rcmc_net_train(asarf_rcmc_net* net, const float* pairs, long n_pairs, int batch, int steps, float lr,
+                         float* loss_history);
+/* Evaluates the network at the centres of nd x ns (d, slope) cells into the tap table, then judges it on the
+ * validation pairs: per judge cell (judge_nd x ns, nd a multiple of judge_nd) the RMS error of the table taps
+ * and of a plain 8-tap Blackman sinc against the 16-tap target; cells with at least 16 pairs and an RMS error
+ * <= threshold use the table, the rest fall back to the 16-tap sinc. judge_rms / judge_sinc8_rms (host,
+ * judge_nd*ns floats, may be NULL) receive the errors; *n_judge_ok the number of accepted cells. */
+int asarf_rcmc_net_build_table(asarf_rcmc_net* net, const float* val_pairs, long n_val, int nd, int ns, int judge_nd,
+                               float threshold, float* judge_rms, float* judge_sinc8_rms, int* n_judge_ok);
+/* Runs the plan's front half (statistics, range compression, azimuth FFT) on raw and writes n_pairs training
+ * pairs (device, n_pairs x ASARF_RCMC_PAIR_FLOATS floats) drawn at seeded random (bin, column) positions. */
+int asarf_plan_rcmc_pairs(asarf_plan* plan, const void* raw, long n_pairs, unsigned long long seed, float* pairs);
+/* Selects the plan's RCMC interpolator (ASARF_RCMC_*); the learned one needs a net with a built table, which
+ * must outlive the plan's use of it. */
+int asarf_plan_set_rcmc(asarf_plan* plan, int interpolator, const asarf_rcmc_net* net);
+
 #ifdef __cplusplus
 }
 #endif

@@ -156,6 +156,11 @@ int or_focus_slc(const float* raw_iq, const or_focus_params* fp, or_zc* slc_out,
 void or_set_threads(int n);
 int or_get_threads(void);
 
+/* ---------------- learned RCMC interpolator network (mlp.c; no reference counterpart) ---------------- */
+int or_mlp_num_params(void);
+void or_mlp_forward(const double* w, double u0, double u1, double* out16);
+double or_mlp_loss_grad(const double* w, const float* pairs, long n, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,9 @@ def lib():
             "or_focus_slc": (ci, [vp, ctypes.POINTER(FocusParams), vp, ctypes.POINTER(FocusResults)]),
             "or_set_threads": (None, [ci]),
             "or_get_threads": (ci, []),
+            "or_mlp_num_params": (ci, []),
+            "or_mlp_forward": (None, [vp, cd, cd, vp]),
+            "or_mlp_loss_grad": (cd, [vp, vp, ctypes.c_long, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -252,3 +255,27 @@ def focus_slc(raw_c64: np.ndarray, cfg, quirks=QUIRKS_REFERENCE, nrp=None, nap=N
     if rc != 0:
         raise RuntimeError(f"oracle focus failed ({rc})")
     return slc, {"iq": res.iq.as_dict(), "dc_poly": list(res.dc_poly), "nrv": res.nrv}
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even of float32 values to bfloat16, returned as float32 (finite inputs)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    lsb = (u >> 16) & 1
+    u = ((u + 0x7FFF + lsb) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32)
+
+
+def mlp_loss_grad(params: np.ndarray, pairs: np.ndarray, with_grad=True):
+    """FP64 loss sum and gradient sum of the learned-RCMC network over float32 pairs (n x 20)."""
+    w = np.ascontiguousarray(params, np.float64)
+    pr = np.ascontiguousarray(pairs, np.float32)
+    g = np.zeros(w.size, np.float64) if with_grad else None
+    loss = lib().or_mlp_loss_grad(w.ctypes.data, pr.ctypes.data, pr.shape[0], g.ctypes.data if with_grad else None)
+    return loss, g
+
+
+def mlp_forward(params: np.ndarray, u0: float, u1: float) -> np.ndarray:
+    w = np.ascontiguousarray(params, np.float64)
+    out = np.zeros(16, np.float64)
+    lib().or_mlp_forward(w.ctypes.data, float(u0), float(u1), out.ctypes.data)
+    return out

@@ -1,0 +1,118 @@
+"""Learned RCMC interpolator on the GPU: gradients of the hand-written forward/backward kernels against the FP64
+oracle, training on pairs produced by the plan's own 16-tap sinc RCMC, the error judge, and what the trained
+network gains over the plain 8-tap sinc at the same tap count (SLC error against the 16-tap result, point-target
+quality against the oracle)."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+from asarf import analysis, configs, focus, scene
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def test_net_gradient_matches_oracle(ctx):
+    net = focus.RcmcNet(ctx, seed=7)
+    rng = np.random.default_rng(11)
+    n = 1000  # not a multiple of the 64-pair workgroup: the tail is masked
+    pairs = rng.standard_normal((n, 20)).astype(np.float32)
+    pairs[:, 0] = rng.uniform(-1, 1, n)
+    pairs[:, 1] = rng.uniform(-1, 1, n)
+    loss, g = net.grad(torch.from_numpy(pairs).cuda())
+    wq = po.bf16_round(net.params())
+    lref, gref = po.mlp_loss_grad(wq.astype(np.float64), pairs)
+    err = np.linalg.norm(g - gref) / np.linalg.norm(gref)
+    print(f"grad rel err {err:.2e}, loss {loss:.6f} vs {lref:.6f}")
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert err < 1e-4
+    # a second parameter set, after a set_params round trip
+    w2 = (rng.uniform(-0.2, 0.2, net.n_params)).astype(np.float32)
+    net.set_params(w2)
+    assert np.array_equal(net.params(), w2)
+    loss2, g2 = net.grad(torch.from_numpy(pairs[:640]).cuda())
+    lref2, gref2 = po.mlp_loss_grad(po.bf16_round(w2).astype(np.float64), pairs[:640])
+    assert abs(loss2 - lref2) <= 1e-5 * lref2
+    assert np.linalg.norm(g2 - gref2) / np.linalg.norm(gref2) < 1e-4
+    net.close()
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    cfg = configs.high_migration().scaled(na=2048, nr=2000, n_targets_az=3, n_targets_rg=3, tgt_az=(0.35, 0.65))
+    raw, _ = scene.generate(cfg)
+    traw = torch.from_numpy(raw).cuda()
+    plan = focus.Plan(ctx, cfg)
+    train = plan.rcmc_pairs(traw, 1 << 18, seed=101)
+    val = plan.rcmc_pairs(traw, 1 << 16, seed=202)
+    net = focus.RcmcNet(ctx, seed=3)
+    hist = net.train(train, batch=4096, steps=2000, lr=3e-3)
+    table = net.build_table(val, nd=8192, ns=2, judge_nd=256, threshold=1e-3)
+    yield cfg, raw, traw, plan, train, val, net, hist, table
+    plan.close()
+    net.close()
+
+
+def test_pairs_are_normalised_and_in_range(trained):
+    _, _, _, _, train, _, _, _, _ = trained
+    p = train[:20000].cpu().numpy()
+    live = np.abs(p).sum(1) > 0
+    assert live.mean() > 0.99
+    p = p[live]
+    assert np.all(np.abs(p[:, 0]) <= 1.0) and np.all(np.abs(p[:, 1]) <= 1.0)
+    rms = np.sqrt((p[:, 2:18] ** 2).sum(1) / 8)
+    assert np.allclose(rms, 1.0, atol=1e-5)
+    # at (almost) zero fractional shift the 16-tap target is the centre sample
+    z = np.abs(p[:, 0]) < 2e-4
+    if z.any():
+        err = np.abs((p[z, 18] + 1j * p[z, 19]) - (p[z, 10] + 1j * p[z, 11]))
+        assert err.max() < 2e-3
+
+
+def test_training_reduces_loss(trained):
+    hist = trained[7]
+    first, last = float(hist[:10].mean()), float(hist[-50:].mean())
+    print(f"loss {first:.3e} -> {last:.3e}")
+    assert np.all(np.isfinite(hist))
+    assert last < 1e-2 * first
+
+
+def test_judge_shows_gain_over_plain_8_tap_sinc(trained):
+    table = trained[8]
+    rms, rms8 = table["rms"], table["sinc8_rms"]
+    ok = np.isfinite(rms) & np.isfinite(rms8)
+    print(f"judge cells accepted {table['n_ok']} of {rms.size}; median rms learned {np.median(rms[ok]):.2e}, "
+          f"plain 8-tap sinc {np.median(rms8[ok]):.2e}")
+    assert ok.sum() > 0.9 * rms.size
+    assert np.median(rms[ok]) < 0.5 * np.median(rms8[ok])
+
+
+def test_learned_rcmc_slc_against_sinc16_and_oracle(trained):
+    cfg, raw, traw, plan, _, _, net, _, table = trained
+    out = {}
+    for name, mode in (("sinc16", focus.RCMC_SINC16), ("learned", focus.RCMC_LEARNED), ("sinc8", focus.RCMC_SINC8)):
+        plan.set_rcmc(mode, net if mode == focus.RCMC_LEARNED else None)
+        plan.execute(traw)
+        out[name] = plan.slc_tensor().cpu().numpy()
+    plan.set_rcmc(focus.RCMC_SINC16)
+    e_learned = rel_rms(out["learned"], out["sinc16"])
+    e_sinc8 = rel_rms(out["sinc8"], out["sinc16"])
+    print(f"SLC vs 16-tap sinc: learned {e_learned:.2e}, plain 8-tap sinc {e_sinc8:.2e}")
+    assert e_learned < e_sinc8
+    assert e_learned < 2e-3
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    worst = 0.0
+    tx, ty = scene.targets(cfg)
+    for x, y in zip(tx, ty):
+        mg = analysis.point_target(out["learned"], x, y)
+        mr = analysis.point_target(ref, x, y)
+        for ax in ("range", "azimuth"):
+            worst = max(worst, abs(mg[ax]["pslr"] - mr[ax]["pslr"]), abs(mg[ax]["islr"] - mr[ax]["islr"]))
+    print(f"learned RCMC worst PSLR/ISLR delta vs oracle {worst:.3f} dB")
+    assert worst < 0.1
