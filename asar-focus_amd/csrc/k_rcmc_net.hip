@@ -123,10 +123,12 @@ __device__ __forceinline__ float tanh_f(float x) {
 
 constexpr int kNetPitch = kNetH + 1;
 
-__global__ __launch_bounds__(kNetGroupSamples) void k_net_grad(const float* __restrict__ wq,
-                                                               const float* __restrict__ pairs, long n,
-                                                               float* __restrict__ partials,
-                                                               float* __restrict__ loss_part) {
+__global__ __launch_bounds__(kNetGroupSamples) void k_net_grad(NetGradArgs a) {
+    const float* __restrict__ wq = a.wq;
+    const float* __restrict__ pairs = a.pairs;
+    const long n = a.n;
+    float* __restrict__ partials = a.partials;
+    float* __restrict__ loss_part = a.loss_part;
     __shared__ float W[kNetP];
     __shared__ float H1[kNetGroupSamples][kNetPitch];  // h1, later dz1
     __shared__ float H2[kNetGroupSamples][kNetPitch];  // h2, later dz2
@@ -160,6 +162,12 @@ __global__ __launch_bounds__(kNetGroupSamples) void k_net_grad(const float* __re
         const float hv = h2[i];
 #pragma unroll
         for (int k = 0; k < kNetOut; ++k) o[k] += W[kNetOW3 + k * kNetH + i] * hv;
+    }
+    {
+        float base[kNetTaps];
+        sinc8_weights(0.5f * pr[0], a.window8, base);
+#pragma unroll
+        for (int j = 0; j < kNetTaps; ++j) o[2 * j] += base[j];
     }
     // prediction sum_j (a_j + i b_j) s_j and its squared error against the target
     float er = -pr[18], ei = -pr[19];
@@ -252,12 +260,10 @@ __global__ __launch_bounds__(kNetGroupSamples) void k_net_grad(const float* __re
     if (l == 0) loss_part[blockIdx.x] = loss;
 }
 
-hipError_t launch_net_grad(const float* wq, const float* pairs, long n, float* partials, float* loss_part,
-                           hipStream_t s) {
-    const long groups = (n + kNetGroupSamples - 1) / kNetGroupSamples;
+hipError_t launch_net_grad(const NetGradArgs& a, hipStream_t s) {
+    const long groups = (a.n + kNetGroupSamples - 1) / kNetGroupSamples;
     if (groups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_net_grad, dim3((unsigned)groups), dim3(kNetGroupSamples), 0, s, wq, pairs, n, partials,
-                       loss_part);
+    hipLaunchKernelGGL(k_net_grad, dim3((unsigned)groups), dim3(kNetGroupSamples), 0, s, a);
     return hipGetLastError();
 }
 
@@ -334,8 +340,10 @@ __global__ __launch_bounds__(256) void k_net_lut(NetLutArgs a) {
 #pragma unroll
         for (int k = 0; k < kNetOut; ++k) o[k] += W[kNetOW3 + k * kNetH + i] * h;
     }
+    float base[kNetTaps];
+    sinc8_weights(d, a.window8, base);
 #pragma unroll
-    for (int j = 0; j < kNetTaps; ++j) a.lut[(long)c * kNetTaps + j] = make_float2(o[2 * j], o[2 * j + 1]);
+    for (int j = 0; j < kNetTaps; ++j) a.lut[(long)c * kNetTaps + j] = make_float2(base[j] + o[2 * j], o[2 * j + 1]);
 }
 
 hipError_t launch_net_lut(const NetLutArgs& a, hipStream_t s) {

@@ -121,7 +121,9 @@ extern "C" int asarf_rcmc_net_grad(asarf_rcmc_net* n, const float* pairs, long n
     int st = ensure_groups(n, groups);
     if (st != ASARF_OK) return st;
     hipStream_t s = n->ctx->stream;
-    HIP_CHECK(launch_net_grad(n->wq, pairs, n_pairs, n->partials, n->loss_part, s));
+    NetGradArgs ga{n->wq, pairs, n_pairs, n->partials, n->loss_part, {}};
+    for (int i = 0; i < 8; ++i) ga.window8[i] = n->window8[i];
+    HIP_CHECK(launch_net_grad(ga, s));
     HIP_CHECK(launch_net_reduce(n->partials, (int)groups, n->grad, s));
     HIP_CHECK(hipMemcpyAsync(grad_out, n->grad, sizeof(float) * kNetP, hipMemcpyDeviceToHost, s));
     std::vector<float> lp(groups);
@@ -151,7 +153,9 @@ extern "C" int asarf_rcmc_net_train(asarf_rcmc_net* n, const float* pairs, long 
     if (loss_history) HIP_CHECK(hipMalloc(&d_hist, sizeof(float) * (size_t)steps));
     for (int k = 0; k < steps; ++k) {
         const float* b = pairs + (long)(k % nb) * batch * kPairFloats;
-        hipError_t e = launch_net_grad(n->wq, b, batch, n->partials, n->loss_part, s);
+        NetGradArgs ga{n->wq, b, batch, n->partials, n->loss_part, {}};
+        for (int i = 0; i < 8; ++i) ga.window8[i] = n->window8[i];
+        hipError_t e = launch_net_grad(ga, s);
         // cosine decay to 1 % of the initial rate
         const float lrk = lr * (0.01f + 0.99f * 0.5f * (1.0f + std::cos(3.14159265f * (float)k / (float)steps)));
         n->step += 1;
@@ -193,7 +197,8 @@ extern "C" int asarf_rcmc_net_build_table(asarf_rcmc_net* n, const float* val_pa
     n->nd = nd;
     n->ns = ns;
     n->judge_nd = judge_nd;
-    NetLutArgs la{n->wq, nd, ns, n->lut};
+    NetLutArgs la{n->wq, nd, ns, n->lut, {}};
+    for (int i = 0; i < 8; ++i) la.window8[i] = n->window8[i];
     HIP_CHECK(launch_net_lut(la, s));
     HIP_CHECK(hipMemsetAsync(n->judge, 0, sizeof(float) * 3 * (size_t)nj, s));
     NetJudgeArgs ja{};

@@ -6,6 +6,7 @@
 // along the Doppler axis, clamped to [-1, 1]. Taps apply to the 8 samples at cx - 4 .. cx + 3 (cx = x + round
 // (shift)), the centre of the reference's 16-tap window cx - 8 .. cx + 7 (range_doppler_algorithm.cu:171-190).
 // MLP 2 -> 64 -> 64 -> 16, tanh hidden units, linear output; weights are used rounded to bfloat16, sums in FP32.
+// Interpolation weights = plain 8-tap Blackman sinc weights (real) + network outputs (complex).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,6 +55,27 @@ __host__ __device__ inline int net_cell(float d, float slope, int nd, int ns) {
     return is * nd + id;
 }
 
+// Plain 8-tap Blackman-windowed sinc weights (taps cx - 4 .. cx + 3) at fractional shift d, normalised to unit
+// sum: sinc(d + 4 - j) = (-1)^j sin(pi d) / (pi (d + 4 - j)) and the common factor cancels. The network
+// outputs corrections to these weights (real part), so its outputs stay small and bfloat16 weights keep
+// their relative precision where it matters.
+__device__ inline void sinc8_weights(float d, const float* w8, float* out) {
+    if (d == 0.0f) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[j] = j == 4 ? 1.0f : 0.0f;
+        return;
+    }
+    float den = 0.f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        out[j] = ((j & 1) ? -w8[j] : w8[j]) * __builtin_amdgcn_rcpf(d + (float)(4 - j));
+        den += out[j];
+    }
+    const float inv = __builtin_amdgcn_rcpf(den);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) out[j] *= inv;
+}
+
 // numpy.blackman(M) value n, computed in FP64 and rounded to float (host side).
 inline float blackman_value(int n, int m) {
     const double a = 2.0 * 3.14159265358979323846 * (double)n / (double)(m - 1);
@@ -76,6 +98,16 @@ struct NetLutArgs {
     const float* wq;    // bf16-rounded parameters
     int nd, ns;
     float2* lut;        // [ns][nd][8] complex taps at the cell centres
+    float window8[8];
+};
+
+struct NetGradArgs {
+    const float* wq;
+    const float* pairs;
+    long n;
+    float* partials;    // [groups][kNetP]
+    float* loss_part;   // [groups]
+    float window8[8];
 };
 
 struct NetJudgeArgs {
@@ -90,8 +122,7 @@ struct NetJudgeArgs {
 };
 
 hipError_t launch_rcmc_pairs(const PairArgs& a, hipStream_t s);
-hipError_t launch_net_grad(const float* wq, const float* pairs, long n, float* partials, float* loss_part,
-                           hipStream_t s);
+hipError_t launch_net_grad(const NetGradArgs& a, hipStream_t s);
 hipError_t launch_net_reduce(const float* partials, int groups, float* out, hipStream_t s);
 // loss_slot (may be null) receives the mean loss of the batch whose partials are reduced
 hipError_t launch_net_adam(float* master, float* m, float* v, float* wq, const float* partials, int groups,

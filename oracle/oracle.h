@@ -160,6 +160,7 @@ int or_get_threads(void);
 int or_mlp_num_params(void);
 void or_mlp_forward(const double* w, double u0, double u1, double* out16);
 double or_mlp_loss_grad(const double* w, const float* pairs, long n, double* grad);
+void or_sinc8_weights(double d, double* out8);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,7 @@ def lib():
             "or_mlp_num_params": (ci, []),
             "or_mlp_forward": (None, [vp, cd, cd, vp]),
             "or_mlp_loss_grad": (cd, [vp, vp, ctypes.c_long, vp]),
+            "or_sinc8_weights": (None, [cd, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -278,4 +279,10 @@ def mlp_forward(params: np.ndarray, u0: float, u1: float) -> np.ndarray:
     w = np.ascontiguousarray(params, np.float64)
     out = np.zeros(16, np.float64)
     lib().or_mlp_forward(w.ctypes.data, float(u0), float(u1), out.ctypes.data)
+    return out
+
+
+def sinc8_weights(d: float) -> np.ndarray:
+    out = np.zeros(8, np.float64)
+    lib().or_sinc8_weights(float(d), out.ctypes.data)
     return out

@@ -50,7 +50,8 @@ def test_bias_only_network_closed_form():
     s = pairs[:, 2:18:2].astype(np.float64) + 1j * pairs[:, 3:18:2]
     t = pairs[:, 18].astype(np.float64) + 1j * pairs[:, 19]
     c = taps[0::2] + 1j * taps[1::2]
-    e = s @ c - t
+    base = np.stack([po.sinc8_weights(0.5 * float(u)) for u in pairs[:, 0]])
+    e = np.sum(s * (c[None, :] + base), 1) - t
     assert abs(loss - np.sum(np.abs(e) ** 2)) < 1e-9 * loss
     # d loss / d b_re_j = sum 2 Re(conj(e) s_j); d / d b_im_j = sum 2 Re(conj(e) i s_j)
     gre = 2 * np.real(np.conj(e)[:, None] * s).sum(0)
@@ -66,9 +67,20 @@ def test_forward_matches_loss_of_constant_taps():
     out = po.mlp_forward(w, 0.25, -0.125)
     pair = np.zeros((1, 20), np.float32)
     pair[0, 0], pair[0, 1] = 0.25, -0.125
-    pair[0, 2] = 1.0  # s_0 = 1: prediction = tap 0
+    pair[0, 2] = 1.0  # s_0 = 1: prediction = tap 0 = sinc weight 0 + network output 0
     loss, _ = po.mlp_loss_grad(w, pair, False)
-    assert abs(loss - (out[0] ** 2 + out[1] ** 2)) < 1e-12 * max(1.0, loss)
+    b0 = po.sinc8_weights(0.125)[0]
+    assert abs(loss - ((out[0] + b0) ** 2 + out[1] ** 2)) < 1e-12 * max(1.0, loss)
+
+
+def test_sinc8_weights():
+    assert np.array_equal(po.sinc8_weights(0.0), np.eye(8)[4])
+    win = np.blackman(8).astype(np.float32).astype(np.float64)
+    for d in (-0.5, -0.3, 0.01, 0.25, 0.5):
+        w = po.sinc8_weights(d)
+        assert abs(w.sum() - 1.0) < 1e-14
+        ref = win * np.sinc(d + 4 - np.arange(8))
+        assert np.allclose(w, ref / ref.sum(), rtol=1e-12, atol=1e-14)
 
 
 def test_bf16_round_known_values():
