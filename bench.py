@@ -32,7 +32,7 @@ def kernel_bytes(name, cfg, plan, res):
         return 2 * na * nr + 8 * na * nrv
     if name in ("azimuth_fft_a",):
         return 8 * na * nrv + 8 * nap * nrv
-    if name in ("azimuth_fft_b", "rcmc_matched_filter", "azimuth_ifft_b"):
+    if name in ("azimuth_fft_b", "rcmc_matched_filter", "azimuth_ifft_b", "azimuth_core_fused"):
         return 16 * nap * nrv
     if name == "azimuth_ifft_a":
         return 8 * nap * nrv + 8 * na * nrv
@@ -77,6 +77,11 @@ def main():
     ap.add_argument("--no-imp", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=5)
+    ap.add_argument("--rcmc", choices=["sinc16", "learned", "sinc8"], default="sinc16",
+                    help="RCMC interpolator; 'learned' first trains the network on pairs from this scene")
+    ap.add_argument("--train-pairs", type=int, default=10_000_000)
+    ap.add_argument("--train-steps", type=int, default=2400)
+    ap.add_argument("--judge-threshold", type=float, default=1e-3)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (5 times): the known-size reference that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of the plan's kernels")
@@ -108,6 +113,31 @@ def main():
     traw = torch.from_numpy(raw).cuda()
     ctx = focus.Context(0)
     plan = focus.Plan(ctx, cfg, imp=imp)
+    rcmc_info = {"interpolator": args.rcmc}
+    net = None
+    if args.rcmc == "learned":
+        t1 = time.perf_counter()
+        pairs = plan.rcmc_pairs(traw, args.train_pairs, seed=0x7A1)
+        val = plan.rcmc_pairs(traw, 1 << 18, seed=0x7A2)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        net = focus.RcmcNet(ctx, seed=3)
+        hist = net.train(pairs, batch=8192, steps=args.train_steps, lr=2e-3)
+        t3 = time.perf_counter()
+        table = net.build_table(val, nd=8192, ns=2, judge_nd=256, threshold=args.judge_threshold)
+        ok = np.isfinite(table["rms"]) & np.isfinite(table["sinc8_rms"])
+        rcmc_info.update({
+            "train_pairs": args.train_pairs, "pair_generation_s": round(t2 - t1, 3), "train_steps": args.train_steps,
+            "batch": 8192, "train_s": round(t3 - t2, 3), "final_loss": float(np.mean(hist[-50:])),
+            "judge_threshold": args.judge_threshold, "judge_cells_accepted": table["n_ok"],
+            "judge_cells": int(table["rms"].size),
+            "median_rms_learned": float(np.median(table["rms"][ok])),
+            "median_rms_sinc8": float(np.median(table["sinc8_rms"][ok])),
+        })
+        del pairs, val
+        plan.set_rcmc(focus.RCMC_LEARNED, net)
+    elif args.rcmc == "sinc8":
+        plan.set_rcmc(focus.RCMC_SINC8)
 
     for _ in range(args.warmup):
         plan.execute(traw)
@@ -171,8 +201,10 @@ def main():
             "raw_format": cfg.raw_format,
             "fft_sizes": [plan.params.nrp, plan.params.nap],
             "imp": imp,
-            "form": "plain FP32 Stockham, 16-tap sinc RCMC",
+            "form": "plain FP32 Stockham, " + {"sinc16": "16-tap sinc RCMC", "sinc8": "8-tap sinc RCMC",
+                                                 "learned": "learned 8-tap RCMC (16-tap sinc where judged)"}[args.rcmc],
         },
+        "rcmc": rcmc_info,
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
@@ -195,6 +227,8 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     plan.close()
+    if net is not None:
+        net.close()
     ctx.close()
     print(json.dumps(line))
 
