@@ -126,6 +126,8 @@ SIGNATURES = {
     "asarf_rcmc_net_build_table": (_ci, [_vp, _vp, ctypes.c_long, _ci, _ci, _ci, _cf, _P(_cf), _P(_cf), _P(_ci)]),
     "asarf_plan_rcmc_pairs": (_ci, [_vp, _vp, ctypes.c_long, ctypes.c_ulonglong, _vp]),
     "asarf_plan_set_rcmc": (_ci, [_vp, _ci, _vp]),
+    "asarf_fft_c2c_split_bf16": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_long, _ci, _ci, _cf]),
+    "asarf_plan_set_fft": (_ci, [_vp, _ci]),
 }
 
 _LIB = None

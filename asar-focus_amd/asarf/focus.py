@@ -138,8 +138,9 @@ def srgr_fit_spherical(r_first, dr, range_size, orbit_radius, earth_radius, gs=1
 
 
 def fft(ctx: Context, t: torch.Tensor, layout: int = L.FFT_ROWS, inverse: bool = False, scale: float = 1.0,
-        n: int | None = None, batch: int | None = None):
-    """Batched C2C FFT in place (rows: transforms along the last axis; columns: along the first)."""
+        n: int | None = None, batch: int | None = None, split_bf16: bool = False):
+    """Batched C2C FFT in place (rows: transforms along the last axis; columns: along the first). split_bf16
+    selects the matrix-core transforms with bfloat16 hi/lo operands."""
     assert t.dtype == torch.complex64 and t.is_contiguous() and t.is_cuda and t.dim() == 2
     if layout == L.FFT_ROWS:
         n = t.shape[1] if n is None else n
@@ -147,8 +148,8 @@ def fft(ctx: Context, t: torch.Tensor, layout: int = L.FFT_ROWS, inverse: bool =
     else:
         n = t.shape[0] if n is None else n
         batch = t.shape[1] if batch is None else batch
-    L.check(L.lib().asarf_fft_c2c(ctx.h, t.data_ptr(), n, batch, t.shape[1], layout,
-                                  L.INVERSE if inverse else L.FORWARD, scale))
+    f = L.lib().asarf_fft_c2c_split_bf16 if split_bf16 else L.lib().asarf_fft_c2c
+    L.check(f(ctx.h, t.data_ptr(), n, batch, t.shape[1], layout, L.INVERSE if inverse else L.FORWARD, scale))
     return t
 
 
@@ -234,6 +235,9 @@ class Plan:
         L.check(L.lib().asarf_plan_rcmc_pairs(self.h, raw.data_ptr(), n, seed, out.data_ptr()))
         return out
 
+    def set_fft(self, split_bf16: bool):
+        L.check(L.lib().asarf_plan_set_fft(self.h, FFT_SPLIT_BF16 if split_bf16 else FFT_FP32))
+
     def set_rcmc(self, interpolator: int, net: "RcmcNet | None" = None):
         L.check(L.lib().asarf_plan_set_rcmc(self.h, interpolator, net.h if net is not None else None))
         self._net = net  # keep the table alive while the plan uses it
@@ -277,6 +281,7 @@ class Plan:
 
 
 RCMC_PAIR_FLOATS = 20
+FFT_FP32, FFT_SPLIT_BF16 = 0, 1
 RCMC_SINC16, RCMC_LEARNED, RCMC_SINC8 = 0, 1, 2
 
 

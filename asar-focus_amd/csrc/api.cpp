@@ -591,49 +591,58 @@ ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stri
 }  // namespace asarf
 
 namespace asarf {
+static ColFftArgs with_mfma(int mfma, ColFftArgs a) {
+    a.mfma = mfma;
+    return a;
+}
+
 // Forward column transform into permuted storage: row k1*n2 + k2 holds bin k1 + n1*k2. pass 0 = A (strided
 // length-n1 transforms with the inter-pass twiddle), pass 1 = B (contiguous length-n2 transforms).
 int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
-                     float2* img, long stride, int ncols) {
+                     float2* img, long stride, int ncols, int mfma) {
     const int n = cs.n;
     if (cs.n2 == 1) {
-        if (pass == 0)
-            HIP_CHECK(launch_col_fft(col_args(in, in_stride, img, stride, ncols, 1, 0, 1, in_rows, 0, 1, n, nullptr,
-                                              1.f, cs.p1),
-                                     false, c->stream));
+        if (pass == 0) {
+            const ColFftArgs a = col_args(in, in_stride, img, stride, ncols, 1, 0, 1, in_rows, 0, 1, n, nullptr, 1.f,
+                                          cs.p1);
+            HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
+        }
         return ASARF_OK;
     }
-    if (pass == 0)
-        HIP_CHECK(launch_col_fft(col_args(in, in_stride, img, stride, ncols, cs.n2, 1, cs.n2, in_rows, 1, cs.n2, n,
-                                          cs.full->d_tw, 1.f, cs.p1),
-                                 false, c->stream));
-    else
-        HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr,
-                                          1.f, cs.p2),
-                                 false, c->stream));
+    if (pass == 0) {
+        const ColFftArgs a = col_args(in, in_stride, img, stride, ncols, cs.n2, 1, cs.n2, in_rows, 1, cs.n2, n,
+                                      cs.full->d_tw, 1.f, cs.p1);
+        HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
+    } else {
+        const ColFftArgs a = col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr, 1.f,
+                                      cs.p2);
+        HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
+    }
     return ASARF_OK;
 }
 
 // Inverse column transform from permuted storage; writes rows < out_rows scaled by `scale`.
 // pass 0 = B' (contiguous, inverse inter-pass twiddle), pass 1 = A' (strided, scale and row limit).
 int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,
-                     float scale) {
+                     float scale, int mfma) {
     const int n = cs.n;
     if (cs.n2 == 1) {
-        if (pass == 0)
-            HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, 1, 0, 1, n, 0, 1, out_rows, nullptr,
-                                              scale, cs.p1),
-                                     true, c->stream));
+        if (pass == 0) {
+            const ColFftArgs a = col_args(img, stride, img, stride, ncols, 1, 0, 1, n, 0, 1, out_rows, nullptr, scale,
+                                          cs.p1);
+            HIP_CHECK(launch_col_fft(with_mfma(mfma, a), true, c->stream));
+        }
         return ASARF_OK;
     }
-    if (pass == 0)
-        HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n,
-                                          cs.full->d_tw, 1.f, cs.p2),
-                                 true, c->stream));
-    else
-        HIP_CHECK(launch_col_fft(col_args(img, stride, img, stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows,
-                                          nullptr, scale, cs.p1),
-                                 true, c->stream));
+    if (pass == 0) {
+        const ColFftArgs a = col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, cs.full->d_tw,
+                                      1.f, cs.p2);
+        HIP_CHECK(launch_col_fft(with_mfma(mfma, a), true, c->stream));
+    } else {
+        const ColFftArgs a = col_args(img, stride, img, stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows, nullptr,
+                                      scale, cs.p1);
+        HIP_CHECK(launch_col_fft(with_mfma(mfma, a), true, c->stream));
+    }
     return ASARF_OK;
 }
 }  // namespace asarf
@@ -837,5 +846,70 @@ extern "C" int asarf_fft_c2c(asarf_ctx* c, void* data, int n, int batch, long di
                                  true, c->stream));
     }
     HIP_CHECK(hipStreamSynchronize(c->stream));  // the scratch may be reused by the next call
+    return clear_error();
+}
+
+extern "C" int asarf_fft_c2c_split_bf16(asarf_ctx* c, void* data, int n, int batch, long dist, int layout,
+                                        int direction, float scale) {
+    if (!c || !data) return set_error(ASARF_EINVAL, "asarf_fft_c2c_split_bf16: null argument");
+    if (batch <= 0) return set_error(ASARF_EINVAL, "asarf_fft_c2c_split_bf16: batch %d", batch);
+    if (direction != ASARF_FORWARD && direction != ASARF_INVERSE)
+        return set_error(ASARF_EINVAL, "asarf_fft_c2c_split_bf16: direction %d", direction);
+    const bool inv = direction == ASARF_INVERSE;
+    auto* d = static_cast<float2*>(data);
+    if (layout == ASARF_FFT_ROWS) {
+        if (n != 8192) return set_error(ASARF_EUNSUPPORTED, "asarf_fft_c2c_split_bf16: rows of %d (8192 only)", n);
+        if (dist < n) return set_error(ASARF_ESHAPE, "asarf_fft_c2c_split_bf16: dist %ld < n %d", dist, n);
+        ST_CHECK(ctx_init(c));
+        int st = ASARF_OK;
+        const FftPlanHost* p = get_plan(c, n, kRowThreads, &st);
+        if (!p) return st;
+        RowFftArgs a{};
+        a.in = d;
+        a.in_stride = dist;
+        a.out = d;
+        a.out_stride = dist;
+        a.rows = batch;
+        a.scale = scale;
+        a.plan = p->dev();
+        a.mfma = 1;
+        HIP_CHECK(launch_row_fft(a, inv, c->stream));
+        return clear_error();
+    }
+    if (layout != ASARF_FFT_COLUMNS) return set_error(ASARF_EINVAL, "asarf_fft_c2c_split_bf16: layout %d", layout);
+    if (dist < batch) return set_error(ASARF_ESHAPE, "asarf_fft_c2c_split_bf16: row stride %ld < columns %d", dist, batch);
+    ST_CHECK(ctx_init(c));
+    auto col_ok = [](int m) { return m == 128 || m == 256; };
+    if (col_ok(n)) {
+        int st = ASARF_OK;
+        const FftPlanHost* p = get_plan(c, n, kThreads / kColTile, &st);
+        if (!p) return st;
+        ColFftArgs a = col_args(d, dist, d, dist, batch, 1, 0, 1, n, 0, 1, n, nullptr, scale, p);
+        a.mfma = 1;
+        HIP_CHECK(launch_col_fft(a, inv, c->stream));
+        return clear_error();
+    }
+    ColSplit cs;
+    ST_CHECK(get_col_split(c, n, &cs, 256));
+    if (cs.n2 == 1 || !col_ok(cs.n1) || !col_ok(cs.n2))
+        return set_error(ASARF_EUNSUPPORTED, "asarf_fft_c2c_split_bf16: columns of %d (128, 256 or their products)",
+                         n);
+    // natural order through a temporary, as asarf_fft_c2c
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(float2) * (size_t)n * batch, &scratch));
+    auto* t = static_cast<float2*>(scratch);
+    const long ts = batch;
+    ColFftArgs a1, a2;
+    if (!inv) {
+        a1 = col_args(d, dist, t, ts, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, cs.full->d_tw, 1.f, cs.p1);
+        a2 = col_args(t, ts, d, dist, batch, cs.n1, cs.n2, 1, n, 1, cs.n1, n, nullptr, scale, cs.p2);
+    } else {
+        a1 = col_args(d, dist, t, ts, batch, cs.n1, 1, cs.n1, n, cs.n2, 1, n, cs.full->d_tw, 1.f, cs.p2);
+        a2 = col_args(t, ts, d, dist, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, nullptr, scale, cs.p1);
+    }
+    a1.mfma = a2.mfma = 1;
+    HIP_CHECK(launch_col_fft(a1, inv, c->stream));
+    HIP_CHECK(launch_col_fft(a2, inv, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     return clear_error();
 }

@@ -75,6 +75,11 @@ ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stri
                     long in_m, int in_rows, long out_g, long out_m, int out_rows, const float2* twf, float scale,
                     const FftPlanHost* sub);
 bool radix_plan(int n, int lanes, std::vector<int>& out);
+// Column transform passes of a four-step split (api.cpp); mfma selects the split-bfloat16 kernels.
+int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
+                     float2* img, long stride, int ncols, int mfma = 0);
+int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,
+                     float scale, int mfma = 0);
 }  // namespace asarf
 
 // Learned RCMC interpolator state (device buffers; the host API is in rcmc_net.cpp).

@@ -68,6 +68,7 @@ static int plan_mask(const FftPlanDev& p) {
 }
 
 hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s) {
+    if (a.mfma) return launch_row_mfma(a, inverse, s);
     const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
     if (lds > g_max_lds) return hipErrorInvalidValue;
     const int m = plan_mask(a.plan);
@@ -174,6 +175,7 @@ __global__ __launch_bounds__(kRowThreads) void k_range_compress(RcArgs a) {
 }
 
 hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
+    if (a.mfma) return launch_rc_mfma(a, fmt, s);
     const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
     if (lds > g_max_lds) return hipErrorInvalidValue;
     if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8) return hipErrorInvalidValue;
@@ -256,6 +258,7 @@ __global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
 }
 
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s) {
+    if (a.mfma) return launch_col_mfma(a, inverse, s);
     const size_t lds = sizeof(float2) * (size_t)(a.sub.n * (kColTile + 1) + tw_lds_elems(a.sub.n));
     if (lds > g_max_lds) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((long)a.ntiles * a.groups));

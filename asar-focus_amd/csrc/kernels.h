@@ -63,6 +63,7 @@ struct RowFftArgs {
     int rows;
     float scale;
     FftPlanDev plan;
+    int mfma;           // split-bfloat16 MFMA transform (length 8192 only)
 };
 
 struct RcArgs {
@@ -77,6 +78,7 @@ struct RcArgs {
     const IqCoef* coef; // device, nullptr = identity
     const float2* hconj; // conj(FFT(chirp)) / n
     FftPlanDev plan;
+    int mfma;           // split-bfloat16 MFMA transforms (length 8192 only)
 };
 
 struct ColFftArgs {
@@ -95,6 +97,7 @@ struct ColFftArgs {
     float scale;
     FftPlanDev sub;
     const int* fused_gate; // non-null: do nothing when *fused_gate <= kAzFusedReach (the fused kernel ran)
+    int mfma;              // split-bfloat16 MFMA sub-transform (lengths 128 and 256 only)
 };
 
 struct RcmcArgs {
@@ -207,6 +210,9 @@ hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s);
 bool launch_range_compress_static(const RcArgs& a, int raw_format, hipStream_t s, hipError_t* err);
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s);
+hipError_t launch_rc_mfma(const RcArgs& a, int raw_format, hipStream_t s);
+hipError_t launch_row_mfma(const RowFftArgs& a, bool inverse, hipStream_t s);
+hipError_t launch_col_mfma(const ColFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_az_fused(const AzFusedArgs& a, hipStream_t s);
 size_t az_fused_lds_bytes(int n2);

@@ -20,13 +20,6 @@
 
 using namespace asarf;
 
-namespace asarf {
-int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
-                     float2* img, long stride, int ncols);
-int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,
-                     float scale);
-}  // namespace asarf
-
 #define HIP_CHECK(expr)                                                                                   \
     do {                                                                                                  \
         hipError_t e_ = (expr);                                                                           \
@@ -67,6 +60,7 @@ struct asarf_plan {
     // RCMC interpolator (0 = 16-tap sinc, 1 = learned table with sinc fallback, 2 = 8-tap sinc)
     int rcmc_interp = 0;
     const asarf_rcmc_net* net = nullptr;
+    int fft_mfma = 0;  // split-bfloat16 MFMA transforms
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -330,13 +324,15 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     ra.coef = &pl->res->coef;
     ra.hconj = pl->hconj;
     ra.plan = pl->rplan->dev();
+    ra.mfma = pl->fft_mfma;
     HIP_CHECK(launch_range_compress(ra, p.raw_format, s));
     ST_CHECK(mark(pl, "range_compression"));
     // 3. azimuth FFT (permuted bins), rows >= Na read as zero
-    ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
+    ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv, pl->fft_mfma));
     ST_CHECK(mark(pl, "azimuth_fft_a"));
     if (pl->cs.n2 > 1) {
-        ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv));
+        ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv,
+                                  pl->fft_mfma));
         ST_CHECK(mark(pl, "azimuth_fft_b"));
     }
     return ASARF_OK;
@@ -371,7 +367,7 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
         rc.judge_nd = pl->net->judge_nd;
     }
     for (int i = 0; i < 8; ++i) rc.window8[i] = blackman_value(i, 8);
-    if (pl->img2 && pl->rcmc_interp == 0) {
+    if (pl->img2 && pl->rcmc_interp == 0 && !pl->fft_mfma) {
         const ColSplit& cs = pl->cs;
         const int n = cs.n;
         // 4. fused RCMC + matched filter + inverse pass B' (img -> img2) when the RCMC shift bound computed on
@@ -413,10 +409,12 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
         HIP_CHECK(launch_rcmc_mf(rc, s));
         ST_CHECK(mark(pl, "rcmc_matched_filter"));
         // 5. azimuth IFFT, 1/Nap, rows < Na
-        ST_CHECK(col_inverse_pass(c, pl->cs, 0, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap));
+        ST_CHECK(col_inverse_pass(c, pl->cs, 0, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap,
+                                  pl->fft_mfma));
         ST_CHECK(mark(pl, "azimuth_ifft_b"));
         if (pl->cs.n2 > 1) {
-            ST_CHECK(col_inverse_pass(c, pl->cs, 1, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap));
+            ST_CHECK(col_inverse_pass(c, pl->cs, 1, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap,
+                                      pl->fft_mfma));
             ST_CHECK(mark(pl, "azimuth_ifft_a"));
         }
     }
@@ -533,5 +531,24 @@ extern "C" int asarf_plan_rcmc_pairs(asarf_plan* pl, const void* raw, long n_pai
     a.seed = seed;
     a.pairs = pairs;
     HIP_CHECK(launch_rcmc_pairs(a, pl->ctx->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_plan_set_fft(asarf_plan* pl, int precision) {
+    if (!pl) return set_error(ASARF_EINVAL, "asarf_plan_set_fft: null plan");
+    if (precision == ASARF_FFT_FP32) {
+        pl->fft_mfma = 0;
+        return clear_error();
+    }
+    if (precision != ASARF_FFT_SPLIT_BF16) return set_error(ASARF_EINVAL, "asarf_plan_set_fft: precision %d", precision);
+    auto col_ok = [](int n) { return n == 128 || n == 256; };
+    if (pl->nrp != 8192)
+        return set_error(ASARF_EUNSUPPORTED, "asarf_plan_set_fft: split-bf16 range transforms need nrp = 8192, not %d",
+                         pl->nrp);
+    if (!col_ok(pl->cs.n1) || (pl->cs.n2 != 1 && !col_ok(pl->cs.n2)))
+        return set_error(ASARF_EUNSUPPORTED,
+                         "asarf_plan_set_fft: split-bf16 azimuth passes need a %d = n1 x n2 split into 128/256, got "
+                         "%d x %d", pl->nap, pl->cs.n1, pl->cs.n2);
+    pl->fft_mfma = 1;
     return clear_error();
 }

@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=5)
     ap.add_argument("--rcmc", choices=["sinc16", "learned", "sinc8"], default="sinc16",
                     help="RCMC interpolator; 'learned' first trains the network on pairs from this scene")
+    ap.add_argument("--fft", choices=["fp32", "split_bf16"], default="fp32",
+                    help="transform precision (split_bf16: matrix-core DFT passes, nrp 8192 / nap 32768 configs)")
     ap.add_argument("--train-pairs", type=int, default=10_000_000)
     ap.add_argument("--train-steps", type=int, default=2400)
     ap.add_argument("--judge-threshold", type=float, default=1e-3)
@@ -113,6 +115,8 @@ def main():
     traw = torch.from_numpy(raw).cuda()
     ctx = focus.Context(0)
     plan = focus.Plan(ctx, cfg, imp=imp)
+    if args.fft == "split_bf16":
+        plan.set_fft(True)
     rcmc_info = {"interpolator": args.rcmc}
     net = None
     if args.rcmc == "learned":
@@ -192,7 +196,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "bf16x2 (hi/lo) MFMA, f32 accumulate" if args.fft == "split_bf16" else "f32",
         "data": "synthetic (seeded Philox scene: point targets in gamma-textured clutter)",
         "config": {
             "workload": cfg.name,
@@ -201,7 +205,7 @@ def main():
             "raw_format": cfg.raw_format,
             "fft_sizes": [plan.params.nrp, plan.params.nap],
             "imp": imp,
-            "form": "plain FP32 Stockham, " + {"sinc16": "16-tap sinc RCMC", "sinc8": "8-tap sinc RCMC",
+            "form": ("split-bf16 MFMA DFT passes, " if args.fft == "split_bf16" else "plain FP32 Stockham, ") + {"sinc16": "16-tap sinc RCMC", "sinc8": "8-tap sinc RCMC",
                                                  "learned": "learned 8-tap RCMC (16-tap sinc where judged)"}[args.rcmc],
         },
         "rcmc": rcmc_info,

@@ -238,6 +238,17 @@ int asarf_plan_results(asarf_plan* plan, asarf_focus_results* results);
 int asarf_plan_set_timing(asarf_plan* plan, int enable);
 int asarf_plan_kernel_times(asarf_plan* plan, const char** names, float* ms, int capacity, int* count);
 
+/* ---------------------------------------------------------------------------- split-precision transforms */
+/* No counterpart in the reference (whose transforms are cuFFT FP32, cufft_plan.cc:34-91): FFTs whose radix-16
+ * passes run as batched 16-point DFT GEMMs on the matrix cores with bfloat16 hi/lo operands and FP32 sums. */
+enum { ASARF_FFT_FP32 = 0, ASARF_FFT_SPLIT_BF16 = 1 };
+/* As asarf_fft_c2c with split-bf16 kernels: rows of 8192; columns of 128, 256 or a product of the two. */
+int asarf_fft_c2c_split_bf16(asarf_ctx* ctx, void* data, int n, int batch, long dist, int layout, int direction,
+                             float scale);
+/* Transform precision of a plan: ASARF_FFT_SPLIT_BF16 needs nrp = 8192 and an azimuth length that splits into
+ * 128/256 passes (e.g. 32768 = 128 x 256); otherwise ASARF_EUNSUPPORTED. Default ASARF_FFT_FP32. */
+int asarf_plan_set_fft(asarf_plan* plan, int precision);
+
 /* ------------------------------------------------------------------------------ learned RCMC interpolator */
 /* No counterpart in the reference: an 8-tap interpolator whose complex taps are produced by a small network
  * (MLP 2 -> 64 -> 64 -> 16, tanh, bfloat16 weights, FP32 sums) from the fractional RCMC shift d and the local
