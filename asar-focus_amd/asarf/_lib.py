@@ -128,6 +128,15 @@ SIGNATURES = {
     "asarf_plan_set_rcmc": (_ci, [_vp, _ci, _vp]),
     "asarf_fft_c2c_split_bf16": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_long, _ci, _ci, _cf]),
     "asarf_plan_set_fft": (_ci, [_vp, _ci]),
+    "asarf_autotune_fft": (_ci, [_vp, _ci, _ci, _ci, _ci, _P(_ci), _P(_cf), _P(_ci)]),
+    "asarf_batch_create": (_ci, [_vp, _ci, _P(_vp)]),
+    "asarf_batch_add": (_ci, [_vp, _P(FocusParams), _vp, _vp, _vp, _P(_ci)]),
+    "asarf_batch_execute": (_ci, [_vp, _ci]),
+    "asarf_batch_sync": (_ci, [_vp]),
+    "asarf_batch_size": (_ci, [_vp]),
+    "asarf_batch_slc": (_ci, [_vp, _ci, _P(Image)]),
+    "asarf_batch_results": (_ci, [_vp, _ci, _P(FocusResults)]),
+    "asarf_batch_destroy": (_ci, [_vp]),
 }
 
 _LIB = None

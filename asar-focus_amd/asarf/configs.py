@@ -124,6 +124,32 @@ def high_migration() -> SceneConfig:
     )
 
 
+def batch_blocks(n_blocks: int = 64, na_range=(2048, 16384), seed: int = SEED_BASE + 5) -> list:
+    """Heterogeneous batch: half ERS-like blocks (5616 samples, 704-sample chirp, u8), half ASAR swath-varied
+    blocks (range 4000-6000 samples, chirp 400-560 samples, i8); azimuth lengths log-uniform in na_range; point
+    targets plus 10 dB noise as in the ERS block. Block k is seeded with seed + k."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_blocks):
+        na = int(round(float(np.exp(rng.uniform(np.log(na_range[0]), np.log(na_range[1]))))))
+        if k % 2 == 0:
+            c = ers_block().scaled(na=na, name=f"batch{k:02d}_ers_{na}x5616", n_targets_az=3, n_targets_rg=3,
+                                   seed=seed + k)
+        else:
+            fs = 19.208e6
+            nr = int(rng.integers(4000, 6001))
+            nc = int(rng.integers(400, 561))
+            c = SceneConfig(
+                name=f"batch{k:02d}_asar_{na}x{nr}", na=na, nr=nr, fs=fs, pulse_len=nc / fs,
+                bandwidth=16e6, prf=1652.0, wavelength=0.05623, vr=7100.0, r_first=850e3, dc_near=-200.0,
+                dc_far=250.0, raw_format="i8", n_targets_az=3, n_targets_rg=3, snr_db=10.0, quant_std_lsb=12.0,
+                seed=seed + k)
+        out.append(c)
+    return out
+
+
 CONFIGS = {
     "ers_block": ers_block,
     "asar_scene": asar_scene,

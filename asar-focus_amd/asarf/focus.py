@@ -172,6 +172,37 @@ def subset_copy(ctx: Context, img: L.Image, x0: int, y0: int, w: int, h: int, ds
     return dst
 
 
+def focus_params(cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
+                 orbit_height: float | None = None):
+    """asarf_focus_params of a scene configuration (and the SRGR fit when imp)."""
+    p = L.FocusParams()
+    p.na, p.nr, p.nc = cfg.na, cfg.nr, cfg.nc
+    p.nrp = cfg.nrp or padding_size(cfg.nr + cfg.nc)
+    p.nap = cfg.nap or padding_size(cfg.na)
+    p.raw_format = L.RAW_U8 if cfg.raw_format == "u8_ers" else L.RAW_I8
+    p.prf, p.wavelength, p.range_spacing = cfg.prf, cfg.wavelength, cfg.range_spacing
+    p.slant_range_first, p.az_bandwidth_fraction = cfg.r_first, cfg.az_bandwidth_fraction
+    for i, v in enumerate(cfg.vr_poly):
+        p.vr_poly[i] = v
+    p.n_total_samples = 0
+    p.quirks = quirks
+    p.imp = int(imp)
+    grsr, ng = None, None
+    if imp:
+        near = 8
+        nr_s = cfg.nr - cfg.nc - 16
+        h = cfg.orbit_height if orbit_height is None else orbit_height
+        grsr, ng = srgr_fit_spherical(cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, nr_s,
+                                      cfg.earth_radius + h, cfg.earth_radius, 12.5)
+        p.azimuth_spacing = cfg.az_spacing
+        p.reference_range = 800000.0
+        for i in range(5):
+            p.grsr_poly[i] = grsr[i]
+        p.gr_range_size = ng
+        p.ground_spacing = 12.5
+    return p, grsr, ng
+
+
 class Plan:
     """Fused focusing plan (the GPU stage of main.cc:233-327) for one scene configuration."""
 
@@ -179,32 +210,8 @@ class Plan:
                  chirp: np.ndarray | None = None, orbit_height: float | None = None):
         self.ctx = ctx
         self.cfg = cfg
-        p = L.FocusParams()
-        p.na, p.nr, p.nc = cfg.na, cfg.nr, cfg.nc
-        p.nrp = cfg.nrp or padding_size(cfg.nr + cfg.nc)
-        p.nap = cfg.nap or padding_size(cfg.na)
-        p.raw_format = L.RAW_U8 if cfg.raw_format == "u8_ers" else L.RAW_I8
-        p.prf, p.wavelength, p.range_spacing = cfg.prf, cfg.wavelength, cfg.range_spacing
-        p.slant_range_first, p.az_bandwidth_fraction = cfg.r_first, cfg.az_bandwidth_fraction
-        for i, v in enumerate(cfg.vr_poly):
-            p.vr_poly[i] = v
-        p.n_total_samples = 0
-        p.quirks = quirks
-        p.imp = int(imp)
+        p, self.grsr, self.ng = focus_params(cfg, quirks, imp, orbit_height)
         self.nrv = cfg.nr - cfg.nc
-        if imp:
-            near = 8
-            nr_s = self.nrv - 16
-            h = cfg.orbit_height if orbit_height is None else orbit_height
-            grsr, ng = srgr_fit_spherical(cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, nr_s,
-                                          cfg.earth_radius + h, cfg.earth_radius, 12.5)
-            p.azimuth_spacing = cfg.az_spacing
-            p.reference_range = 800000.0
-            for i in range(5):
-                p.grsr_poly[i] = grsr[i]
-            p.gr_range_size = ng
-            p.ground_spacing = 12.5
-            self.grsr, self.ng = grsr, ng
         self.params = p
         self.chirp = generate_chirp(cfg, p.nrp) if chirp is None else np.ascontiguousarray(chirp, np.complex64)
         h = ctypes.c_void_p()
@@ -333,6 +340,65 @@ class RcmcNet:
     def close(self):
         if self.h:
             L.lib().asarf_rcmc_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(ctx: Context, n: int, columns: bool, batch: int, reps: int = 3) -> dict:
+    """Runs the library's autotuner for one transform length; later plans on ctx use its choice."""
+    choice, n_cand = ctypes.c_int(), ctypes.c_int()
+    ms = ctypes.c_float()
+    L.check(L.lib().asarf_autotune_fft(ctx.h, n, L.FFT_COLUMNS if columns else L.FFT_ROWS, batch, reps,
+                                       ctypes.byref(choice), ctypes.byref(ms), ctypes.byref(n_cand)))
+    return {"choice": choice.value, "ms": ms.value, "candidates": n_cand.value}
+
+
+BATCH_SEQUENTIAL, BATCH_GRAPH = 0, 1
+
+
+class Batch:
+    """Blocks of uneven size focused as one batch (sequential launches or one captured graph)."""
+
+    def __init__(self, ctx: Context, n_streams: int = 4):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        L.check(L.lib().asarf_batch_create(ctx.h, n_streams, ctypes.byref(h)))
+        self.h = h
+        self._keep = []
+        self.cfgs = []
+
+    def add(self, cfg: SceneConfig, raw: torch.Tensor, quirks: int = L.QUIRKS_REFERENCE) -> int:
+        assert raw.is_cuda and raw.is_contiguous() and raw.numel() == cfg.na * cfg.nr * 2
+        p, _, _ = focus_params(cfg, quirks, False)
+        chirp = generate_chirp(cfg, p.nrp)
+        idx = ctypes.c_int()
+        L.check(L.lib().asarf_batch_add(self.h, ctypes.byref(p), chirp.ctypes.data, raw.data_ptr(), None,
+                                        ctypes.byref(idx)))
+        self._keep.append((raw, chirp))
+        self.cfgs.append(cfg)
+        return idx.value
+
+    def execute(self, mode: int):
+        L.check(L.lib().asarf_batch_execute(self.h, mode))
+
+    def sync(self):
+        L.check(L.lib().asarf_batch_sync(self.h))
+
+    def slc_tensor(self, i: int) -> torch.Tensor:
+        img = L.Image()
+        L.check(L.lib().asarf_batch_slc(self.h, i, ctypes.byref(img)))
+        out = torch.empty((img.y_size, img.x_size), dtype=torch.complex64, device=f"cuda:{self.ctx.device}")
+        subset_copy(self.ctx, img, 0, 0, img.x_size, img.y_size, out)
+        return out
+
+    def close(self):
+        if self.h:
+            L.lib().asarf_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

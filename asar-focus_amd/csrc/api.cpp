@@ -143,7 +143,19 @@ bool radix_plan(int n, int lanes, std::vector<int>& out) {
     return true;
 }
 
-static bool radix_plan_relaxed(int n, int lanes, std::vector<int>& out) {
+bool radix_plan_fits(int n, int lanes, const std::vector<int>& radix) {
+    if (radix.empty() || (int)radix.size() > kMaxPasses) return false;
+    long prod = 1;
+    for (int r : radix) {
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 16) return false;
+        const int nb = (n / r + lanes - 1) / lanes;
+        if (nb > kMaxE / r) return false;
+        prod *= r;
+    }
+    return prod == n;
+}
+
+bool radix_plan_relaxed(int n, int lanes, std::vector<int>& out) {
     if (radix_plan(n, lanes, out)) return true;
     // replace 16s by 4x4 (more passes, smaller register blocks)
     std::vector<int> alt;
@@ -188,6 +200,10 @@ float2* get_twiddles(asarf_ctx* c, int n, int* status) {
 }
 
 const FftPlanHost* get_plan(asarf_ctx* c, int n, int lanes, int* status) {
+    if (lanes == kRowThreads) {
+        auto t = c->tuned_rows.find(n);
+        if (t != c->tuned_rows.end() && t->second.plan) return t->second.plan;
+    }
     const auto key = std::make_pair(n, lanes);
     auto it = c->plans.find(key);
     if (it != c->plans.end()) return it->second.get();
@@ -210,7 +226,9 @@ int get_col_split(asarf_ctx* c, int n, ColSplit* cs, int n2_max) {
     std::vector<int> tmp;
     cs->n = n;
     int st = ASARF_OK;
-    if (n2_max <= 0 && (size_t)n <= lds_cap && radix_plan_relaxed(n, lanes, tmp)) {
+    const auto tuned_it = c->tuned_col_n1.find(n);
+    const int tuned = (n2_max <= 0 && tuned_it != c->tuned_col_n1.end()) ? tuned_it->second : 0;
+    if (n2_max <= 0 && (tuned == 0 || tuned == n) && (size_t)n <= lds_cap && radix_plan_relaxed(n, lanes, tmp)) {
         cs->n1 = n;
         cs->n2 = 1;
         cs->p1 = get_plan(c, n, lanes, &st);
@@ -218,9 +236,15 @@ int get_col_split(asarf_ctx* c, int n, ColSplit* cs, int n2_max) {
         cs->full = nullptr;
         return cs->p1 ? ASARF_OK : st;
     }
-    // n2_max <= 0: the most balanced split (smallest max(n1, n2)); otherwise the largest n2 <= n2_max
+    // n2_max <= 0: the autotuned split if there is one, else the most balanced split (smallest max(n1, n2));
+    // otherwise the largest n2 <= n2_max
     int best1 = 0, best2 = 0;
-    for (int n1 = 2; n1 <= n / 2; ++n1) {
+    if (tuned > 1 && tuned < n && n % tuned == 0) {
+        best1 = tuned;
+        best2 = n / tuned;
+    }
+    const bool use_tuned = best1 != 0;
+    for (int n1 = 2; n1 <= n / 2 && !use_tuned; ++n1) {
         if (n % n1) continue;
         const int n2 = n / n1;
         if ((size_t)n1 > lds_cap || (size_t)n2 > lds_cap) continue;

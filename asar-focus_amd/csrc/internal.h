@@ -60,6 +60,15 @@ struct asarf_ctx {
     size_t scratch_bytes = 0;
     void* h_pinned = nullptr;
     size_t pinned_bytes = 0;
+    // autotuner choices (asarf_autotune_fft): fastest column split (n1) per column length, and per row length
+    // whether the compile-time range-compression kernel wins and otherwise the fastest runtime radix plan
+    std::map<int, int> tuned_col_n1;
+    struct RowTune {
+        bool use_static = true;
+        const asarf::FftPlanHost* plan = nullptr;
+    };
+    std::map<int, RowTune> tuned_rows;
+    int tune_key = 0;  // next private plan-cache key for tuner candidates
 };
 
 namespace asarf {
@@ -75,6 +84,8 @@ ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stri
                     long in_m, int in_rows, long out_g, long out_m, int out_rows, const float2* twf, float scale,
                     const FftPlanHost* sub);
 bool radix_plan(int n, int lanes, std::vector<int>& out);
+bool radix_plan_relaxed(int n, int lanes, std::vector<int>& out);
+bool radix_plan_fits(int n, int lanes, const std::vector<int>& radix);
 // Column transform passes of a four-step split (api.cpp); mfma selects the split-bfloat16 kernels.
 int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
                      float2* img, long stride, int ncols, int mfma = 0);

@@ -180,7 +180,7 @@ hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
     if (lds > g_max_lds) return hipErrorInvalidValue;
     if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8) return hipErrorInvalidValue;
     hipError_t serr = hipSuccess;
-    if (launch_range_compress_static(a, fmt, s, &serr)) return serr;
+    if (!a.no_static && launch_range_compress_static(a, fmt, s, &serr)) return serr;
     const int m = plan_mask(a.plan);
 #define X_(SET)                                                                                                 \
     if ((m & ~(SET)) == 0) {                                                                                    \

@@ -79,6 +79,7 @@ struct RcArgs {
     const float2* hconj; // conj(FFT(chirp)) / n
     FftPlanDev plan;
     int mfma;           // split-bfloat16 MFMA transforms (length 8192 only)
+    int no_static;      // use the runtime-radix kernel even where a compile-time kernel exists (autotuner)
 };
 
 struct ColFftArgs {

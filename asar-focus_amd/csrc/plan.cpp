@@ -325,6 +325,10 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     ra.hconj = pl->hconj;
     ra.plan = pl->rplan->dev();
     ra.mfma = pl->fft_mfma;
+    {
+        const auto t = c->tuned_rows.find(pl->nrp);
+        ra.no_static = (t != c->tuned_rows.end() && !t->second.use_static) ? 1 : 0;
+    }
     HIP_CHECK(launch_range_compress(ra, p.raw_format, s));
     ST_CHECK(mark(pl, "range_compression"));
     // 3. azimuth FFT (permuted bins), rows >= Na read as zero

@@ -249,6 +249,32 @@ int asarf_fft_c2c_split_bf16(asarf_ctx* ctx, void* data, int n, int batch, long 
  * 128/256 passes (e.g. 32768 = 128 x 256); otherwise ASARF_EUNSUPPORTED. Default ASARF_FFT_FP32. */
 int asarf_plan_set_fft(asarf_plan* plan, int precision);
 
+/* ------------------------------------------------------------------------------- autotuner and batches */
+/* No counterpart in the reference. Measures the candidate schedules of one transform length on the device and
+ * records the fastest in the context; plans created afterwards use it. ROWS: range compression of `batch`
+ * lines of length n (the compile-time kernel where one exists, runtime radix orders); *choice = -1 for the
+ * compile-time kernel, else the candidate index. COLUMNS: forward azimuth passes of n x batch; *choice = n1 of
+ * the chosen split (n = single pass). *best_ms = its time, *n_candidates = candidates measured. Synchronises. */
+int asarf_autotune_fft(asarf_ctx* ctx, int n, int layout, int batch, int reps, int* choice, float* best_ms,
+                       int* n_candidates);
+
+/* Many blocks of uneven size in one batch: each block gets its own plan on one of n_streams streams (round
+ * robin; each stream has its own context that inherits ctx's autotuner choices). SEQUENTIAL runs the blocks
+ * one after another, waiting for each; GRAPH captures all blocks once (fork-join over the streams) and replays
+ * the graph with one launch per call (asynchronous: asarf_batch_sync waits). Raw inputs and IMP outputs are
+ * device buffers the caller keeps alive. */
+enum { ASARF_BATCH_SEQUENTIAL = 0, ASARF_BATCH_GRAPH = 1 };
+typedef struct asarf_batch asarf_batch;
+int asarf_batch_create(asarf_ctx* ctx, int n_streams, asarf_batch** out);
+int asarf_batch_add(asarf_batch* batch, const asarf_focus_params* params, const float* chirp_iq, const void* raw,
+                    float* imp_out, int* index);
+int asarf_batch_execute(asarf_batch* batch, int mode);
+int asarf_batch_sync(asarf_batch* batch);
+int asarf_batch_size(const asarf_batch* batch);
+int asarf_batch_slc(asarf_batch* batch, int index, asarf_image* slc);
+int asarf_batch_results(asarf_batch* batch, int index, asarf_focus_results* results);
+int asarf_batch_destroy(asarf_batch* batch);
+
 /* ------------------------------------------------------------------------------ learned RCMC interpolator */
 /* No counterpart in the reference: an 8-tap interpolator whose complex taps are produced by a small network
  * (MLP 2 -> 64 -> 64 -> 16, tanh, bfloat16 weights, FP32 sums) from the fractional RCMC shift d and the local
