@@ -2,6 +2,7 @@
 // Every entry point validates its arguments before touching the device.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -71,6 +72,9 @@ int ctx_init(asarf_ctx* c) {
     HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     c->max_lds = (size_t)lds;
     set_max_lds_bytes(c->max_lds);
+    // ASARF_COL_STATIC=0 keeps the runtime-radix column kernels for every length (comparison runs)
+    const char* cs_env = std::getenv("ASARF_COL_STATIC");
+    set_col_static_enabled(!(cs_env && cs_env[0] == '0'));
     c->initialized = true;
     return ASARF_OK;
 }

@@ -241,6 +241,29 @@ __device__ __forceinline__ TwLds tw_fill(float2* dst, const float2* tw_global, i
 #define ASARF_FOR_EACH_RADIX(X) X(2) X(3) X(4) X(5) X(7) X(8) X(16)
 #endif
 
+// Compile-time radix sequence of a transform (forward order; the inverse runs it reversed).
+template <int... Rs>
+struct RadixSeq {
+    static constexpr int count = sizeof...(Rs);
+    static constexpr int at(int p) {
+        constexpr int a[] = {Rs...};
+        return a[p];
+    }
+    static constexpr int rat(int p) { return at(count - 1 - p); }
+    static constexpr int prefix(int p) {
+        int x = 1;
+        for (int i = 0; i < p; ++i) x *= at(i);
+        return x;
+    }
+    static constexpr int rprefix(int p) {
+        int x = 1;
+        for (int i = 0; i < p; ++i) x *= rat(i);
+        return x;
+    }
+    static constexpr int product() { return prefix(count); }
+};
+
+
 constexpr int kMaxPasses = 8;
 
 struct FftPlanDev {

@@ -1,0 +1,112 @@
+// Column transforms with compile-time radix sequences for the production sub-lengths of the four-step azimuth
+// FFT. Same workgroup layout and global access as k_col_fft (16 columns x 16 lanes, LDS rows of 17 elements),
+// but two large radices (composite DFTs in registers) replace three or four runtime-dispatched passes: one LDS
+// exchange per transform instead of two or three, and compile-time index arithmetic.
+//   168 = 12 x 14, 180 = 12 x 15   (ASAR scene, 30240 = 168 x 180)
+//   84 = 12 x 7, 49 = 7 x 7        (ERS block, 4116 = 84 x 49)
+//   112 = 16 x 7, 147 = 21 x 7     (high-migration scene, 16464 = 112 x 147)
+//   128 = 16 x 8, 256 = 16 x 16    (32768 = 128 x 256)
+#include "kernels.h"
+
+namespace asarf {
+
+// Elements per lane: the largest R * ceil((N / R) / lanes) over the passes.
+template <int N, class Seq, int LANES>
+constexpr int seq_max_elems() {
+    int m = 0;
+    for (int p = 0; p < Seq::count; ++p) {
+        const int r = Seq::at(p);
+        const int nb = (N / r + LANES - 1) / LANES;
+        m = nb * r > m ? nb * r : m;
+    }
+    return m;
+}
+
+template <bool INV, int N, class Seq>
+__global__ __launch_bounds__(kThreads) void k_col_static(ColFftArgs a) {
+    static_assert(Seq::product() == N, "radix sequence must factor N");
+    extern __shared__ float2 lds[];
+    if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
+    constexpr int L = kThreads / kColTile;  // lanes per column
+    constexpr int E = seq_max_elems<N, Seq, L>();
+    constexpr int P = kColTile + 1;
+    const int tile = blockIdx.x % a.ntiles;
+    const int g = blockIdx.x / a.ntiles;
+    const int b = threadIdx.x & (kColTile - 1);
+    const int lane = threadIdx.x / kColTile;
+    const int col = tile * kColTile + b;
+    const bool col_ok = col < a.ncols;
+    const TwLds tw_at = tw_fill(lds + N * P, a.sub.tw, N, threadIdx.x, kThreads);
+    float2 v[E];
+    const float2* in = a.in + (col_ok ? col : 0);
+    const long in_base = (long)g * a.in_g;
+    const int in_rows = a.in_rows;
+    auto gl_load = [&](int m) -> float2 {
+        const long r = in_base + (long)m * a.in_m;
+        if (!col_ok || r >= in_rows) return make_float2(0.f, 0.f);
+        return in[r * a.in_stride];
+    };
+    auto lds_store = [&](int i, float2 x) { lds[i * P + b] = x; };
+    auto lds_load = [&](int i) { return lds[i * P + b]; };
+    constexpr int R0 = INV ? Seq::rat(0) : Seq::at(0);
+    pass_load<R0>(v, lane, L, N, gl_load);
+    __syncthreads();  // in-place passes: every element of the group is read before any is written
+    pass_compute<R0, E, INV>(v, lane, L, N, 1, tw_at);
+    static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
+        constexpr int p = decltype(pc)::value;
+        constexpr int Rp = INV ? Seq::rat(p - 1) : Seq::at(p - 1);
+        constexpr int R = INV ? Seq::rat(p) : Seq::at(p);
+        constexpr int ns_prev = INV ? Seq::rprefix(p - 1) : Seq::prefix(p - 1);
+        constexpr int ns = INV ? Seq::rprefix(p) : Seq::prefix(p);
+        pass_store<Rp>(v, lane, L, N, ns_prev, lds_store);
+        __syncthreads();
+        pass_load<R>(v, lane, L, N, lds_load);
+        __syncthreads();
+        pass_compute<R, E, INV>(v, lane, L, N, ns, tw_at);
+    });
+    float2* out = a.out + (col_ok ? col : 0);
+    const long out_base = (long)g * a.out_g;
+    const float sc = a.scale;
+    const float2* twf = a.tw_full;
+    auto out_write = [&](int k, float2& x, bool ok) {
+        const long r = out_base + (long)k * a.out_m;
+        if (!ok || !col_ok || r >= a.out_rows) return;
+        float2 y = x;
+        if (twf) {
+            const float2 w = twf[(long)g * k];
+            y = INV ? cmulc(y, w) : cmul(y, w);
+        }
+        out[r * a.out_stride] = cscale(y, sc);
+    };
+    constexpr int RL = INV ? Seq::rat(Seq::count - 1) : Seq::at(Seq::count - 1);
+    pass_visit<RL>(v, lane, L, N, out_write);
+}
+
+template <int N, class Seq>
+static hipError_t launch_cs(const ColFftArgs& a, bool inverse, hipStream_t s) {
+    const size_t lds = sizeof(float2) * (size_t)(N * (kColTile + 1) + tw_lds_elems(N));
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((long)a.ntiles * a.groups));
+    if (inverse)
+        hipLaunchKernelGGL((k_col_static<true, N, Seq>), grid, dim3(kThreads), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_col_static<false, N, Seq>), grid, dim3(kThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+// Returns true (and the launch status in *err) when a compile-time kernel exists for the sub-transform length.
+bool launch_col_static(const ColFftArgs& a, bool inverse, hipStream_t s, hipError_t* err) {
+    switch (a.sub.n) {
+        case 168: *err = launch_cs<168, RadixSeq<12, 14>>(a, inverse, s); return true;
+        case 180: *err = launch_cs<180, RadixSeq<12, 15>>(a, inverse, s); return true;
+        case 84: *err = launch_cs<84, RadixSeq<12, 7>>(a, inverse, s); return true;
+        case 49: *err = launch_cs<49, RadixSeq<7, 7>>(a, inverse, s); return true;
+        case 112: *err = launch_cs<112, RadixSeq<16, 7>>(a, inverse, s); return true;
+        case 147: *err = launch_cs<147, RadixSeq<21, 7>>(a, inverse, s); return true;
+        case 128: *err = launch_cs<128, RadixSeq<16, 8>>(a, inverse, s); return true;
+        case 256: *err = launch_cs<256, RadixSeq<16, 16>>(a, inverse, s); return true;
+        default: return false;
+    }
+}
+
+}  // namespace asarf

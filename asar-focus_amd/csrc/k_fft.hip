@@ -257,8 +257,13 @@ __global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
 #undef VISIT_WR_
 }
 
+static bool g_col_static = true;
+void set_col_static_enabled(bool on) { g_col_static = on; }
+
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s) {
     if (a.mfma) return launch_col_mfma(a, inverse, s);
+    hipError_t serr = hipSuccess;
+    if (g_col_static && launch_col_static(a, inverse, s, &serr)) return serr;
     const size_t lds = sizeof(float2) * (size_t)(a.sub.n * (kColTile + 1) + tw_lds_elems(a.sub.n));
     if (lds > g_max_lds) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((long)a.ntiles * a.groups));

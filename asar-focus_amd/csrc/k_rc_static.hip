@@ -11,27 +11,6 @@
 
 namespace asarf {
 
-template <int... Rs>
-struct RadixSeq {
-    static constexpr int count = sizeof...(Rs);
-    static constexpr int at(int p) {
-        constexpr int a[] = {Rs...};
-        return a[p];
-    }
-    static constexpr int rat(int p) { return at(count - 1 - p); }
-    static constexpr int prefix(int p) {
-        int x = 1;
-        for (int i = 0; i < p; ++i) x *= at(i);
-        return x;
-    }
-    static constexpr int rprefix(int p) {
-        int x = 1;
-        for (int i = 0; i < p; ++i) x *= rat(i);
-        return x;
-    }
-    static constexpr int product() { return prefix(count); }
-};
-
 template <typename T>
 struct RawPairS;
 template <>

@@ -214,6 +214,8 @@ hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_rc_mfma(const RcArgs& a, int raw_format, hipStream_t s);
 hipError_t launch_row_mfma(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_col_mfma(const ColFftArgs& a, bool inverse, hipStream_t s);
+bool launch_col_static(const ColFftArgs& a, bool inverse, hipStream_t s, hipError_t* err);
+void set_col_static_enabled(bool on);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_az_fused(const AzFusedArgs& a, hipStream_t s);
 size_t az_fused_lds_bytes(int n2);

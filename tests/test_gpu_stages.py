@@ -40,7 +40,9 @@ def test_row_fft_matches_oracle(ctx, n):
     assert rel_rms(t.cpu().numpy(), x) < 4e-6
 
 
-@pytest.mark.parametrize("n,cols", [(64, 16), (168, 37), (210, 5), (625, 33), (4116, 40), (16464, 17), (30240, 19)])
+@pytest.mark.parametrize("n,cols", [(64, 16), (168, 37), (180, 21), (84, 16), (49, 3), (112, 40), (147, 17),
+                                    (128, 16), (256, 33), (210, 5), (625, 33), (4116, 40), (16464, 17), (30240, 19),
+                                    (32768, 9)])
 def test_column_fft_matches_oracle(ctx, n, cols):
     x = rand_c((n, cols), n + cols)
     t = cuda(x)
