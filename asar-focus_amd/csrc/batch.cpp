@@ -78,6 +78,7 @@ extern "C" int asarf_batch_create(asarf_ctx* ctx, int n_streams, asarf_batch** o
         for (const auto& kv : ctx->tuned_rows) {
             asarf_ctx::RowTune rt;
             rt.use_static = kv.second.use_static;
+            rt.static_variant = kv.second.static_variant;
             if (kv.second.plan) {
                 auto p = std::make_unique<FftPlanHost>();
                 p->n = kv.second.plan->n;

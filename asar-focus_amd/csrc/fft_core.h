@@ -264,6 +264,18 @@ struct RadixSeq {
 };
 
 
+// Elements per lane: the largest R * ceil((N / R) / lanes) over the passes.
+template <int N, class Seq, int LANES>
+constexpr int seq_max_elems() {
+    int m = 0;
+    for (int p = 0; p < Seq::count; ++p) {
+        const int r = Seq::at(p);
+        const int nb = (N / r + LANES - 1) / LANES;
+        m = nb * r > m ? nb * r : m;
+    }
+    return m;
+}
+
 constexpr int kMaxPasses = 8;
 
 struct FftPlanDev {

@@ -65,6 +65,7 @@ struct asarf_ctx {
     std::map<int, int> tuned_col_n1;
     struct RowTune {
         bool use_static = true;
+        int static_variant = 0;
         const asarf::FftPlanHost* plan = nullptr;
     };
     std::map<int, RowTune> tuned_rows;

@@ -10,18 +10,6 @@
 
 namespace asarf {
 
-// Elements per lane: the largest R * ceil((N / R) / lanes) over the passes.
-template <int N, class Seq, int LANES>
-constexpr int seq_max_elems() {
-    int m = 0;
-    for (int p = 0; p < Seq::count; ++p) {
-        const int r = Seq::at(p);
-        const int nb = (N / r + LANES - 1) / LANES;
-        m = nb * r > m ? nb * r : m;
-    }
-    return m;
-}
-
 template <bool INV, int N, class Seq>
 __global__ __launch_bounds__(kThreads) void k_col_static(ColFftArgs a) {
     static_assert(Seq::product() == N, "radix sequence must factor N");

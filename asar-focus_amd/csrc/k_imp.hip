@@ -6,24 +6,35 @@
 
 namespace asarf {
 
-// Each lane owns one range column and walks a chunk of output rows; the |s|^2 of every input row is read once
-// and kept in a per-lane ring of 8 rows in LDS (a window spans at most 5 input rows).
-constexpr int kMlRows = 64;  // output rows per workgroup
+// Each lane owns one range column of a tile of `rows` output rows. The |s|^2 of every input row the tile needs
+// (at most kMlSpan) is loaded first, all loads independent and in flight together, into the lane's own LDS
+// column (no barrier: a lane only reads what it wrote); the boxcars and the interpolation then run from LDS.
+constexpr int kMlSpan = 32;     // input rows per tile
+constexpr int kMlLanes = 256;
 
-__global__ __launch_bounds__(256) void k_multilook(MultilookArgs a) {
-    __shared__ float ring[8][256];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kMlLanes) void k_multilook(MultilookArgs a, int rows) {
+    __shared__ float pw[kMlSpan][kMlLanes];
+    const int x = blockIdx.x * kMlLanes + threadIdx.x;
     const int out_rows = min(a.out_rows_dev ? *a.out_rows_dev : a.out_rows, a.out_rows);
-    const int y0 = blockIdx.y * kMlRows;
-    const int y1 = min(y0 + kMlRows, out_rows);
-    if (x >= a.nr || y0 >= y1) return;
+    const int y0 = blockIdx.y * rows;
+    const int y1 = min(y0 + rows, out_rows);
+    if (y0 >= y1) return;
     const int row0 = a.row0_dev ? *a.row0_dev : a.row0;
     const int in_rows = a.in_rows_dev ? *a.in_rows_dev : a.in_rows;
-    const float2* col = a.slc + (long)row0 * a.stride + a.col0 + x;
-    float* rg = &ring[0][threadIdx.x];
-    int next = -1;
+    // input rows of the tile: resampling positions in float, exactly as the reference computes them
+    const float first = (float)y0 * a.ratio, last = (float)(y1 - 1) * a.ratio;
+    const int lo = max(min((int)(first - 2.0f), (int)(first - 1.0f)), 0);
+    const int hi = min(max((int)(last + 2.0f), (int)(last + 3.0f)), in_rows);
+    const int span = min(hi - lo, kMlSpan);  // the host sizes tiles so that hi - lo <= kMlSpan
+    const bool xok = x < a.nr;
+    const float2* col = a.slc + (long)(row0 + lo) * a.stride + a.col0 + (xok ? x : 0);
+#pragma unroll 8
+    for (int i = 0; i < span; ++i) {
+        const float2 v = col[(long)i * a.stride];
+        pw[i][threadIdx.x] = v.x * v.x + v.y * v.y;
+    }
+    if (!xok) return;
     for (int y = y0; y < y1; ++y) {
-        // resampling position in float, exactly as the reference computes it
         const float in_idx = (float)y * a.ratio;
         const float mu = in_idx - (float)(int)in_idx;
         int s0 = (int)(in_idx - 2.0f), e0 = (int)(in_idx + 2.0f);
@@ -32,15 +43,9 @@ __global__ __launch_bounds__(256) void k_multilook(MultilookArgs a) {
         e0 = min(e0, in_rows);
         s1 = max(s1, 0);
         e1 = min(e1, in_rows);
-        if (next < s0) next = s0;
-        const int hi = max(e0, e1);
-        for (; next < hi; ++next) {
-            const float2 v = col[(long)next * a.stride];
-            rg[(next & 7) * 256] = v.x * v.x + v.y * v.y;
-        }
         float sum0 = 0.f, sum1 = 0.f;
-        for (int i = s0; i < e0; ++i) sum0 += rg[(i & 7) * 256];
-        for (int i = s1; i < e1; ++i) sum1 += rg[(i & 7) * 256];
+        for (int i = s0; i < e0; ++i) sum0 += pw[i - lo][threadIdx.x];
+        for (int i = s1; i < e1; ++i) sum1 += pw[i - lo][threadIdx.x];
         const int c0 = max(e0 - s0, 0), c1 = max(e1 - s1, 0);
         if (c0) sum0 /= (float)c0;
         if (c1) sum1 /= (float)c1;
@@ -52,8 +57,12 @@ __global__ __launch_bounds__(256) void k_multilook(MultilookArgs a) {
 
 hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s) {
     if (a.out_rows <= 0 || a.nr <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_multilook, dim3((a.nr + 255) / 256, (a.out_rows + kMlRows - 1) / kMlRows), dim3(256), 0, s,
-                       a);
+    if (!(a.ratio > 0.f) || a.ratio > (float)(kMlSpan - 8)) return hipErrorInvalidValue;
+    // rows per tile: the span (rows - 1) * ratio + 6 must fit kMlSpan (with one row of margin for rounding)
+    int rows = (int)((float)(kMlSpan - 7) / a.ratio) + 1;
+    rows = max(1, min(rows, 16));
+    hipLaunchKernelGGL(k_multilook, dim3((a.nr + kMlLanes - 1) / kMlLanes, (a.out_rows + rows - 1) / rows),
+                       dim3(kMlLanes), 0, s, a, rows);
     return hipGetLastError();
 }
 

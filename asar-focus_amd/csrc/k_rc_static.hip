@@ -30,8 +30,9 @@ __host__ __device__ constexpr int static_lds_elems() {
 template <int FMT, int N, int T, int PADSHIFT, int WAVES, class Seq>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_rc_static(RcArgs a) {
     static_assert(Seq::product() == N, "radix sequence must factor N");
-    constexpr int E = N / T;
-    static_assert(E * T == N, "lanes must divide N");
+    // elements per lane: N / T when every radix divides it, else the largest R * ceil((N / R) / T) (lanes
+    // beyond a pass's butterfly count compute clamped copies and store nothing)
+    constexpr int E = seq_max_elems<N, Seq, T>();
     extern __shared__ float2 lds[];
     const int row = blockIdx.x;
     const int lane = threadIdx.x;
@@ -124,18 +125,40 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Compile-time variants per production length (radix sequence, lanes per line); the autotuner times them and
+// records the fastest (RcArgs::static_variant). Variant 0 is the default.
+int range_compress_static_variants(int n) {
+    switch (n) {
+        case 6144: return 4;
+        case 6400: return 3;
+        case 8192: return 3;
+        default: return 0;
+    }
+}
+
 // Returns true (and the launch status in *err) when a specialisation exists for the plan's length.
 bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipError_t* err) {
+    const int v = a.static_variant;
     switch (a.plan.n) {
         case 6144:
-            *err = launch_static<6144, 256, 5, 2, RadixSeq<8, 8, 8, 12>>(a, fmt, s);
-            return true;
+            switch (v) {
+                case 1: *err = launch_static<6144, 384, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
+                case 2: *err = launch_static<6144, 384, 5, 2, RadixSeq<16, 16, 24>>(a, fmt, s); return true;
+                case 3: *err = launch_static<6144, 512, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
+                default: *err = launch_static<6144, 256, 5, 2, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
+            }
         case 6400:
-            *err = launch_static<6400, 320, 7, 3, RadixSeq<20, 20, 4, 4>>(a, fmt, s);
-            return true;
+            switch (v) {
+                case 1: *err = launch_static<6400, 400, 5, 2, RadixSeq<16, 16, 25>>(a, fmt, s); return true;
+                case 2: *err = launch_static<6400, 400, 5, 2, RadixSeq<25, 16, 16>>(a, fmt, s); return true;
+                default: *err = launch_static<6400, 320, 7, 3, RadixSeq<20, 20, 4, 4>>(a, fmt, s); return true;
+            }
         case 8192:
-            *err = launch_static<8192, 256, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s);
-            return true;
+            switch (v) {
+                case 1: *err = launch_static<8192, 512, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
+                case 2: *err = launch_static<8192, 512, 5, 2, RadixSeq<16, 16, 16, 2>>(a, fmt, s); return true;
+                default: *err = launch_static<8192, 256, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
+            }
         default:
             return false;
     }

@@ -75,13 +75,16 @@ __device__ __forceinline__ double alias_about_dc(double f, double dc, double prf
 }
 
 // One workgroup per frequency row (in place). The row is staged in LDS so every output can read its 16 taps.
-__global__ __launch_bounds__(kThreads) void k_rcmc_mf(RcmcArgs a) {
+// 512 lanes per row: the LDS row (8 B x nrv) limits the workgroups per CU, so wider workgroups keep more waves
+// in flight.
+constexpr int kRcmcThreads = 512;
+__global__ __launch_bounds__(kRcmcThreads) void k_rcmc_mf(RcmcArgs a) {
     extern __shared__ float2 row[];
     if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
     const int q = blockIdx.x;
     float2* img = a.img + (long)q * a.stride;
     const int nrv = a.nrv;
-    for (int x = threadIdx.x; x < nrv; x += kThreads) row[x] = img[x];
+    for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) row[x] = img[x];
     __syncthreads();
     const int k = (a.n2 == 1) ? q : (q / a.n2 + a.n1 * (q % a.n2));
     const double step = a.prf / (double)a.nbins;
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void k_rcmc_mf(RcmcArgs a) {
     float w[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
-    for (int x = threadIdx.x; x < nrv; x += kThreads) {
+    for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) {
         const ColConst cc = a.col[x];
         const double fn = alias_about_dc(fb, cc.dc, a.prf);
         float2 val;
@@ -194,7 +197,7 @@ hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s) {
     ensure_window(s);
     const size_t lds = sizeof(float2) * (size_t)a.nrv;
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rcmc_mf, dim3(a.nbins), dim3(kThreads), lds, s, a);
+    hipLaunchKernelGGL(k_rcmc_mf, dim3(a.nbins), dim3(kRcmcThreads), lds, s, a);
     return hipGetLastError();
 }
 

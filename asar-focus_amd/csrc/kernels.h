@@ -80,6 +80,7 @@ struct RcArgs {
     FftPlanDev plan;
     int mfma;           // split-bfloat16 MFMA transforms (length 8192 only)
     int no_static;      // use the runtime-radix kernel even where a compile-time kernel exists (autotuner)
+    int static_variant; // which compile-time variant of the length (autotuner; 0 = default)
 };
 
 struct ColFftArgs {
@@ -210,6 +211,7 @@ struct SrgrArgs {
 hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s);
 bool launch_range_compress_static(const RcArgs& a, int raw_format, hipStream_t s, hipError_t* err);
+int range_compress_static_variants(int n);
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_rc_mfma(const RcArgs& a, int raw_format, hipStream_t s);
 hipError_t launch_row_mfma(const RowFftArgs& a, bool inverse, hipStream_t s);

@@ -328,6 +328,13 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     {
         const auto t = c->tuned_rows.find(pl->nrp);
         ra.no_static = (t != c->tuned_rows.end() && !t->second.use_static) ? 1 : 0;
+        ra.static_variant = (t != c->tuned_rows.end()) ? t->second.static_variant : 0;
+        // ASARF_RC_VARIANT=<v> forces compile-time variant v (comparison runs and tests)
+        const char* v_env = std::getenv("ASARF_RC_VARIANT");
+        if (v_env && range_compress_static_variants(pl->nrp) > 0) {
+            ra.no_static = 0;
+            ra.static_variant = std::atoi(v_env) % range_compress_static_variants(pl->nrp);
+        }
     }
     HIP_CHECK(launch_range_compress(ra, p.raw_format, s));
     ST_CHECK(mark(pl, "range_compression"));

@@ -1,7 +1,7 @@
 // Autotuner: measures candidate transform schedules on the device and records the fastest per length in the
 // context, where the planner (get_plan, get_col_split) and the plans created afterwards pick them up.
-//   rows (range compression): the compile-time kernel where one exists, and runtime-radix plans with the
-//     factors of the length in different orders and groupings;
+//   rows (range compression): the compile-time variants where the length has them (radix sequences and lanes
+//     per line), and runtime-radix plans with the factors of the length in different orders and groupings;
 //   columns (azimuth): every two-pass split n1 x n2 (and the single pass where it fits LDS).
 // Timing: one warm-up launch, then the minimum of `reps` launches bracketed by HIP events.
 #include <algorithm>
@@ -127,9 +127,9 @@ extern "C" int asarf_autotune_fft(asarf_ctx* c, int n, int layout, int batch, in
             plans.push_back(p.get());
             c->plans[std::make_pair(n, -1000 - c->tune_key++)] = std::move(p);
         }
-        // candidate -1: the compile-time kernel (if the length has one)
-        const bool has_static = n == 6144 || n == 6400 || n == 8192;
-        for (int ci = has_static ? -1 : 0; ci < (int)plans.size(); ++ci) {
+        // candidates -1 .. -nv: the compile-time variants of the length (if any)
+        const int nv = range_compress_static_variants(n);
+        for (int ci = -nv; ci < (int)plans.size(); ++ci) {
             RcArgs a{};
             a.in = raw;
             a.in_stride = nr_in;
@@ -152,6 +152,7 @@ extern "C" int asarf_autotune_fft(asarf_ctx* c, int n, int layout, int batch, in
             }
             a.plan = pd;
             a.no_static = ci >= 0;
+            a.static_variant = ci < 0 ? -1 - ci : 0;
             hipError_t e = launch_range_compress(a, RAW_U8, s);
             if (e != hipSuccess) continue;  // candidate not launchable (e.g. LDS): skip
             float ms_min = INFINITY;
@@ -176,6 +177,7 @@ extern "C" int asarf_autotune_fft(asarf_ctx* c, int n, int layout, int batch, in
         if (!found) return set_error(ASARF_EUNSUPPORTED, "asarf_autotune_fft: no row candidate ran");
         asarf_ctx::RowTune rt;
         rt.use_static = best_idx < 0;
+        rt.static_variant = best_idx < 0 ? -1 - best_idx : 0;
         rt.plan = best_idx >= 0 ? plans[best_idx] : nullptr;
         c->tuned_rows[n] = rt;
         if (choice) *choice = best_idx;

@@ -64,3 +64,30 @@ def test_batch_graph_equals_sequential_and_replays(ctx):
     print(f"batch block 1 ({cfgs[1].name}) slc rel rms {err:.2e}")
     assert err < 1e-4
     b.close()
+
+
+@pytest.mark.parametrize("which,nvar", [("asar", 4), ("ers", 3), ("ers8192", 3)])
+def test_range_compression_variants_match_oracle(ctx, which, nvar):
+    """Every compile-time range-compression variant the autotuner may pick gives the oracle's SLC."""
+    import os
+    if which == "asar":
+        cfg = configs.asar_scene().scaled(na=256, n_targets_random=3)
+    elif which == "ers":
+        cfg = configs.ers_block().scaled(na=256, n_targets_az=1, n_targets_rg=3, tgt_az=(0.5, 0.5))
+    else:
+        cfg = configs.ers_block().scaled(na=256, n_targets_az=1, n_targets_rg=3, tgt_az=(0.5, 0.5), nrp=8192)
+    raw, _ = scene.generate(cfg)
+    traw = torch.from_numpy(raw).cuda()
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    for v in range(nvar):
+        os.environ["ASARF_RC_VARIANT"] = str(v)
+        try:
+            plan = focus.Plan(ctx, cfg)
+            plan.execute(traw)
+            slc = plan.slc_tensor().cpu().numpy()
+            plan.close()
+        finally:
+            del os.environ["ASARF_RC_VARIANT"]
+        err = rel_rms(slc, ref)
+        print(f"{which} variant {v}: slc rel rms {err:.2e}")
+        assert err < 1e-4, v
