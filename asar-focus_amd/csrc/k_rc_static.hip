@@ -1,12 +1,12 @@
 // Range compression with compile-time transform plans for the production row lengths.
 //
 // Same algorithm as k_range_compress (correct on load, forward Stockham FFT, x conj(H)/n, inverse FFT, store the
-// kept columns), but the radix sequence, the lane count and the LDS padding are template parameters chosen so
-// that every lane owns exactly N/T elements in every pass: no runtime radix dispatch, no idle lanes, compile-time
-// index arithmetic. Lengths without a specialisation use the generic kernel.
-//   6144 (ASAR IM, 5616 + 521 samples): 256 lanes x 24 elements, radices 8, 8, 8, 12
-//   6400 (ERS IM,  5616 + 704 samples): 320 lanes x 20 elements, radices 20, 20, 4, 4
-//   8192 (split-precision ERS form):    256 lanes x 32 elements, radices 8, 8, 8, 16
+// kept columns), but the radix sequence, the lane count, the LDS padding and the occupancy target are template
+// parameters: no runtime radix dispatch and compile-time index arithmetic. A lane owns R * ceil((N/R)/T) elements
+// in each pass (lanes beyond a pass's butterfly count compute clamped copies). Lengths without a specialisation
+// use the generic kernel.
+//   6144 (ASAR IM, 5616 + 521 samples), 6400 (ERS IM, 5616 + 704 samples), 8192 (split-precision ERS form);
+//   several radix sequences and lane counts per length (the autotuner picks among them; see the table below).
 #include "kernels.h"
 
 namespace asarf {
@@ -129,35 +129,41 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
 // records the fastest (RcArgs::static_variant). Variant 0 is the default.
 int range_compress_static_variants(int n) {
     switch (n) {
-        case 6144: return 4;
-        case 6400: return 3;
-        case 8192: return 3;
+        case 6144: return 5;
+        case 6400: return 5;
+        case 8192: return 5;
         default: return 0;
     }
 }
 
-// Returns true (and the launch status in *err) when a specialisation exists for the plan's length.
+// Returns true (and the launch status in *err) when a specialisation exists for the plan's length. Variant 0 is
+// the fastest measured on the ASAR/ERS scenes (512 lanes with a 4-wave occupancy target keeps 16 waves per CU).
 bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipError_t* err) {
     const int v = a.static_variant;
     switch (a.plan.n) {
         case 6144:
             switch (v) {
-                case 1: *err = launch_static<6144, 384, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
-                case 2: *err = launch_static<6144, 384, 5, 2, RadixSeq<16, 16, 24>>(a, fmt, s); return true;
-                case 3: *err = launch_static<6144, 512, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
-                default: *err = launch_static<6144, 256, 5, 2, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
+                case 1: *err = launch_static<6144, 256, 5, 2, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
+                case 2: *err = launch_static<6144, 384, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
+                case 3: *err = launch_static<6144, 512, 5, 4, RadixSeq<12, 8, 8, 8>>(a, fmt, s); return true;
+                case 4: *err = launch_static<6144, 512, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
+                default: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
             }
         case 6400:
             switch (v) {
-                case 1: *err = launch_static<6400, 400, 5, 2, RadixSeq<16, 16, 25>>(a, fmt, s); return true;
+                case 1: *err = launch_static<6400, 512, 5, 4, RadixSeq<20, 20, 16>>(a, fmt, s); return true;
                 case 2: *err = launch_static<6400, 400, 5, 2, RadixSeq<25, 16, 16>>(a, fmt, s); return true;
-                default: *err = launch_static<6400, 320, 7, 3, RadixSeq<20, 20, 4, 4>>(a, fmt, s); return true;
+                case 3: *err = launch_static<6400, 320, 7, 3, RadixSeq<20, 20, 4, 4>>(a, fmt, s); return true;
+                case 4: *err = launch_static<6400, 512, 5, 4, RadixSeq<4, 4, 20, 20>>(a, fmt, s); return true;
+                default: *err = launch_static<6400, 512, 5, 4, RadixSeq<16, 20, 20>>(a, fmt, s); return true;
             }
         case 8192:
             switch (v) {
-                case 1: *err = launch_static<8192, 512, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
+                case 1: *err = launch_static<8192, 256, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
                 case 2: *err = launch_static<8192, 512, 5, 2, RadixSeq<16, 16, 16, 2>>(a, fmt, s); return true;
-                default: *err = launch_static<8192, 256, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
+                case 3: *err = launch_static<8192, 512, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
+                case 4: *err = launch_static<8192, 512, 5, 4, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
+                default: *err = launch_static<8192, 512, 5, 4, RadixSeq<16, 8, 8, 8>>(a, fmt, s); return true;
             }
         default:
             return false;
