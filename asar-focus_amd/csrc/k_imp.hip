@@ -79,11 +79,15 @@ hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_
     return hipGetLastError();
 }
 
-__global__ void k_srgr(SrgrArgs a) {
+// One lane per output column and a tile of kSrgrRows rows: the FP64 polynomial and the source index are
+// computed once per column and reused for every row of the tile.
+constexpr int kSrgrRows = 16;
+
+__global__ __launch_bounds__(256) void k_srgr(SrgrArgs a) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int rows = a.rows_dev ? *a.rows_dev : a.rows;
-    if (x >= a.out_nr || y >= rows || y >= a.rows) return;
+    const int rows = min(a.rows_dev ? *a.rows_dev : a.rows, a.rows);
+    const int y0 = blockIdx.y * kSrgrRows;
+    if (x >= a.out_nr || y0 >= rows) return;
     const double rd = polyval(a.grsr, 5, (double)x * a.gs);
     const double ri = rd / a.dr;
     int r0 = (int)ri;
@@ -97,14 +101,19 @@ __global__ void k_srgr(SrgrArgs a) {
         fx = 0.f;
     }
     const int r1 = min(r0 + 1, a.in_nr - 1);
-    const float* row = a.in + (long)y * a.in_nr;
-    const float v0 = row[r0], v1 = row[r1];
-    a.out[(long)y * a.out_nr + x] = v0 + fx * (v1 - v0);
+    const int y1 = min(y0 + kSrgrRows, rows);
+#pragma unroll 4
+    for (int y = y0; y < y1; ++y) {
+        const float* row = a.in + (long)y * a.in_nr;
+        const float v0 = row[r0], v1 = row[r1];
+        a.out[(long)y * a.out_nr + x] = v0 + fx * (v1 - v0);
+    }
 }
 
 hipError_t launch_srgr(const SrgrArgs& a, hipStream_t s) {
     if (a.rows <= 0 || a.out_nr <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_srgr, dim3((a.out_nr + 255) / 256, a.rows), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_srgr, dim3((a.out_nr + 255) / 256, (a.rows + kSrgrRows - 1) / kSrgrRows), dim3(256), 0, s,
+                       a);
     return hipGetLastError();
 }
 

@@ -78,7 +78,7 @@ __device__ __forceinline__ double alias_about_dc(double f, double dc, double prf
 // 512 lanes per row: the LDS row (8 B x nrv) limits the workgroups per CU, so wider workgroups keep more waves
 // in flight.
 constexpr int kRcmcThreads = 512;
-__global__ __launch_bounds__(kRcmcThreads) void k_rcmc_mf(RcmcArgs a) {
+__global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_rcmc_mf(RcmcArgs a) {
     extern __shared__ float2 row[];
     if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
     const int q = blockIdx.x;
