@@ -27,7 +27,7 @@ __host__ __device__ constexpr int static_lds_elems() {
     return N + (N >> PADSHIFT) + 1;
 }
 
-template <int FMT, int N, int T, int PADSHIFT, int WAVES, class Seq>
+template <int FMT, int N, int T, int PADSHIFT, int WAVES, class Seq, bool STAGE = false>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_rc_static(RcArgs a) {
     static_assert(Seq::product() == N, "radix sequence must factor N");
     // elements per lane: N / T when every radix divides it, else the largest R * ceil((N / R) / T) (lanes
@@ -71,7 +71,43 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
 
     // forward transform
     constexpr int R0 = Seq::at(0);
-    pass_load<R0>(v, lane, T, N, gl_load);
+    if constexpr (STAGE && FMT != RAW_CF32) {
+        // raw bytes with 16-byte loads (8 I/Q pairs per load) into LDS in natural order, then the first pass
+        // reads LDS: one wide load instead of eight 2-byte loads per lane
+        const uint8_t* rb = static_cast<const uint8_t*>(a.in) + (long)row * a.in_stride * 2;
+        const int nbytes = nr_in * 2;
+        if ((reinterpret_cast<uintptr_t>(rb) & 15) == 0) {
+            for (int off = lane * 16; off < nbytes; off += T * 16) {
+                if (off + 16 <= nbytes) {
+                    const uint4 w = *reinterpret_cast<const uint4*>(rb + off);
+                    const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const unsigned word = ws[k >> 1] >> (16 * (k & 1));
+                        float fi, fq;
+                        if constexpr (FMT == RAW_U8) {
+                            fi = (float)(word & 0xFFu);
+                            fq = (float)((word >> 8) & 0xFFu);
+                        } else {
+                            fi = (float)(int8_t)(word & 0xFFu);
+                            fq = (float)(int8_t)((word >> 8) & 0xFFu);
+                        }
+                        lds_store(off / 2 + k, correct(fi, fq));
+                    }
+                } else {
+                    for (int x = off / 2; x < nr_in; ++x) lds_store(x, gl_load(x));
+                }
+            }
+            for (int x = nr_in + lane; x < N; x += T) lds_store(x, make_float2(0.f, 0.f));
+        } else {
+            for (int x = lane; x < N; x += T) lds_store(x, gl_load(x));
+        }
+        __syncthreads();
+        pass_load<R0>(v, lane, T, N, lds_load);
+        __syncthreads();  // every staged element is read before the passes overwrite LDS
+    } else {
+        pass_load<R0>(v, lane, T, N, gl_load);
+    }
     pass_compute<R0, E, false>(v, lane, T, N, 1, tw_at);
     static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
         constexpr int p = decltype(pc)::value;
@@ -112,16 +148,16 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
     pass_visit<RIL>(v, lane, T, N, out_write);
 }
 
-template <int N, int T, int PADSHIFT, int WAVES, class Seq>
+template <int N, int T, int PADSHIFT, int WAVES, class Seq, bool STAGE = false>
 static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
     const size_t lds = sizeof(float2) * (size_t)(static_lds_elems<N, PADSHIFT>() + tw_lds_elems(N));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     if (fmt == RAW_CF32)
-        hipLaunchKernelGGL((k_rc_static<RAW_CF32, N, T, PADSHIFT, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
+        hipLaunchKernelGGL((k_rc_static<RAW_CF32, N, T, PADSHIFT, WAVES, Seq, STAGE>), dim3(a.rows), dim3(T), lds, s, a);
     else if (fmt == RAW_U8)
-        hipLaunchKernelGGL((k_rc_static<RAW_U8, N, T, PADSHIFT, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
+        hipLaunchKernelGGL((k_rc_static<RAW_U8, N, T, PADSHIFT, WAVES, Seq, STAGE>), dim3(a.rows), dim3(T), lds, s, a);
     else
-        hipLaunchKernelGGL((k_rc_static<RAW_I8, N, T, PADSHIFT, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
+        hipLaunchKernelGGL((k_rc_static<RAW_I8, N, T, PADSHIFT, WAVES, Seq, STAGE>), dim3(a.rows), dim3(T), lds, s, a);
     return hipGetLastError();
 }
 
@@ -129,15 +165,16 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
 // records the fastest (RcArgs::static_variant). Variant 0 is the default.
 int range_compress_static_variants(int n) {
     switch (n) {
-        case 6144: return 5;
-        case 6400: return 5;
-        case 8192: return 5;
+        case 6144: return 6;
+        case 6400: return 6;
+        case 8192: return 6;
         default: return 0;
     }
 }
 
 // Returns true (and the launch status in *err) when a specialisation exists for the plan's length. Variant 0 is
-// the fastest measured on the ASAR/ERS scenes (512 lanes with a 4-wave occupancy target keeps 16 waves per CU).
+// the fastest measured on the ASAR/ERS scenes (512 lanes with a 4-wave occupancy target keeps 16 waves per CU;
+// for 6144 the raw bytes are also staged through LDS with 16-byte loads).
 bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipError_t* err) {
     const int v = a.static_variant;
     switch (a.plan.n) {
@@ -146,8 +183,9 @@ bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipEr
                 case 1: *err = launch_static<6144, 256, 5, 2, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
                 case 2: *err = launch_static<6144, 384, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
                 case 3: *err = launch_static<6144, 512, 5, 4, RadixSeq<12, 8, 8, 8>>(a, fmt, s); return true;
-                case 4: *err = launch_static<6144, 512, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
-                default: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
+                case 4: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
+                case 5: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
+                default: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
             }
         case 6400:
             switch (v) {
@@ -155,6 +193,7 @@ bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipEr
                 case 2: *err = launch_static<6400, 400, 5, 2, RadixSeq<25, 16, 16>>(a, fmt, s); return true;
                 case 3: *err = launch_static<6400, 320, 7, 3, RadixSeq<20, 20, 4, 4>>(a, fmt, s); return true;
                 case 4: *err = launch_static<6400, 512, 5, 4, RadixSeq<4, 4, 20, 20>>(a, fmt, s); return true;
+                case 5: *err = launch_static<6400, 512, 5, 4, RadixSeq<16, 20, 20>, true>(a, fmt, s); return true;
                 default: *err = launch_static<6400, 512, 5, 4, RadixSeq<16, 20, 20>>(a, fmt, s); return true;
             }
         case 8192:
@@ -163,6 +202,7 @@ bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipEr
                 case 2: *err = launch_static<8192, 512, 5, 2, RadixSeq<16, 16, 16, 2>>(a, fmt, s); return true;
                 case 3: *err = launch_static<8192, 512, 5, 2, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
                 case 4: *err = launch_static<8192, 512, 5, 4, RadixSeq<8, 8, 8, 16>>(a, fmt, s); return true;
+                case 5: *err = launch_static<8192, 512, 5, 4, RadixSeq<16, 8, 8, 8>, true>(a, fmt, s); return true;
                 default: *err = launch_static<8192, 512, 5, 4, RadixSeq<16, 8, 8, 8>>(a, fmt, s); return true;
             }
         default:
