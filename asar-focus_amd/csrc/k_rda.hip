@@ -81,8 +81,10 @@ constexpr int kRcmcThreads = 512;
 __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_rcmc_mf(RcmcArgs a) {
     extern __shared__ float2 row[];
     if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
+    if (a.tile_gate && *a.tile_gate <= kRcmcTileReach) return;
     const int q = blockIdx.x;
-    float2* img = a.img + (long)q * a.stride;
+    const float2* img = a.img + (long)q * a.stride;
+    float2* dst = (a.out ? a.out : a.img) + (long)q * a.stride;
     const int nrv = a.nrv;
     for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) row[x] = img[x];
     __syncthreads();
@@ -189,8 +191,120 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8)
                 val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
             }
         }
-        img[x] = val;
+        dst[x] = val;
     }
+}
+
+// Column-tiled RCMC + matched filter (16-tap sinc, out of place): a workgroup owns 256 columns for kRtRows
+// consecutive storage rows. Each lane loads its column's FP64 constants once and reuses them for every row; the
+// rows' range segments (with an 8-column left halo and an 8 + kRcmcTileReach right halo) are double-buffered in
+// LDS. Identical arithmetic to k_rcmc_mf, so both give the same result; the plan gates between them on the
+// device-side shift bound.
+constexpr int kRtCols = 256;
+constexpr int kRtRows = 32;
+constexpr int kRtSeg = kRtCols + 16 + kRcmcTileReach;
+
+__global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
+    __shared__ float2 seg[2][kRtSeg];
+    if (a.tile_gate && *a.tile_gate > kRcmcTileReach) return;
+    const int x0 = blockIdx.x * kRtCols;
+    const int q0 = blockIdx.y * kRtRows;
+    const int q1 = min(q0 + kRtRows, a.nbins);
+    const int t = threadIdx.x;
+    const int nrv = a.nrv;
+    const int x = x0 + t;
+    const bool xok = x < nrv;
+    const ColConst cc = a.col[xok ? x : nrv - 1];
+    const double step = a.prf / (double)a.nbins;
+    const double half = 0.5 * a.prf;
+    const double band_hi = half * a.beta + cc.dc_off;
+    const double band_lo = -half * a.beta + cc.dc_off;
+    float w[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
+    float2 pre0, pre1;
+    auto fetch = [&](int q) {
+        const float2* r = a.img + (long)q * a.stride;
+        const int xa = x0 - 8 + t, xb = x0 - 8 + t + kRtCols;
+        pre0 = (xa >= 0 && xa < nrv) ? r[xa] : make_float2(0.f, 0.f);
+        pre1 = (t < kRtSeg - kRtCols && xb < nrv) ? r[xb] : make_float2(0.f, 0.f);
+    };
+    auto deposit = [&](int buf) {
+        seg[buf][t] = pre0;
+        if (t < kRtSeg - kRtCols) seg[buf][t + kRtCols] = pre1;
+    };
+    fetch(q0);
+    deposit(0);
+    if (q0 + 1 < q1) fetch(q0 + 1);
+    __syncthreads();
+    for (int q = q0; q < q1; ++q) {
+        const int buf = (q - q0) & 1;
+        if (q + 1 < q1) {
+            deposit(buf ^ 1);  // its readers finished before the barrier that ended the previous row
+            if (q + 2 < q1) fetch(q + 2);
+        }
+        const int k = (a.n2 == 1) ? q : (q / a.n2 + a.n1 * (q % a.n2));
+        const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
+        const double fn = alias_about_dc(fb, cc.dc, a.prf);
+        float2 val;
+        if (a.mode != 2) {
+            const double shift = cc.shift_k * fn * fn;
+            const double p = round(shift);
+            const int pi = (int)fmin(p, 4096.0);
+            const int cx = x + pi;
+            if (cx - 8 < 0 || cx + 8 >= nrv) {
+                val = make_float2(0.f, 0.f);
+            } else {
+                const float d = (float)(shift - p);
+                // the gate guarantees pi <= kRcmcTileReach; the clamp only keeps the taps inside the segment
+                const float2* tap = &seg[buf][min(t + pi, kRtSeg - 16)];
+                if (d == 0.0f) {
+                    val = tap[8];
+                } else {
+                    float2 acc = make_float2(0.f, 0.f);
+                    float den = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const float wi = w[i] * __builtin_amdgcn_rcpf(d + (float)(8 - i));
+                        const float2 tp = tap[i];
+                        acc.x += wi * tp.x;
+                        acc.y += wi * tp.y;
+                        den += wi;
+                    }
+                    const float inv = __builtin_amdgcn_rcpf(den);
+                    val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
+                }
+            }
+        } else {
+            val = seg[buf][t + 8];
+        }
+        if (a.mode != 1) {
+            if (fn < band_lo || fn > band_hi) {
+                val = make_float2(0.f, 0.f);
+            } else {
+                double fc = fb - cc.dc_off;
+                if (fc > half)
+                    fc -= a.prf;
+                else if (fc < -half)
+                    fc += a.prf;
+                const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
+                const float fr = (float)(turns - rint(turns));
+                float sn, cs;
+                sincospif(2.0f * fr, &sn, &cs);
+                val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
+            }
+        }
+        if (xok) a.out[(long)q * a.stride + x] = val;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s) {
+    ensure_window(s);
+    if (!a.out || a.out == a.img || a.interp != 0) return hipErrorInvalidValue;  // out of place, 16-tap sinc only
+    hipLaunchKernelGGL(k_rcmc_tile, dim3((a.nrv + kRtCols - 1) / kRtCols, (a.nbins + kRtRows - 1) / kRtRows),
+                       dim3(kRtCols), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s) {

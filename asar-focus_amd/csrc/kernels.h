@@ -21,6 +21,8 @@ constexpr int kMaxDcBlocks = 64;
 // Largest integer RCMC shift (pixels) the fused azimuth kernel's range halo covers; above it the plan runs the
 // separate RCMC/matched-filter and inverse-pass kernels instead.
 constexpr int kAzFusedReach = 8;
+// Largest integer RCMC shift (pixels) covered by the column-tiled RCMC kernel's right halo.
+constexpr int kRcmcTileReach = 24;
 // Default upper bound on the second-pass length of the column split when the fused azimuth kernel runs (its
 // LDS tile is n2 x 41 complex samples, so a shorter second pass fits more workgroups per CU); 0 = balanced.
 constexpr int kAzFusedN2Max = 0;
@@ -116,6 +118,8 @@ struct RcmcArgs {
     // interpolator: 0 = 16-tap Blackman sinc (the reference's), 1 = learned 8-tap table where its judge cell
     // passed (16-tap sinc elsewhere), 2 = plain 8-tap Blackman sinc
     int interp;
+    float2* out;                  // nullptr: in place on img
+    const int* tile_gate;         // non-null: the row kernel does nothing when *tile_gate <= kRcmcTileReach
     const float2* lut;            // [ns][nd][8] complex taps
     const unsigned char* lut_ok;  // [ns][judge_nd]
     int lut_nd, lut_ns, judge_nd;
@@ -219,6 +223,7 @@ hipError_t launch_col_mfma(const ColFftArgs& a, bool inverse, hipStream_t s);
 bool launch_col_static(const ColFftArgs& a, bool inverse, hipStream_t s, hipError_t* err);
 void set_col_static_enabled(bool on);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
+hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_az_fused(const AzFusedArgs& a, hipStream_t s);
 size_t az_fused_lds_bytes(int n2);
 void rcmc_window(float w[16]);

@@ -43,7 +43,8 @@ struct asarf_plan {
     ColSplit cs;
     // device buffers
     float2* img = nullptr;
-    float2* img2 = nullptr;  // out-of-place target of the fused azimuth core (nullptr: in-place path)
+    float2* img2 = nullptr;  // out-of-place target of the RCMC stage
+    bool fused = false;      // fused RCMC + matched filter + inverse pass B' (opt-in)
     float2* hconj = nullptr;
     ColConst* col = nullptr;
     long long* partials = nullptr;
@@ -151,9 +152,9 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
         clear_error();
     }
     if ((st = plan_alloc(pl, (void**)&pl->img, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
-    if (fz_on && az_fused_lds_bytes(pl->cs.n2) <= c->max_lds) {
-        if ((st = plan_alloc(pl, (void**)&pl->img2, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
-    }
+    // second image: out-of-place target of the RCMC stage (the column-tiled RCMC kernel and the fused core)
+    pl->fused = fz_on && az_fused_lds_bytes(pl->cs.n2) <= c->max_lds;
+    if ((st = plan_alloc(pl, (void**)&pl->img2, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->hconj, sizeof(float2) * (size_t)nrp))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->col, sizeof(ColConst) * (size_t)nrv))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->partials,
@@ -308,7 +309,7 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     ca.r_first = p.slant_range_first;
     ca.prf = p.prf;
     ca.nbins = pl->nap;
-    ca.reach = pl->img2 ? &pl->res->rcmc_reach : nullptr;
+    ca.reach = &pl->res->rcmc_reach;
     HIP_CHECK(launch_col_const(ca, s));
     ST_CHECK(mark(pl, "column_constants"));
     // 2. fused range compression
@@ -378,7 +379,7 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
         rc.judge_nd = pl->net->judge_nd;
     }
     for (int i = 0; i < 8; ++i) rc.window8[i] = blackman_value(i, 8);
-    if (pl->img2 && pl->rcmc_interp == 0 && !pl->fft_mfma) {
+    if (pl->fused && pl->rcmc_interp == 0 && !pl->fft_mfma) {
         const ColSplit& cs = pl->cs;
         const int n = cs.n;
         // 4. fused RCMC + matched filter + inverse pass B' (img -> img2) when the RCMC shift bound computed on
@@ -415,8 +416,35 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
                                  p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
         HIP_CHECK(launch_col_fft(ia, true, s));
         ST_CHECK(mark(pl, "azimuth_ifft_a"));
+    } else if (pl->rcmc_interp == 0) {
+        // 4. RCMC + azimuth matched filter (img -> img2): the column-tiled kernel when the device-side shift bound
+        // fits its halo, else the row kernel; both are enqueued and the bound makes exactly one of them work
+        const ColSplit& cs = pl->cs;
+        const int n = cs.n;
+        rc.out = pl->img2;
+        rc.tile_gate = &pl->res->rcmc_reach;
+        HIP_CHECK(launch_rcmc_tile(rc, s));
+        HIP_CHECK(launch_rcmc_mf(rc, s));
+        ST_CHECK(mark(pl, "rcmc_matched_filter"));
+        // 5. azimuth IFFT (img2 -> img), 1/Nap, rows < Na
+        if (cs.n2 == 1) {
+            ColFftArgs ia = col_args(pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, 1, 0, 1, n, 0, 1, p.na,
+                                     nullptr, 1.0f / (float)pl->nap, cs.p1);
+            ia.mfma = pl->fft_mfma;
+            HIP_CHECK(launch_col_fft(ia, true, s));
+            ST_CHECK(mark(pl, "azimuth_ifft_b"));
+        } else {
+            ColFftArgs ib = col_args(pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, cs.n1, cs.n2, 1, n, cs.n2,
+                                     1, n, cs.full->d_tw, 1.f, cs.p2);
+            ib.mfma = pl->fft_mfma;
+            HIP_CHECK(launch_col_fft(ib, true, s));
+            ST_CHECK(mark(pl, "azimuth_ifft_b"));
+            ST_CHECK(col_inverse_pass(c, cs, 1, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap,
+                                      pl->fft_mfma));
+            ST_CHECK(mark(pl, "azimuth_ifft_a"));
+        }
     } else {
-        // 4. RCMC + azimuth matched filter
+        // 4. RCMC (learned or 8-tap interpolator) + azimuth matched filter, in place
         HIP_CHECK(launch_rcmc_mf(rc, s));
         ST_CHECK(mark(pl, "rcmc_matched_filter"));
         // 5. azimuth IFFT, 1/Nap, rows < Na
@@ -505,8 +533,9 @@ extern "C" int asarf_plan_results(asarf_plan* pl, asarf_focus_results* r) {
     r->crop_near = pl->near;
     r->crop_far = pl->far;
     r->imp_rows = pl->p.imp ? d.imp_rows : 0;
-    r->rcmc_shift_bound = pl->img2 ? d.rcmc_reach : 0;
-    r->azimuth_core_fused = (pl->img2 && d.rcmc_reach <= kAzFusedReach) ? 1 : 0;
+    r->rcmc_shift_bound = d.rcmc_reach;
+    const bool fused_ran = pl->fused && pl->rcmc_interp == 0 && !pl->fft_mfma && d.rcmc_reach <= kAzFusedReach;
+    r->azimuth_core_fused = fused_ran ? 1 : 0;
     return clear_error();
 }
 

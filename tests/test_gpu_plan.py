@@ -130,3 +130,17 @@ def test_plan_large_migration_takes_separate_path(ctx):
     err = rel_rms(slc, ref)
     print(f"large migration slc rel rms {err:.3e} (shift bound {res['rcmc_shift_bound']})")
     assert err < SLC_TOL
+
+
+@pytest.mark.parametrize("wavelength", [0.236, 0.32])
+def test_plan_tiled_and_row_rcmc_kernels(ctx, wavelength):
+    """Default path: the column-tiled RCMC kernel covers shifts up to 24 pixels; beyond that the device-side
+    bound hands the work to the row kernel. Both must reproduce the oracle."""
+    cfg = configs.ers_block().scaled(na=700, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6),
+                                     wavelength=wavelength)
+    raw, slc, res, _ = run_plan(ctx, cfg)
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    err = rel_rms(slc, ref)
+    print(f"wavelength {wavelength}: shift bound {res['rcmc_shift_bound']}, slc rel rms {err:.3e}")
+    assert (res["rcmc_shift_bound"] <= 24) == (wavelength < 0.3)
+    assert err < SLC_TOL
