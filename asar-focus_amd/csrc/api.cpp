@@ -75,6 +75,9 @@ int ctx_init(asarf_ctx* c) {
     // ASARF_COL_STATIC=0 keeps the runtime-radix column kernels for every length (comparison runs)
     const char* cs_env = std::getenv("ASARF_COL_STATIC");
     set_col_static_enabled(!(cs_env && cs_env[0] == '0'));
+    // ASARF_COL_WIDTH=16|32 forces the column tile width of the compile-time column kernels (default: by pass)
+    const char* cw_env = std::getenv("ASARF_COL_WIDTH");
+    set_col_static_width(cw_env ? std::atoi(cw_env) : 0);
     c->initialized = true;
     return ASARF_OK;
 }

@@ -1,6 +1,6 @@
 // Column transforms with compile-time radix sequences for the production sub-lengths of the four-step azimuth
-// FFT. Same workgroup layout and global access as k_col_fft (16 columns x 16 lanes, LDS rows of 17 elements),
-// but two large radices (composite DFTs in registers) replace three or four runtime-dispatched passes: one LDS
+// FFT. Same workgroup layout and global access as k_col_fft (16 lanes per column, 16 or 32 columns, LDS rows of
+// COLS + 1 elements), but two large radices (composite DFTs in registers) replace three or four runtime-dispatched passes: one LDS
 // exchange per transform instead of two or three, and compile-time index arithmetic.
 //   168 = 12 x 14, 180 = 12 x 15   (ASAR scene, 30240 = 168 x 180)
 //   84 = 12 x 7, 49 = 7 x 7        (ERS block, 4116 = 84 x 49)
@@ -10,21 +10,23 @@
 
 namespace asarf {
 
-template <bool INV, int N, class Seq>
-__global__ __launch_bounds__(kThreads) void k_col_static(ColFftArgs a) {
+template <bool INV, int N, class Seq, int COLS>
+__global__ __launch_bounds__(16 * COLS) void k_col_static(ColFftArgs a) {
     static_assert(Seq::product() == N, "radix sequence must factor N");
     extern __shared__ float2 lds[];
     if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
-    constexpr int L = kThreads / kColTile;  // lanes per column
+    constexpr int L = 16;         // lanes per column
+    constexpr int T = L * COLS;   // threads per workgroup
     constexpr int E = seq_max_elems<N, Seq, L>();
-    constexpr int P = kColTile + 1;
-    const int tile = blockIdx.x % a.ntiles;
-    const int g = blockIdx.x / a.ntiles;
-    const int b = threadIdx.x & (kColTile - 1);
-    const int lane = threadIdx.x / kColTile;
-    const int col = tile * kColTile + b;
+    constexpr int P = COLS + 1;
+    const int ntiles = (a.ncols + COLS - 1) / COLS;
+    const int tile = blockIdx.x % ntiles;
+    const int g = blockIdx.x / ntiles;
+    const int b = threadIdx.x & (COLS - 1);
+    const int lane = threadIdx.x / COLS;
+    const int col = tile * COLS + b;
     const bool col_ok = col < a.ncols;
-    const TwLds tw_at = tw_fill(lds + N * P, a.sub.tw, N, threadIdx.x, kThreads);
+    const TwLds tw_at = tw_fill(lds + N * P, a.sub.tw, N, threadIdx.x, T);
     float2 v[E];
     const float2* in = a.in + (col_ok ? col : 0);
     const long in_base = (long)g * a.in_g;
@@ -70,16 +72,29 @@ __global__ __launch_bounds__(kThreads) void k_col_static(ColFftArgs a) {
     pass_visit<RL>(v, lane, L, N, out_write);
 }
 
+// Columns per workgroup: 0 = by access pattern (32 where the transform's elements are strided rows, as in passes
+// A and A' of the four-step split, so each row access is 256 contiguous bytes; 16 for contiguous sub-transforms,
+// whose 16-column tiles measured faster), else forced to 16 or 32.
+static int g_col_width = 0;
+void set_col_static_width(int cols) { g_col_width = (cols == 16 || cols == 32) ? cols : 0; }
+
+template <int N, class Seq, int COLS>
+static hipError_t launch_cs_w(const ColFftArgs& a, bool inverse, hipStream_t s) {
+    const size_t lds = sizeof(float2) * (size_t)(N * (COLS + 1) + tw_lds_elems(N));
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
+    const long ntiles = (a.ncols + COLS - 1) / COLS;
+    const dim3 grid((unsigned)(ntiles * a.groups));
+    if (inverse)
+        hipLaunchKernelGGL((k_col_static<true, N, Seq, COLS>), grid, dim3(16 * COLS), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_col_static<false, N, Seq, COLS>), grid, dim3(16 * COLS), lds, s, a);
+    return hipGetLastError();
+}
+
 template <int N, class Seq>
 static hipError_t launch_cs(const ColFftArgs& a, bool inverse, hipStream_t s) {
-    const size_t lds = sizeof(float2) * (size_t)(N * (kColTile + 1) + tw_lds_elems(N));
-    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((long)a.ntiles * a.groups));
-    if (inverse)
-        hipLaunchKernelGGL((k_col_static<true, N, Seq>), grid, dim3(kThreads), lds, s, a);
-    else
-        hipLaunchKernelGGL((k_col_static<false, N, Seq>), grid, dim3(kThreads), lds, s, a);
-    return hipGetLastError();
+    const int w = g_col_width ? g_col_width : (a.in_m > 1 ? 32 : 16);
+    return w == 32 ? launch_cs_w<N, Seq, 32>(a, inverse, s) : launch_cs_w<N, Seq, 16>(a, inverse, s);
 }
 
 // Returns true (and the launch status in *err) when a compile-time kernel exists for the sub-transform length.

@@ -153,57 +153,83 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Workgroup (tile, strip): lane t owns column 256*tile + t over rows [128*strip, +128) and accumulates exact
-// integer sums; lag-1 products pair each row with the one above (the strip re-reads its preceding row).
-// Statistics are reduced over the workgroup; lag sums per Doppler block (slot = block - first block of the tile).
-template <typename P>
+// Workgroup (tile, strip): lane t owns columns [V*(256*tile + t), +V) over rows [128*strip, +128) and
+// accumulates exact integer sums; lag-1 products pair each row with the one above (the strip re-reads its
+// preceding row). With V = 4 a row of a lane is one 8-byte load (4 I/Q byte pairs). Statistics are reduced over
+// the workgroup; lag sums per Doppler block (slot = block - first block of the tile) through per-column LDS rows.
+// Integer ranges: a column's sums over 128 rows stay below 128 * 255^2 < 2^31.
+template <typename P, int V>
 __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
+    constexpr bool SIGNED = std::is_same<P, char2>::value;
+    constexpr int TW = kThreads * V;
     const int tile = blockIdx.x;
     const int strip = blockIdx.y;
-    const int x = tile * kThreads + threadIdx.x;
+    const int x0 = (tile * kThreads + threadIdx.x) * V;
     const int y0 = strip * kStripRows;
     const int y1 = min(y0 + kStripRows, a.na);
-    const P* raw = static_cast<const P*>(a.raw);
+    const unsigned char* raw = static_cast<const unsigned char*>(a.raw);
+    const int nv = min(V, max(0, a.nr - x0));  // valid columns of this lane: V or 0 (nr % V == 0)
     int st[5] = {0, 0, 0, 0, 0};
-    int lg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int n = 0, m = 0;
-    if (x < a.nr) {
+    int lg[V][8];
+#pragma unroll
+    for (int j = 0; j < V; ++j)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) lg[j][i] = 0;
+    int rows = 0, lag_rows = 0;
+    // one row of the lane's V samples as raw bytes (V = 4: one 8-byte word; V = 1: one I/Q pair), unpacked later
+    using W = typename std::conditional<V == 4, uint2, unsigned short>::type;
+    auto load_row = [&](int y) -> W { return *reinterpret_cast<const W*>(raw + ((long)y * a.nr + x0) * 2); };
+    auto sample = [&](const W& w, int j, int& si, int& sq) {
+        unsigned int word;
+        if constexpr (V == 4)
+            word = (j < 2 ? w.x : w.y) >> ((j & 1) * 16);
+        else
+            word = w;
+        const unsigned int bi = word & 0xff, bq = (word >> 8) & 0xff;
+        si = SIGNED ? (int)(signed char)bi : (int)bi;
+        sq = SIGNED ? (int)(signed char)bq : (int)bq;
+    };
+    if (nv == V) {
         constexpr int U = 8;  // rows loaded per iteration (independent loads in flight)
-        int pi = 0, pq = 0;
+        int pi[V], pq[V];
         bool have_prev = false;
         if (y0 > 0) {
-            const P p = raw[(long)(y0 - 1) * a.nr + x];
-            pi = p.x;
-            pq = p.y;
+            const W w = load_row(y0 - 1);
+#pragma unroll
+            for (int j = 0; j < V; ++j) sample(w, j, pi[j], pq[j]);
             have_prev = true;
         }
         for (int y = y0; y < y1; y += U) {
-            P c[U];
+            W w[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) c[u] = (y + u < y1) ? raw[(long)(y + u) * a.nr + x] : P{0, 0};
+            for (int u = 0; u < U; ++u) w[u] = (y + u < y1) ? load_row(y + u) : W{};
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (y + u >= y1) break;
-                const int ci = c[u].x, cq = c[u].y;
-                st[0] += ci;
-                st[1] += cq;
-                st[2] += ci * ci;
-                st[3] += cq * cq;
-                st[4] += ci * cq;
-                ++n;
-                if (have_prev) {
-                    lg[0] += pi * ci;
-                    lg[1] += pq * cq;
-                    lg[2] += pi * cq;
-                    lg[3] += pq * ci;
-                    lg[4] += pi;
-                    lg[5] += pq;
-                    lg[6] += ci;
-                    lg[7] += cq;
-                    ++m;
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    int c_i, c_q;
+                    sample(w[u], j, c_i, c_q);
+                    st[0] += c_i;
+                    st[1] += c_q;
+                    st[2] += c_i * c_i;
+                    st[3] += c_q * c_q;
+                    st[4] += c_i * c_q;
+                    if (have_prev) {
+                        lg[j][0] += pi[j] * c_i;
+                        lg[j][1] += pq[j] * c_q;
+                        lg[j][2] += pi[j] * c_q;
+                        lg[j][3] += pq[j] * c_i;
+                        lg[j][4] += pi[j];
+                        lg[j][5] += pq[j];
+                        lg[j][6] += c_i;
+                        lg[j][7] += c_q;
+                    }
+                    pi[j] = c_i;
+                    pq[j] = c_q;
                 }
-                pi = ci;
-                pq = cq;
+                ++rows;
+                if (have_prev) ++lag_rows;
                 have_prev = true;
             }
         }
@@ -212,16 +238,21 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
     long long* out = a.partials + ((long)strip * a.ntiles + tile) * nval;
     // statistics: wave sums, then across the 4 waves
     __shared__ long long shs[kThreads / 64][kRawStats];
-    __shared__ long long shl[kRawLag][kThreads];
+    __shared__ int shl[kRawLag][TW];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
     for (int i = 0; i < kRawStats; ++i) {
-        const long long v = wave_sum<long long>(i < 5 ? (long long)st[i] : (long long)n);
+        const long long v = wave_sum<long long>(i < 5 ? (long long)st[i] : (long long)rows * nv);
         if (lane == 0) shs[wave][i] = v;
     }
+    // column-major per lane: column c of the tile is lane c / V, slot c % V
 #pragma unroll
-    for (int i = 0; i < 8; ++i) shl[i][threadIdx.x] = lg[i];
-    shl[8][threadIdx.x] = m;
+    for (int j = 0; j < V; ++j) {
+        const int c = threadIdx.x * V + j;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) shl[i][c] = lg[j][i];
+        shl[8][c] = nv ? lag_rows : 0;
+    }
     __syncthreads();
     if (threadIdx.x < kRawStats) {
         long long t = 0;
@@ -229,8 +260,8 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
         for (int w = 0; w < kThreads / 64; ++w) t += shs[w][threadIdx.x];
         out[threadIdx.x] = t;
     }
-    // lag sums per block slot: lanes of one block form a contiguous range
-    const int c0 = tile * kThreads;
+    // lag sums per block slot: columns of one block form a contiguous range
+    const int c0 = tile * TW;
     const int b0 = c0 / a.step;
     for (int k = threadIdx.x; k < a.nslots * kRawLag; k += kThreads) {
         const int slot = k / kRawLag, v = k % kRawLag;
@@ -238,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
         long long t = 0;
         if (b < a.nblocks) {
             const int l0 = max(0, b * a.step - c0);
-            const int l1 = min(kThreads, (b + 1) * a.step - c0);
+            const int l1 = min(TW, (b + 1) * a.step - c0);
             for (int l = l0; l < l1; ++l) t += shl[v][l];
         }
         out[kRawStats + k] = t;
@@ -247,12 +278,22 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
 
 hipError_t launch_raw_moments(const RawMomentsArgs& a, int fmt, hipStream_t s) {
     const dim3 grid(a.ntiles, a.nstrips);
-    if (fmt == RAW_U8)
-        hipLaunchKernelGGL(k_raw_moments<uchar2>, grid, dim3(kThreads), 0, s, a);
-    else if (fmt == RAW_I8)
-        hipLaunchKernelGGL(k_raw_moments<char2>, grid, dim3(kThreads), 0, s, a);
-    else
+    if (a.vec != 1 && a.vec != 4) return hipErrorInvalidValue;
+    // 8-byte loads: rows and the base must be 8-byte aligned
+    if (a.vec == 4 && (a.nr % 4 != 0 || (reinterpret_cast<uintptr_t>(a.raw) & 7) != 0)) return hipErrorInvalidValue;
+    if (fmt == RAW_U8) {
+        if (a.vec == 4)
+            hipLaunchKernelGGL((k_raw_moments<uchar2, 4>), grid, dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_raw_moments<uchar2, 1>), grid, dim3(kThreads), 0, s, a);
+    } else if (fmt == RAW_I8) {
+        if (a.vec == 4)
+            hipLaunchKernelGGL((k_raw_moments<char2, 4>), grid, dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_raw_moments<char2, 1>), grid, dim3(kThreads), 0, s, a);
+    } else {
         return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -268,13 +309,13 @@ __global__ __launch_bounds__(kThreads) void k_reduce_raw(RawReduceArgs a) {
     } else {
         const int b = (v - kRawStats) / kRawLag, k = (v - kRawStats) % kRawLag;
         // tiles t with b0(t) <= b < b0(t) + nslots
-        const int t_lo = max(0, (b - a.nslots + 1) * a.step / kThreads - 1);
-        const int t_hi = min(a.ntiles - 1, ((b + 1) * a.step) / kThreads + 1);
+        const int t_lo = max(0, (b - a.nslots + 1) * a.step / a.tile_cols - 1);
+        const int t_hi = min(a.ntiles - 1, ((b + 1) * a.step) / a.tile_cols + 1);
         const int nt = t_hi - t_lo + 1;
         const long parts = (long)a.nstrips * nt;
         for (long p = threadIdx.x; p < parts; p += kThreads) {
             const int strip = (int)(p / nt), t = t_lo + (int)(p % nt);
-            const int slot = b - (t * kThreads) / a.step;
+            const int slot = b - (t * a.tile_cols) / a.step;
             if (slot >= 0 && slot < a.nslots) acc += a.partials[((long)strip * a.ntiles + t) * nval + kRawStats + slot * kRawLag + k];
         }
     }

@@ -149,8 +149,9 @@ struct RawMomentsArgs {
     int nr, na;
     int step;         // DC block width
     int nblocks;      // Doppler blocks (columns at or beyond nblocks*step only enter the statistics)
-    int ntiles;       // 256-column tiles
-    int nslots;       // Doppler blocks a tile can touch: 255/step + 2
+    int vec;          // adjacent columns per lane (4 when nr % 4 == 0: one 8-byte load per row, else 1)
+    int ntiles;       // tiles of 256*vec columns
+    int nslots;       // Doppler blocks a tile can touch: (256*vec - 1)/step + 2
     int nstrips;
     long long* partials; // [strip][tile][kRawStats + nslots*kRawLag]
 };
@@ -158,6 +159,7 @@ struct RawMomentsArgs {
 struct RawReduceArgs {
     const long long* partials;
     int nstrips, ntiles, nslots, step, nblocks;
+    int tile_cols;      // 256 * vec
     long long* totals;  // [kRawStats + nblocks*kRawLag]
 };
 
@@ -222,6 +224,7 @@ hipError_t launch_row_mfma(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_col_mfma(const ColFftArgs& a, bool inverse, hipStream_t s);
 bool launch_col_static(const ColFftArgs& a, bool inverse, hipStream_t s, hipError_t* err);
 void set_col_static_enabled(bool on);
+void set_col_static_width(int cols);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_az_fused(const AzFusedArgs& a, hipStream_t s);

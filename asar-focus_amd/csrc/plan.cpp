@@ -38,7 +38,7 @@ struct asarf_plan {
     asarf_focus_params p{};
     int nrp = 0, nap = 0, nrv = 0;
     long stride = 0;
-    int ntiles = 0, nblocks = 0, step = 0, nstrips = 0, nslots = 0;
+    int nblocks = 0, step = 0, nstrips = 0, raw_vec = 1;
     const FftPlanHost* rplan = nullptr;
     ColSplit cs;
     // device buffers
@@ -68,6 +68,14 @@ struct asarf_plan {
     std::vector<const char*> ev_names;
     std::vector<float> last_ms;
 };
+
+// Tiling of the raw statistics pass at `vec` columns per lane: tiles of 256*vec columns, and the Doppler blocks
+// one tile can touch.
+static void raw_tiles(const asarf_plan* pl, int vec, int* ntiles, int* nslots) {
+    const int tw = kThreads * vec;
+    *ntiles = (pl->p.nr + tw - 1) / tw;
+    *nslots = (tw - 1) / pl->step + 2;
+}
 
 static int plan_alloc(asarf_plan* pl, void** p, size_t bytes) {
     HIP_CHECK(hipMalloc(p, bytes));
@@ -124,8 +132,7 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     pl->stride = (nrv + kColTile - 1) / kColTile * kColTile;
     pl->step = step;
     pl->nblocks = nblocks;
-    pl->ntiles = (p->nr + kThreads - 1) / kThreads;
-    pl->nslots = (kThreads - 1) / step + 2;
+    pl->raw_vec = (p->nr % 4 == 0) ? 4 : 1;  // 4 when the raw rows allow 8-byte loads
     pl->nstrips = (p->na + kStripRows - 1) / kStripRows;
     auto fail = [&](int st) {
         plan_free(pl);
@@ -157,9 +164,13 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     if ((st = plan_alloc(pl, (void**)&pl->img2, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->hconj, sizeof(float2) * (size_t)nrp))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->col, sizeof(ColConst) * (size_t)nrv))) return fail(st);
-    if ((st = plan_alloc(pl, (void**)&pl->partials,
-                         sizeof(long long) * (size_t)pl->nstrips * pl->ntiles * (kRawStats + pl->nslots * kRawLag))))
-        return fail(st);
+    size_t partial_vals = 0;  // the larger of the two tilings (the width is chosen per call from the raw alignment)
+    for (int v : {1, pl->raw_vec}) {
+        int nt = 0, ns = 0;
+        raw_tiles(pl, v, &nt, &ns);
+        partial_vals = std::max(partial_vals, (size_t)pl->nstrips * nt * (kRawStats + ns * kRawLag));
+    }
+    if ((st = plan_alloc(pl, (void**)&pl->partials, sizeof(long long) * partial_vals))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->totals, sizeof(long long) * (size_t)(kRawStats + nblocks * kRawLag))))
         return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->res, sizeof(DevResults)))) return fail(st);
@@ -265,8 +276,8 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     rm.na = p.na;
     rm.step = pl->step;
     rm.nblocks = pl->nblocks;
-    rm.ntiles = pl->ntiles;
-    rm.nslots = pl->nslots;
+    rm.vec = (pl->raw_vec == 4 && (reinterpret_cast<uintptr_t>(raw) & 7) == 0) ? 4 : 1;
+    raw_tiles(pl, rm.vec, &rm.ntiles, &rm.nslots);
     rm.nstrips = pl->nstrips;
     rm.partials = pl->partials;
     HIP_CHECK(launch_raw_moments(rm, p.raw_format, s));
@@ -274,8 +285,9 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     RawReduceArgs rr{};
     rr.partials = pl->partials;
     rr.nstrips = pl->nstrips;
-    rr.ntiles = pl->ntiles;
-    rr.nslots = pl->nslots;
+    rr.ntiles = rm.ntiles;
+    rr.tile_cols = kThreads * rm.vec;
+    rr.nslots = rm.nslots;
     rr.step = pl->step;
     rr.nblocks = pl->nblocks;
     rr.totals = pl->totals;

@@ -252,8 +252,8 @@ int asarf_plan_set_fft(asarf_plan* plan, int precision);
 /* ------------------------------------------------------------------------------- autotuner and batches */
 /* No counterpart in the reference. Measures the candidate schedules of one transform length on the device and
  * records the fastest in the context; plans created afterwards use it. ROWS: range compression of `batch`
- * lines of length n (the compile-time kernel where one exists, runtime radix orders); *choice = -1 for the
- * compile-time kernel, else the candidate index. COLUMNS: forward azimuth passes of n x batch; *choice = n1 of
+ * lines of length n (the compile-time variants where the length has them, runtime radix orders); *choice = -1 - v
+ * for compile-time variant v, else the runtime candidate index. COLUMNS: forward azimuth passes of n x batch; *choice = n1 of
  * the chosen split (n = single pass). *best_ms = its time, *n_candidates = candidates measured. Synchronises. */
 int asarf_autotune_fft(asarf_ctx* ctx, int n, int layout, int batch, int reps, int* choice, float* best_ms,
                        int* n_candidates);

@@ -144,3 +144,27 @@ def test_plan_tiled_and_row_rcmc_kernels(ctx, wavelength):
     print(f"wavelength {wavelength}: shift bound {res['rcmc_shift_bound']}, slc rel rms {err:.3e}")
     assert (res["rcmc_shift_bound"] <= 24) == (wavelength < 0.3)
     assert err < SLC_TOL
+
+
+@pytest.mark.parametrize("nr,offset", [(1400, 0), (1400, 2), (1402, 0)])
+def test_plan_statistics_load_width(ctx, nr, offset):
+    """The raw statistics pass reads 4 samples per lane with 8-byte loads when nr % 4 == 0 and the raw buffer is
+    8-byte aligned, one sample otherwise: both give the oracle's I/Q statistics, Doppler centroid and SLC."""
+    cfg = configs.asar_scene().scaled(na=640, nr=nr, n_targets_random=4)
+    raw, _ = scene.generate(cfg)
+    buf = torch.empty(raw.size + 16, dtype=torch.uint8, device="cuda")
+    traw = buf[offset:offset + raw.size]
+    traw.copy_(torch.from_numpy(raw.reshape(-1).view(np.uint8)))
+    plan = focus.Plan(ctx, cfg)
+    plan.execute(traw)
+    slc = plan.slc_tensor().cpu().numpy()
+    res = plan.results()
+    plan.close()
+    ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    err = rel_rms(slc, ref)
+    print(f"nr={nr} offset={offset} slc rel rms {err:.3e}")
+    assert err < SLC_TOL
+    for k, v in rres["iq"].items():
+        assert abs(res["iq"][k] - v) <= 1e-9 * max(1.0, abs(v)), k
+    xs = np.arange(0, cfg.nr - cfg.nc)
+    assert np.abs(np.polyval(res["dc_poly"], xs) - np.polyval(rres["dc_poly"], xs)).max() < 1e-6
