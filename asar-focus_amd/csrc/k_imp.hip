@@ -9,6 +9,7 @@ namespace asarf {
 // Each lane owns one range column of a tile of `rows` output rows. The |s|^2 of every input row the tile needs
 // (at most kMlSpan) is loaded first, all loads independent and in flight together, into the lane's own LDS
 // column (no barrier: a lane only reads what it wrote); the boxcars and the interpolation then run from LDS.
+// (Two columns per lane with 16-byte loads measured slower: the LDS per wave doubles and halves the waves per CU.)
 constexpr int kMlSpan = 32;     // input rows per tile
 constexpr int kMlLanes = 256;
 
@@ -28,11 +29,13 @@ __global__ __launch_bounds__(kMlLanes) void k_multilook(MultilookArgs a, int row
     const int span = min(hi - lo, kMlSpan);  // the host sizes tiles so that hi - lo <= kMlSpan
     const bool xok = x < a.nr;
     const float2* col = a.slc + (long)(row0 + lo) * a.stride + a.col0 + (xok ? x : 0);
-#pragma unroll 8
-    for (int i = 0; i < span; ++i) {
-        const float2 v = col[(long)i * a.stride];
-        pw[i][threadIdx.x] = v.x * v.x + v.y * v.y;
-    }
+    float2 v[kMlSpan];
+#pragma unroll
+    for (int i = 0; i < kMlSpan; ++i)
+        if (i < span) v[i] = col[(long)i * a.stride];
+#pragma unroll
+    for (int i = 0; i < kMlSpan; ++i)
+        if (i < span) pw[i][threadIdx.x] = v[i].x * v[i].x + v[i].y * v[i].y;
     if (!xok) return;
     for (int y = y0; y < y1; ++y) {
         const float in_idx = (float)y * a.ratio;

@@ -69,8 +69,9 @@ def test_plan_ers_block_full_size_parity_and_quality(ctx):
     assert worst["pslr"] < 0.05 and worst["islr"] < 0.05 and worst["irw"] < 0.01
 
 
-def test_plan_imp_matches_oracle_chain(ctx):
-    cfg = configs.asar_scene().scaled(na=1200, nr=1400, n_targets_random=8)
+@pytest.mark.parametrize("nr", [1400, 1401])  # even / odd multilook width (two columns per lane / one)
+def test_plan_imp_matches_oracle_chain(ctx, nr):
+    cfg = configs.asar_scene().scaled(na=1200, nr=nr, n_targets_random=8)
     raw, slc, res, imp = run_plan(ctx, cfg, imp=True)
     ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg)
     front, back, near, far = res["crop"]

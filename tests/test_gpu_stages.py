@@ -147,11 +147,12 @@ def test_range_doppler_adopts_and_trades_workspace(ctx):
 
 
 # ------------------------------------------------------------------------------------------- IMP tail ----
-def test_multilook_rsl_srgr_match_oracle(ctx):
+@pytest.mark.parametrize("width", [400, 399])  # two columns per lane (16-byte loads) / one column per lane
+def test_multilook_rsl_srgr_match_oracle(ctx, width):
     rng = np.random.default_rng(11)
-    slc = rand_c((900, 400), 5)
+    slc = rand_c((900, width), 5)
     ml_r = po.az_multilook(slc.astype(np.complex128), 4.298)
-    ml_g, sp = focus.az_multilook(ctx, cuda(slc), 400, 900, 4.298)
+    ml_g, sp = focus.az_multilook(ctx, cuda(slc), width, 900, 4.298)
     assert ml_g.shape == ml_r.shape
     assert rel_rms(ml_g.cpu().numpy(), ml_r) < 1e-6
     assert abs(sp - 4.298 * np.float32(12.5 / 4.298)) < 1e-9
