@@ -292,12 +292,19 @@ __host__ __device__ inline int plan_radix(const FftPlanDev& p, int pass) {
 // Load phase: v[i*R + r] = src(j + r*n/R) for this thread's butterflies j = s + i*S. Lanes past the last
 // butterfly load the last butterfly again: the register array is written unconditionally, which keeps it in
 // registers (a predicated write invites the compiler to merge writes with a computed index and demote the array).
-template <int R, int MAXE, class Src>
+// A butterfly that is past the end for every lane of the wave is skipped (a wave vote and a scalar branch)
+// instead of computed as a clamped copy; its registers keep stale values that no store or visit uses.
+// VOTE0 = false: the caller guarantees that the first butterfly of every lane is valid (n / R >= S), and the
+// vote is only taken for the later ones.
+__device__ __forceinline__ bool wave_any(bool c) { return __builtin_amdgcn_ballot_w64(c) != 0; }
+
+template <int R, int MAXE, bool VOTE0 = true, class Src>
 __device__ __forceinline__ void pass_load(float2 (&v)[MAXE], int s, int S, int n, Src&& src) {
     constexpr int NB = MAXE / R;
     const int nr = n / R;
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
+        if ((VOTE0 || i > 0) && !wave_any(s + i * S < nr)) break;
         const int j = min(s + i * S, nr - 1);
 #pragma unroll
         for (int r = 0; r < R; ++r) v[i * R + r] = src(j + r * nr);
@@ -312,7 +319,7 @@ __device__ __forceinline__ int fast_div(int j, int ns, float inv_ns) {
 
 // Twiddle + DFT of every butterfly of the thread (pass with prefix product ns); clamped lanes compute a copy.
 // TW(e) returns W_n^e for 0 <= e < n.
-template <int R, int MAXE, bool INV, class TwAt>
+template <int R, int MAXE, bool INV, bool VOTE0 = true, class TwAt>
 __device__ __forceinline__ void pass_compute(float2 (&v)[MAXE], int s, int S, int n, int ns, TwAt&& tw_at) {
     constexpr int NB = MAXE / R;
     const int nr = n / R;
@@ -321,6 +328,7 @@ __device__ __forceinline__ void pass_compute(float2 (&v)[MAXE], int s, int S, in
         const float inv_ns = 1.0f / (float)ns;
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
+            if ((VOTE0 || i > 0) && !wave_any(s + i * S < nr)) break;
             const int j = min(s + i * S, nr - 1);
             const int k = j - fast_div(j, ns, inv_ns) * ns;
             const int base = k * tstride;
@@ -347,7 +355,10 @@ __device__ __forceinline__ void pass_compute(float2 (&v)[MAXE], int s, int S, in
         return;
     }
 #pragma unroll
-    for (int i = 0; i < NB; ++i) Dft<R, INV>::run(&v[i * R]);
+    for (int i = 0; i < NB; ++i) {
+        if ((VOTE0 || i > 0) && !wave_any(s + i * S < nr)) break;
+        Dft<R, INV>::run(&v[i * R]);
+    }
 }
 
 // Store phase: dst((j/ns)*ns*R + j%ns + r*ns, value).

@@ -33,6 +33,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
     // elements per lane: N / T when every radix divides it, else the largest R * ceil((N / R) / T) (lanes
     // beyond a pass's butterfly count compute clamped copies and store nothing)
     constexpr int E = seq_max_elems<N, Seq, T>();
+    // vote on a lane's first butterfly only where some pass has fewer butterflies than lanes
+    constexpr bool V0 = [] {
+        for (int p = 0; p < Seq::count; ++p)
+            if (N / Seq::at(p) < T) return true;
+        return false;
+    }();
     extern __shared__ float2 lds[];
     const int row = blockIdx.x;
     const int lane = threadIdx.x;
@@ -103,20 +109,20 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
             for (int x = lane; x < N; x += T) lds_store(x, gl_load(x));
         }
         __syncthreads();
-        pass_load<R0>(v, lane, T, N, lds_load);
+        pass_load<R0, E, V0>(v, lane, T, N, lds_load);
         __syncthreads();  // every staged element is read before the passes overwrite LDS
     } else {
-        pass_load<R0>(v, lane, T, N, gl_load);
+        pass_load<R0, E, V0>(v, lane, T, N, gl_load);
     }
-    pass_compute<R0, E, false>(v, lane, T, N, 1, tw_at);
+    pass_compute<R0, E, false, V0>(v, lane, T, N, 1, tw_at);
     static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
         constexpr int p = decltype(pc)::value;
         constexpr int Rp = Seq::at(p - 1), R = Seq::at(p);
         pass_store<Rp>(v, lane, T, N, Seq::prefix(p - 1), lds_store);
         __syncthreads();
-        pass_load<R>(v, lane, T, N, lds_load);
+        pass_load<R, E, V0>(v, lane, T, N, lds_load);
         __syncthreads();
-        pass_compute<R, E, false>(v, lane, T, N, Seq::prefix(p), tw_at);
+        pass_compute<R, E, false, V0>(v, lane, T, N, Seq::prefix(p), tw_at);
     });
     // matched filter in the range frequency domain
     constexpr int RL = Seq::at(Seq::count - 1);
@@ -125,15 +131,15 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
     pass_visit<RL>(v, lane, T, N, spec_mul);
     __syncthreads();  // all LDS reads of the forward transform precede the inverse stores
     // inverse transform (radices in reverse order; its first pass consumes the registers as they are)
-    pass_compute<RL, E, true>(v, lane, T, N, 1, tw_at);
+    pass_compute<RL, E, true, V0>(v, lane, T, N, 1, tw_at);
     static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
         constexpr int p = decltype(pc)::value;
         constexpr int Rp = Seq::rat(p - 1), R = Seq::rat(p);
         pass_store<Rp>(v, lane, T, N, Seq::rprefix(p - 1), lds_store);
         __syncthreads();
-        pass_load<R>(v, lane, T, N, lds_load);
+        pass_load<R, E, V0>(v, lane, T, N, lds_load);
         __syncthreads();
-        pass_compute<R, E, true>(v, lane, T, N, Seq::rprefix(p), tw_at);
+        pass_compute<R, E, true, V0>(v, lane, T, N, Seq::rprefix(p), tw_at);
     });
     float2* out = a.out + (long)row * a.out_stride;
     const int nrv = a.nrv, zt = a.zero_to;
@@ -184,8 +190,8 @@ bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipEr
                 case 2: *err = launch_static<6144, 384, 5, 2, RadixSeq<24, 16, 16>>(a, fmt, s); return true;
                 case 3: *err = launch_static<6144, 512, 5, 4, RadixSeq<12, 8, 8, 8>>(a, fmt, s); return true;
                 case 4: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
-                case 5: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
-                default: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
+                case 5: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
+                default: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
             }
         case 6400:
             switch (v) {
