@@ -9,6 +9,7 @@
 //   halo:           RCMC reads x + p - 8 .. x + p + 7 with p = round(shift) in [0, kAzFusedReach]
 // The arithmetic is that of k_rcmc_mf (k_rda.hip) followed by k_col_fft's inverse pass B', operation for
 // operation, so both paths give identical results.
+#include "interp.h"
 #include "kernels.h"
 
 namespace asarf {
@@ -84,6 +85,9 @@ __global__ __launch_bounds__(kThreads) void k_az_fused(AzFusedArgs a) {
     const ColConst cc = a.col[xc_ok ? xc : 0];
     const double step = a.prf / (double)a.nbins;
     const double half = 0.5 * a.prf;
+    float ws[16];  // signed window (-1)^i w[i]
+#pragma unroll
+    for (int i = 0; i < 16; ++i) ws[i] = (i & 1) ? -a.window[i] : a.window[i];
     const double band_hi = half * a.beta + cc.dc_off;
     const double band_lo = -half * a.beta + cc.dc_off;
     auto rcmc_load = [&](int m) -> float2 {
@@ -100,23 +104,7 @@ __global__ __launch_bounds__(kThreads) void k_az_fused(AzFusedArgs a) {
             const int lc = min(max(kFzLeft + b + pi - 8, 0), kFzCols - 16);
             const float2* tap = tl + m * kFzPitch + lc;
             const float d = (float)(shift - p);
-            if (d == 0.0f) {
-                val = tap[8];
-            } else {
-                float2 acc = make_float2(0.f, 0.f);
-                float den = 0.f;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const float wi =
-                        ((i & 1) ? -a.window[i] : a.window[i]) * __builtin_amdgcn_rcpf(d + (float)(8 - i));
-                    const float2 t = tap[i];
-                    acc.x += wi * t.x;
-                    acc.y += wi * t.y;
-                    den += wi;
-                }
-                const float inv = __builtin_amdgcn_rcpf(den);
-                val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
-            }
+            val = (d == 0.0f) ? tap[8] : sinc16_interp(tap, ws, d);
         }
         if (fn < band_lo || fn > band_hi) return make_float2(0.f, 0.f);
         double fc = fb - cc.dc_off;
@@ -127,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void k_az_fused(AzFusedArgs a) {
         const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
         const float frc = (float)(turns - rint(turns));
         float sn, cs;
-        sincospif(2.0f * frc, &sn, &cs);
+        sincos_turns(frc, &sn, &cs);
         return make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
     };
 

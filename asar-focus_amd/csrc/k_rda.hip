@@ -3,9 +3,10 @@
 // RCMC follows sar/range_doppler_algorithm.cu:107-195: shift = lambda^2 f^2 R0 / (8 Vr^2 dr), integer part by
 // round-half-away, 16-tap Blackman-windowed sinc normalised by the sum of its weights, zero when a tap leaves
 // [0, Nrv). The weights use sinc(d + 8 - i) = (-1)^i sin(pi d) / (pi (d + 8 - i)); the common factor
-// sin(pi d)/pi cancels in the normalisation, so no transcendental is evaluated per tap.
+// sin(pi d)/pi cancels in the normalisation, so no transcendental is evaluated per tap (sinc16_interp).
 // The matched filter follows range_doppler_algorithm.cu:197-309 with FP64 geometry and phase (reduced to turns
-// before the single-precision sincospi).
+// before the hardware sine/cosine, whose argument is in turns).
+#include "interp.h"
 #include "kernels.h"
 #include "poly.h"
 #include "rcmc_net.h"
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8)
     const double step = a.prf / (double)a.nbins;
     const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
     const double half = 0.5 * a.prf;
-    float w[16];
+    float w[16];  // signed window (-1)^i w[i]
 #pragma unroll
     for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
     for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) {
@@ -101,7 +102,7 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8)
         float2 val;
         if (a.mode != 2) {
             const double shift = cc.shift_k * fn * fn;
-            const double p = round(shift);
+            const double p = floor(shift + 0.5);  // round half away from zero: shift >= 0
             const int cx = x + (int)p;
             if (cx - 8 < 0 || cx + 8 >= nrv) {
                 val = make_float2(0.f, 0.f);
@@ -146,27 +147,7 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8)
                         val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
                     }
                 }
-                if (sinc16) {
-                    if (d == 0.0f) {
-                        val = row[cx];
-                    } else {
-                        float2 acc = make_float2(0.f, 0.f);
-                        float den = 0.f;
-                        const float2* tap = row + cx - 8;
-#pragma unroll
-                        for (int i = 0; i < 16; ++i) {
-                            // d + (8 - i) is exact to one rounding; the 1-ulp reciprocal error cancels in the
-                            // normalisation to first order
-                            const float wi = w[i] * __builtin_amdgcn_rcpf(d + (float)(8 - i));
-                            const float2 t = tap[i];
-                            acc.x += wi * t.x;
-                            acc.y += wi * t.y;
-                            den += wi;
-                        }
-                        const float inv = __builtin_amdgcn_rcpf(den);
-                        val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
-                    }
-                }
+                if (sinc16) val = (d == 0.0f) ? row[cx] : sinc16_interp(row + cx - 8, w, d);
             }
         } else {
             val = row[x];
@@ -186,7 +167,7 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8)
                 const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
                 const float fr = (float)(turns - rint(turns));
                 float sn, cs;
-                sincospif(2.0f * fr, &sn, &cs);
+                sincos_turns(fr, &sn, &cs);
                 // H = exp(-i phase)
                 val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
             }
@@ -219,7 +200,7 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
     const double half = 0.5 * a.prf;
     const double band_hi = half * a.beta + cc.dc_off;
     const double band_lo = -half * a.beta + cc.dc_off;
-    float w[16];
+    float w[16];  // signed window (-1)^i w[i]
 #pragma unroll
     for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
     float2 pre0, pre1;
@@ -233,6 +214,7 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
         seg[buf][t] = pre0;
         if (t < kRtSeg - kRtCols) seg[buf][t + kRtCols] = pre1;
     };
+    // one row ahead in registers and one in LDS (a deeper register ring measured slower: fewer waves per SIMD)
     fetch(q0);
     deposit(0);
     if (q0 + 1 < q1) fetch(q0 + 1);
@@ -249,7 +231,7 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
         float2 val;
         if (a.mode != 2) {
             const double shift = cc.shift_k * fn * fn;
-            const double p = round(shift);
+            const double p = floor(shift + 0.5);  // round half away from zero: shift >= 0
             const int pi = (int)fmin(p, 4096.0);
             const int cx = x + pi;
             if (cx - 8 < 0 || cx + 8 >= nrv) {
@@ -258,22 +240,7 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
                 const float d = (float)(shift - p);
                 // the gate guarantees pi <= kRcmcTileReach; the clamp only keeps the taps inside the segment
                 const float2* tap = &seg[buf][min(t + pi, kRtSeg - 16)];
-                if (d == 0.0f) {
-                    val = tap[8];
-                } else {
-                    float2 acc = make_float2(0.f, 0.f);
-                    float den = 0.f;
-#pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const float wi = w[i] * __builtin_amdgcn_rcpf(d + (float)(8 - i));
-                        const float2 tp = tap[i];
-                        acc.x += wi * tp.x;
-                        acc.y += wi * tp.y;
-                        den += wi;
-                    }
-                    const float inv = __builtin_amdgcn_rcpf(den);
-                    val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
-                }
+                val = (d == 0.0f) ? tap[8] : sinc16_interp(tap, w, d);
             }
         } else {
             val = seg[buf][t + 8];
@@ -290,7 +257,7 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
                 const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
                 const float fr = (float)(turns - rint(turns));
                 float sn, cs;
-                sincospif(2.0f * fr, &sn, &cs);
+                sincos_turns(fr, &sn, &cs);
                 val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
             }
         }
