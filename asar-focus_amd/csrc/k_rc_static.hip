@@ -22,9 +22,20 @@ struct RawPairS<int8_t> {
     using type = char2;
 };
 
+// LDS position of transform element i. PADSHIFT > 0: one padding element every 2^PADSHIFT. PADSHIFT == 0: no
+// padding, element i sits at i ^ ((i >> 3) & 31), a permutation of every aligned 256-element block that spreads
+// the strided Stockham stores (stride R at the first pass, runs of ns at the next) over the banks. Offsets that
+// are multiples of 256 (n / R for the production radices) pass through unchanged, so loads at j + r n / R keep
+// constant offsets.
+template <int PADSHIFT>
+__host__ __device__ constexpr int lds_pos(int i) {
+    return PADSHIFT > 0 ? i + (i >> PADSHIFT) : i ^ ((i >> 3) & 31);
+}
+
 template <int N, int PADSHIFT>
 __host__ __device__ constexpr int static_lds_elems() {
-    return N + (N >> PADSHIFT) + 1;
+    static_assert(PADSHIFT > 0 || N % 256 == 0, "the swizzled layout permutes whole 256-element blocks");
+    return PADSHIFT > 0 ? N + (N >> PADSHIFT) + 1 : N;
 }
 
 template <int FMT, int N, int T, int PADSHIFT, int WAVES, class Seq, bool STAGE = false>
@@ -72,8 +83,8 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
             return correct((float)r.x, (float)r.y);
         }
     };
-    auto lds_store = [&](int i, float2 x) { lds[i + (i >> PADSHIFT)] = x; };
-    auto lds_load = [&](int i) { return lds[i + (i >> PADSHIFT)]; };
+    auto lds_store = [&](int i, float2 x) { lds[lds_pos<PADSHIFT>(i)] = x; };
+    auto lds_load = [&](int i) { return lds[lds_pos<PADSHIFT>(i)]; };
 
     // forward transform
     constexpr int R0 = Seq::at(0);
@@ -171,7 +182,7 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
 // records the fastest (RcArgs::static_variant). Variant 0 is the default.
 int range_compress_static_variants(int n) {
     switch (n) {
-        case 6144: return 6;
+        case 6144: return 7;
         case 6400: return 6;
         case 8192: return 6;
         default: return 0;
@@ -191,7 +202,8 @@ bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipEr
                 case 3: *err = launch_static<6144, 512, 5, 4, RadixSeq<12, 8, 8, 8>>(a, fmt, s); return true;
                 case 4: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
                 case 5: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
-                default: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
+                case 6: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
+                default: *err = launch_static<6144, 512, 0, 6, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
             }
         case 6400:
             switch (v) {
