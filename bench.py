@@ -269,6 +269,21 @@ def main():
         with open(tfile) as f:
             traffic = json.load(f).get(cfg.name, {}).get(dom)
 
+    # measured device-to-device copy rate over a buffer the size of the plan's image: the second denominator of
+    # the memory-bound stages (what a plain read + write of the same bytes achieves on this device)
+    nbytes = plan.params.nap * (cfg.nr - cfg.nc) * 8
+    csrc = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    cdst = torch.empty_like(csrc)
+    cdst.copy_(csrc)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(5):
+        cdst.copy_(csrc)
+    c1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * nbytes / (c0.elapsed_time(c1) / 5 * 1e-3) / 1e9
+    del csrc, cdst
+
     value = cfg.na * cfg.nr / (ms * 1e-3) / 1e6
     line = {
         "metric": METRIC,
@@ -305,9 +320,12 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes": dom_bytes,
             "ms": round(stages[dom], 4),
+            "copy_gbs_measured": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "stages_gbs": {k: round(v, 1) for k, v in gbs.items()},
+        "stages_frac_of_copy": {k: round(v / copy_gbs, 3) for k, v in gbs.items()},
         "slc_output_msamples_per_s": round(cfg.na * (cfg.nr - cfg.nc) / (ms * 1e-3) / 1e6, 2),
         "plan_device_gb": round(plan.device_bytes / 1e9, 3),
         "scene_generation_s": round(gen_s, 2),
