@@ -111,5 +111,6 @@ struct asarf_rcmc_net {
     unsigned char* lut_ok = nullptr;
     float* judge = nullptr;   // err_sum | count | sinc8_err_sum, each ns * judge_nd
     int nd = 0, ns = 0, judge_nd = 0;
+    int n_ok = 0;             // judge cells accepted by the last table build
     float window8[8] = {};
 };

@@ -75,6 +75,42 @@ __device__ __forceinline__ double alias_about_dc(double f, double dc, double prf
     return f;
 }
 
+// Plain 8-tap Blackman sinc at fractional offset d (d != 0), taps tap[i] at offsets 4 - i (same normalised form
+// as the 16-tap interpolator).
+__device__ __forceinline__ float2 sinc8_interp(const float2* tap, const float* window8, float d) {
+    float2 acc = make_float2(0.f, 0.f);
+    float den = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float wi = ((i & 1) ? -window8[i] : window8[i]) * __builtin_amdgcn_rcpf(d + (float)(4 - i));
+        const float2 t = tap[i];
+        acc.x += wi * t.x;
+        acc.y += wi * t.y;
+        den += wi;
+    }
+    const float inv = __builtin_amdgcn_rcpf(den);
+    return (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
+}
+
+// Learned 8-tap interpolation (complex taps from the network's table, taps tap[i] at offsets 4 - i) where the
+// judge accepted the (d, slope) cell; false (nothing written) where it did not.
+__device__ __forceinline__ bool learned_interp(const RcmcArgs& a, const float2* tap, float d, double shift_k,
+                                               double fn, float2* out) {
+    const int cell = net_cell(d, net_slope(shift_k, fn, a.prf), a.lut_nd, a.lut_ns);
+    const int jc = (cell / a.lut_nd) * a.judge_nd + (cell % a.lut_nd) * a.judge_nd / a.lut_nd;
+    if (!a.lut_ok[jc]) return false;
+    const float2* w8 = a.lut + (long)cell * kNetTaps;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kNetTaps; ++i) {
+        const float2 wi = w8[i], t = tap[i];
+        acc.x += wi.x * t.x - wi.y * t.y;
+        acc.y += wi.x * t.y + wi.y * t.x;
+    }
+    *out = acc;
+    return true;
+}
+
 // One workgroup per frequency row (in place). The row is staged in LDS so every output can read its 16 taps.
 // 512 lanes per row: the LDS row (8 B x nrv) limits the workgroups per CU, so wider workgroups keep more waves
 // in flight.
@@ -111,41 +147,9 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8)
                 bool sinc16 = a.interp == 0;
                 if (a.interp == 1) {
                     // learned taps where the judge accepted the cell, else the 16-tap sinc
-                    const int cell = net_cell(d, net_slope(cc.shift_k, fn, a.prf), a.lut_nd, a.lut_ns);
-                    const int jc = (cell / a.lut_nd) * a.judge_nd + (cell % a.lut_nd) * a.judge_nd / a.lut_nd;
-                    if (a.lut_ok[jc]) {
-                        const float2* w8 = a.lut + (long)cell * kNetTaps;
-                        const float2* tap = row + cx - 4;
-                        float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-                        for (int i = 0; i < kNetTaps; ++i) {
-                            const float2 wi = w8[i], t = tap[i];
-                            acc.x += wi.x * t.x - wi.y * t.y;
-                            acc.y += wi.x * t.y + wi.y * t.x;
-                        }
-                        val = acc;
-                    } else {
-                        sinc16 = true;
-                    }
+                    sinc16 = !learned_interp(a, row + cx - 4, d, cc.shift_k, fn, &val);
                 } else if (a.interp == 2) {
-                    if (d == 0.0f) {
-                        val = row[cx];
-                    } else {
-                        float2 acc = make_float2(0.f, 0.f);
-                        float den = 0.f;
-                        const float2* tap = row + cx - 4;
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            const float wi =
-                                ((i & 1) ? -a.window8[i] : a.window8[i]) * __builtin_amdgcn_rcpf(d + (float)(4 - i));
-                            const float2 t = tap[i];
-                            acc.x += wi * t.x;
-                            acc.y += wi * t.y;
-                            den += wi;
-                        }
-                        const float inv = __builtin_amdgcn_rcpf(den);
-                        val = (den == 0.f) ? make_float2(0.f, 0.f) : make_float2(acc.x * inv, acc.y * inv);
-                    }
+                    val = (d == 0.0f) ? row[cx] : sinc8_interp(row + cx - 4, a.window8, d);
                 }
                 if (sinc16) val = (d == 0.0f) ? row[cx] : sinc16_interp(row + cx - 8, w, d);
             }
@@ -176,7 +180,8 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8)
     }
 }
 
-// Column-tiled RCMC + matched filter (16-tap sinc, out of place): a workgroup owns 256 columns for kRtRows
+// Column-tiled RCMC + matched filter (16-tap sinc, or the learned / 8-tap interpolator as template INTERP = the
+// RcmcArgs::interp code; out of place): a workgroup owns 256 columns for kRtRows
 // consecutive storage rows. Each lane loads its column's FP64 constants once and reuses them for every row; the
 // rows' range segments (with an 8-column left halo and an 8 + kRcmcTileReach right halo) are double-buffered in
 // LDS. Identical arithmetic to k_rcmc_mf, so both give the same result; the plan gates between them on the
@@ -185,6 +190,7 @@ constexpr int kRtCols = 256;
 constexpr int kRtRows = 32;
 constexpr int kRtSeg = kRtCols + 16 + kRcmcTileReach;
 
+template <int INTERP>
 __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
     __shared__ float2 seg[2][kRtSeg];
     if (a.tile_gate && *a.tile_gate > kRcmcTileReach) return;
@@ -240,7 +246,16 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
                 const float d = (float)(shift - p);
                 // the gate guarantees pi <= kRcmcTileReach; the clamp only keeps the taps inside the segment
                 const float2* tap = &seg[buf][min(t + pi, kRtSeg - 16)];
-                val = (d == 0.0f) ? tap[8] : sinc16_interp(tap, w, d);
+                // the same interpolator choices as k_rcmc_mf: learned taps where the judge accepted the cell
+                // (at any d), else the sinc (exact copy at d == 0)
+                if (INTERP == 1 && learned_interp(a, tap + 4, d, cc.shift_k, fn, &val)) {
+                } else if (d == 0.0f) {
+                    val = tap[8];
+                } else if (INTERP == 2) {
+                    val = sinc8_interp(tap + 4, a.window8, d);
+                } else {
+                    val = sinc16_interp(tap, w, d);
+                }
             }
         } else {
             val = seg[buf][t + 8];
@@ -268,9 +283,14 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
 
 hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s) {
     ensure_window(s);
-    if (!a.out || a.out == a.img || a.interp != 0) return hipErrorInvalidValue;  // out of place, 16-tap sinc only
-    hipLaunchKernelGGL(k_rcmc_tile, dim3((a.nrv + kRtCols - 1) / kRtCols, (a.nbins + kRtRows - 1) / kRtRows),
-                       dim3(kRtCols), 0, s, a);
+    if (!a.out || a.out == a.img) return hipErrorInvalidValue;  // out of place only
+    const dim3 grid((a.nrv + kRtCols - 1) / kRtCols, (a.nbins + kRtRows - 1) / kRtRows);
+    switch (a.interp) {
+        case 0: hipLaunchKernelGGL(k_rcmc_tile<0>, grid, dim3(kRtCols), 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_rcmc_tile<1>, grid, dim3(kRtCols), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_rcmc_tile<2>, grid, dim3(kRtCols), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

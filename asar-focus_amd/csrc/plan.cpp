@@ -60,6 +60,7 @@ struct asarf_plan {
     int imp_cap = 0;
     // RCMC interpolator (0 = 16-tap sinc, 1 = learned table with sinc fallback, 2 = 8-tap sinc)
     int rcmc_interp = 0;
+    bool rcmc_row_only = false;
     const asarf_rcmc_net* net = nullptr;
     int fft_mfma = 0;  // split-bfloat16 MFMA transforms
     // timing
@@ -144,6 +145,9 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     if (!pl->rplan) return fail(st);
     st = get_col_split(c, nap, &pl->cs);
     if (st != ASARF_OK) return fail(st);
+    // ASARF_RCMC_ROW=1: always the row RCMC kernel (the column-tiled kernel is not launched); for comparisons
+    const char* row_env = std::getenv("ASARF_RCMC_ROW");
+    pl->rcmc_row_only = row_env && row_env[0] == '1';
     // Fused azimuth core (RCMC + matched filter + inverse pass B' in one tile kernel), selected with
     // ASARF_AZ_FUSED=1 when the split has a second pass and the tile fits LDS. It is exact against the separate
     // kernels but measured slower on the 30000 x 5616 scene (2.41 ms against 1.95 ms for RCMC + pass B'), so the
@@ -383,6 +387,9 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
     rc.beta = p.az_bandwidth_fraction;
     rc.mode = 0;
     rc.interp = pl->rcmc_interp;
+    // a table whose judge accepted no cell falls back to the 16-tap sinc everywhere: run that directly and skip
+    // the per-pixel cell lookups
+    if (rc.interp == ASARF_RCMC_LEARNED && pl->net && pl->net->n_ok == 0) rc.interp = ASARF_RCMC_SINC16;
     if (pl->net) {
         rc.lut = pl->net->lut;
         rc.lut_ok = pl->net->lut_ok;
@@ -428,14 +435,17 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
                                  p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
         HIP_CHECK(launch_col_fft(ia, true, s));
         ST_CHECK(mark(pl, "azimuth_ifft_a"));
-    } else if (pl->rcmc_interp == 0) {
-        // 4. RCMC + azimuth matched filter (img -> img2): the column-tiled kernel when the device-side shift bound
-        // fits its halo, else the row kernel; both are enqueued and the bound makes exactly one of them work
+    } else {
+        // 4. RCMC (any interpolator) + azimuth matched filter (img -> img2): the column-tiled kernel when the
+        // device-side shift bound fits its halo, else the row kernel; both are enqueued and the bound makes
+        // exactly one of them work
         const ColSplit& cs = pl->cs;
         const int n = cs.n;
         rc.out = pl->img2;
-        rc.tile_gate = &pl->res->rcmc_reach;
-        HIP_CHECK(launch_rcmc_tile(rc, s));
+        if (!pl->rcmc_row_only) {
+            rc.tile_gate = &pl->res->rcmc_reach;
+            HIP_CHECK(launch_rcmc_tile(rc, s));
+        }
         HIP_CHECK(launch_rcmc_mf(rc, s));
         ST_CHECK(mark(pl, "rcmc_matched_filter"));
         // 5. azimuth IFFT (img2 -> img), 1/Nap, rows < Na
@@ -452,19 +462,6 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
             HIP_CHECK(launch_col_fft(ib, true, s));
             ST_CHECK(mark(pl, "azimuth_ifft_b"));
             ST_CHECK(col_inverse_pass(c, cs, 1, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap,
-                                      pl->fft_mfma));
-            ST_CHECK(mark(pl, "azimuth_ifft_a"));
-        }
-    } else {
-        // 4. RCMC (learned or 8-tap interpolator) + azimuth matched filter, in place
-        HIP_CHECK(launch_rcmc_mf(rc, s));
-        ST_CHECK(mark(pl, "rcmc_matched_filter"));
-        // 5. azimuth IFFT, 1/Nap, rows < Na
-        ST_CHECK(col_inverse_pass(c, pl->cs, 0, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap,
-                                  pl->fft_mfma));
-        ST_CHECK(mark(pl, "azimuth_ifft_b"));
-        if (pl->cs.n2 > 1) {
-            ST_CHECK(col_inverse_pass(c, pl->cs, 1, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap,
                                       pl->fft_mfma));
             ST_CHECK(mark(pl, "azimuth_ifft_a"));
         }

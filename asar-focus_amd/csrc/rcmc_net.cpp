@@ -230,6 +230,7 @@ extern "C" int asarf_rcmc_net_build_table(asarf_rcmc_net* n, const float* val_pa
     }
     HIP_CHECK(hipMemcpyAsync(n->lut_ok, ok.data(), (size_t)nj, hipMemcpyHostToDevice, s));
     HIP_CHECK(hipStreamSynchronize(s));
+    n->n_ok = n_ok;
     if (n_judge_ok) *n_judge_ok = n_ok;
     return clear_error();
 }

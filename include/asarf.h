@@ -312,7 +312,8 @@ int asarf_rcmc_net_build_table(asarf_rcmc_net* net, const float* val_pairs, long
  * pairs (device, n_pairs x ASARF_RCMC_PAIR_FLOATS floats) drawn at seeded random (bin, column) positions. */
 int asarf_plan_rcmc_pairs(asarf_plan* plan, const void* raw, long n_pairs, unsigned long long seed, float* pairs);
 /* Selects the plan's RCMC interpolator (ASARF_RCMC_*); the learned one needs a net with a built table, which
- * must outlive the plan's use of it. */
+ * must outlive the plan's use of it. When the table's last build accepted no judge cell the learned path is the
+ * 16-tap sinc everywhere, and the plan runs that kernel without the per-pixel table lookups. */
 int asarf_plan_set_rcmc(asarf_plan* plan, int interpolator, const asarf_rcmc_net* net);
 
 #ifdef __cplusplus

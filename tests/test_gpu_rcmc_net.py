@@ -130,3 +130,31 @@ def test_learned_rcmc_slc_against_sinc16_and_oracle(trained):
     # the table is not that accurate.
     assert worst["learned"] < 0.6 * worst["sinc8"]
     assert e_judged < 1e-3
+
+
+def test_tiled_and_row_rcmc_kernels_agree_for_every_interpolator(trained):
+    """The column-tiled RCMC kernel (default when the shift bound fits its halo) and the row kernel
+    (ASARF_RCMC_ROW=1) compute the same interpolators: 16-tap sinc, plain 8-tap sinc and the learned table."""
+    import os
+    cfg, raw, traw, plan, _, val, net, _, _ = trained
+    os.environ["ASARF_RCMC_ROW"] = "1"
+    try:
+        row_plan = focus.Plan(plan.ctx, cfg)
+    finally:
+        del os.environ["ASARF_RCMC_ROW"]
+    net.build_table(val, nd=8192, ns=2, judge_nd=256, threshold=1.0)  # learned taps in every cell with data
+    try:
+        for mode in (focus.RCMC_SINC16, focus.RCMC_SINC8, focus.RCMC_LEARNED):
+            out = []
+            for p in (plan, row_plan):
+                p.set_rcmc(mode, net if mode == focus.RCMC_LEARNED else None)
+                p.execute(traw)
+                out.append(p.slc_tensor().cpu().numpy())
+            bound = plan.results()["rcmc_shift_bound"]
+            err = rel_rms(out[0], out[1])
+            print(f"interpolator {mode}: shift bound {bound}, tiled vs row kernel rel rms {err:.2e}")
+            assert bound <= 24
+            assert err < 1e-6
+    finally:
+        plan.set_rcmc(focus.RCMC_SINC16)
+        row_plan.close()
