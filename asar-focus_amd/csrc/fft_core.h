@@ -319,7 +319,10 @@ __device__ __forceinline__ int fast_div(int j, int ns, float inv_ns) {
 
 // Twiddle + DFT of every butterfly of the thread (pass with prefix product ns); clamped lanes compute a copy.
 // TW(e) returns W_n^e for 0 <= e < n.
-template <int R, int MAXE, bool INV, bool VOTE0 = true, class TwAt>
+// TW1: one table lookup per butterfly (w = W^base) and the other powers by products (w^(2a) = (w^a)^2,
+// w^r = w^hb w^(r - hb)), instead of one lookup per power of two: fewer LDS reads for the same number of complex
+// products; w^r then carries up to about log2(R) + 2 roundings more than the table (a few 1e-7 for R <= 16).
+template <int R, int MAXE, bool INV, bool VOTE0 = true, bool TW1 = false, class TwAt>
 __device__ __forceinline__ void pass_compute(float2 (&v)[MAXE], int s, int S, int n, int ns, TwAt&& tw_at) {
     constexpr int NB = MAXE / R;
     const int nr = n / R;
@@ -339,7 +342,7 @@ __device__ __forceinline__ void pass_compute(float2 (&v)[MAXE], int s, int S, in
 #pragma unroll
             for (int r = 2; r < R; ++r) {
                 if ((r & (r - 1)) == 0) {
-                    pw[r] = tw_at(base * r);
+                    pw[r] = TW1 ? cmul(pw[r / 2], pw[r / 2]) : tw_at(base * r);
                 } else {
                     int hb = 1;
                     while (hb * 2 <= r) hb *= 2;

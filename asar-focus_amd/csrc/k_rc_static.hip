@@ -38,7 +38,7 @@ __host__ __device__ constexpr int static_lds_elems() {
     return PADSHIFT > 0 ? N + (N >> PADSHIFT) + 1 : N;
 }
 
-template <int FMT, int N, int T, int PADSHIFT, int WAVES, class Seq, bool STAGE = false>
+template <int FMT, int N, int T, int PADSHIFT, int WAVES, class Seq, bool STAGE = false, bool TW1 = false>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_rc_static(RcArgs a) {
     static_assert(Seq::product() == N, "radix sequence must factor N");
     // elements per lane: N / T when every radix divides it, else the largest R * ceil((N / R) / T) (lanes
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
     } else {
         pass_load<R0, E, V0>(v, lane, T, N, gl_load);
     }
-    pass_compute<R0, E, false, V0>(v, lane, T, N, 1, tw_at);
+    pass_compute<R0, E, false, V0, TW1>(v, lane, T, N, 1, tw_at);
     static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
         constexpr int p = decltype(pc)::value;
         constexpr int Rp = Seq::at(p - 1), R = Seq::at(p);
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
         __syncthreads();
         pass_load<R, E, V0>(v, lane, T, N, lds_load);
         __syncthreads();
-        pass_compute<R, E, false, V0>(v, lane, T, N, Seq::prefix(p), tw_at);
+        pass_compute<R, E, false, V0, TW1>(v, lane, T, N, Seq::prefix(p), tw_at);
     });
     // matched filter in the range frequency domain
     constexpr int RL = Seq::at(Seq::count - 1);
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
     pass_visit<RL>(v, lane, T, N, spec_mul);
     __syncthreads();  // all LDS reads of the forward transform precede the inverse stores
     // inverse transform (radices in reverse order; its first pass consumes the registers as they are)
-    pass_compute<RL, E, true, V0>(v, lane, T, N, 1, tw_at);
+    pass_compute<RL, E, true, V0, TW1>(v, lane, T, N, 1, tw_at);
     static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
         constexpr int p = decltype(pc)::value;
         constexpr int Rp = Seq::rat(p - 1), R = Seq::rat(p);
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
         __syncthreads();
         pass_load<R, E, V0>(v, lane, T, N, lds_load);
         __syncthreads();
-        pass_compute<R, E, true, V0>(v, lane, T, N, Seq::rprefix(p), tw_at);
+        pass_compute<R, E, true, V0, TW1>(v, lane, T, N, Seq::rprefix(p), tw_at);
     });
     float2* out = a.out + (long)row * a.out_stride;
     const int nrv = a.nrv, zt = a.zero_to;
@@ -165,16 +165,19 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
     pass_visit<RIL>(v, lane, T, N, out_write);
 }
 
-template <int N, int T, int PADSHIFT, int WAVES, class Seq, bool STAGE = false>
+template <int N, int T, int PADSHIFT, int WAVES, class Seq, bool STAGE = false, bool TW1 = false>
 static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
     const size_t lds = sizeof(float2) * (size_t)(static_lds_elems<N, PADSHIFT>() + tw_lds_elems(N));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     if (fmt == RAW_CF32)
-        hipLaunchKernelGGL((k_rc_static<RAW_CF32, N, T, PADSHIFT, WAVES, Seq, STAGE>), dim3(a.rows), dim3(T), lds, s, a);
+        hipLaunchKernelGGL((k_rc_static<RAW_CF32, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T), lds,
+                           s, a);
     else if (fmt == RAW_U8)
-        hipLaunchKernelGGL((k_rc_static<RAW_U8, N, T, PADSHIFT, WAVES, Seq, STAGE>), dim3(a.rows), dim3(T), lds, s, a);
+        hipLaunchKernelGGL((k_rc_static<RAW_U8, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T), lds,
+                           s, a);
     else
-        hipLaunchKernelGGL((k_rc_static<RAW_I8, N, T, PADSHIFT, WAVES, Seq, STAGE>), dim3(a.rows), dim3(T), lds, s, a);
+        hipLaunchKernelGGL((k_rc_static<RAW_I8, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T), lds,
+                           s, a);
     return hipGetLastError();
 }
 
@@ -182,7 +185,7 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
 // records the fastest (RcArgs::static_variant). Variant 0 is the default.
 int range_compress_static_variants(int n) {
     switch (n) {
-        case 6144: return 7;
+        case 6144: return 8;
         case 6400: return 6;
         case 8192: return 6;
         default: return 0;
@@ -203,7 +206,8 @@ bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipEr
                 case 4: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>>(a, fmt, s); return true;
                 case 5: *err = launch_static<6144, 512, 5, 4, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
                 case 6: *err = launch_static<6144, 512, 5, 6, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
-                default: *err = launch_static<6144, 512, 0, 6, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
+                case 7: *err = launch_static<6144, 512, 0, 6, RadixSeq<8, 8, 8, 12>, true>(a, fmt, s); return true;
+                default: *err = launch_static<6144, 512, 0, 6, RadixSeq<8, 8, 8, 12>, true, true>(a, fmt, s); return true;
             }
         case 6400:
             switch (v) {

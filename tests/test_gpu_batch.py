@@ -66,7 +66,7 @@ def test_batch_graph_equals_sequential_and_replays(ctx):
     b.close()
 
 
-@pytest.mark.parametrize("which,nvar", [("asar", 7), ("ers", 6), ("ers8192", 6)])
+@pytest.mark.parametrize("which,nvar", [("asar", 8), ("ers", 6), ("ers8192", 6)])
 def test_range_compression_variants_match_oracle(ctx, which, nvar):
     """Every compile-time range-compression variant the autotuner may pick gives the oracle's SLC."""
     import os
