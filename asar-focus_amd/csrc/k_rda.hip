@@ -111,72 +111,76 @@ __device__ __forceinline__ bool learned_interp(const RcmcArgs& a, const float2* 
     return true;
 }
 
-// One workgroup per frequency row (in place). The row is staged in LDS so every output can read its 16 taps.
-// 512 lanes per row: the LDS row (8 B x nrv) limits the workgroups per CU, so wider workgroups keep more waves
-// in flight.
+// One workgroup per frequency row at a time (in place, or out of place into a.out), rows strided over a grid of
+// at most 2048 workgroups. The row is staged in LDS so every output can read its 16 taps. 512 lanes per row: the
+// LDS row (8 B x nrv) limits the workgroups per CU, so wider workgroups keep more waves in flight. This is the
+// kernel for RCMC shifts beyond the column-tiled kernel's halo.
 constexpr int kRcmcThreads = 512;
-__global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_rcmc_mf(RcmcArgs a) {
+__global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_rcmc_mf(RcmcArgs a) {
     extern __shared__ float2 row[];
     if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
     if (a.tile_gate && *a.tile_gate <= kRcmcTileReach) return;
-    const int q = blockIdx.x;
-    const float2* img = a.img + (long)q * a.stride;
-    float2* dst = (a.out ? a.out : a.img) + (long)q * a.stride;
     const int nrv = a.nrv;
-    for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) row[x] = img[x];
-    __syncthreads();
-    const int k = (a.n2 == 1) ? q : (q / a.n2 + a.n1 * (q % a.n2));
     const double step = a.prf / (double)a.nbins;
-    const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
     const double half = 0.5 * a.prf;
     float w[16];  // signed window (-1)^i w[i]
 #pragma unroll
     for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
-    for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) {
-        const ColConst cc = a.col[x];
-        const double fn = alias_about_dc(fb, cc.dc, a.prf);
-        float2 val;
-        if (a.mode != 2) {
-            const double shift = cc.shift_k * fn * fn;
-            const double p = floor(shift + 0.5);  // round half away from zero: shift >= 0
-            const int cx = x + (int)p;
-            if (cx - 8 < 0 || cx + 8 >= nrv) {
-                val = make_float2(0.f, 0.f);
-            } else {
-                const float d = (float)(shift - p);
-                bool sinc16 = a.interp == 0;
-                if (a.interp == 1) {
-                    // learned taps where the judge accepted the cell, else the 16-tap sinc
-                    sinc16 = !learned_interp(a, row + cx - 4, d, cc.shift_k, fn, &val);
-                } else if (a.interp == 2) {
-                    val = (d == 0.0f) ? row[cx] : sinc8_interp(row + cx - 4, a.window8, d);
+    // rows strided over a bounded grid, so that the launch costs little when the gate sends the work elsewhere
+    for (int q = blockIdx.x; q < a.nbins; q += gridDim.x) {
+        const float2* img = a.img + (long)q * a.stride;
+        float2* dst = (a.out ? a.out : a.img) + (long)q * a.stride;
+        for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) row[x] = img[x];
+        __syncthreads();
+        const int k = (a.n2 == 1) ? q : (q / a.n2 + a.n1 * (q % a.n2));
+        const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
+        for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) {
+            const ColConst cc = a.col[x];
+            const double fn = alias_about_dc(fb, cc.dc, a.prf);
+            float2 val;
+            if (a.mode != 2) {
+                const double shift = cc.shift_k * fn * fn;
+                const double p = floor(shift + 0.5);  // round half away from zero: shift >= 0
+                const int cx = x + (int)p;
+                if (cx - 8 < 0 || cx + 8 >= nrv) {
+                    val = make_float2(0.f, 0.f);
+                } else {
+                    const float d = (float)(shift - p);
+                    bool sinc16 = a.interp == 0;
+                    if (a.interp == 1) {
+                        // learned taps where the judge accepted the cell, else the 16-tap sinc
+                        sinc16 = !learned_interp(a, row + cx - 4, d, cc.shift_k, fn, &val);
+                    } else if (a.interp == 2) {
+                        val = (d == 0.0f) ? row[cx] : sinc8_interp(row + cx - 4, a.window8, d);
+                    }
+                    if (sinc16) val = (d == 0.0f) ? row[cx] : sinc16_interp(row + cx - 8, w, d);
                 }
-                if (sinc16) val = (d == 0.0f) ? row[cx] : sinc16_interp(row + cx - 8, w, d);
-            }
-        } else {
-            val = row[x];
-        }
-        if (a.mode != 1) {
-            const double hi = half * a.beta + cc.dc_off;
-            const double lo = -half * a.beta + cc.dc_off;
-            if (fn < lo || fn > hi) {
-                val = make_float2(0.f, 0.f);
             } else {
-                double fc = fb - cc.dc_off;
-                if (fc > half)
-                    fc -= a.prf;
-                else if (fc < -half)
-                    fc += a.prf;
-                // phase / (2 pi) = inv_ka * (fc^2 / 2 + dc_off * f_bin)
-                const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
-                const float fr = (float)(turns - rint(turns));
-                float sn, cs;
-                sincos_turns(fr, &sn, &cs);
-                // H = exp(-i phase)
-                val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
+                val = row[x];
             }
+            if (a.mode != 1) {
+                const double hi = half * a.beta + cc.dc_off;
+                const double lo = -half * a.beta + cc.dc_off;
+                if (fn < lo || fn > hi) {
+                    val = make_float2(0.f, 0.f);
+                } else {
+                    double fc = fb - cc.dc_off;
+                    if (fc > half)
+                        fc -= a.prf;
+                    else if (fc < -half)
+                        fc += a.prf;
+                    // phase / (2 pi) = inv_ka * (fc^2 / 2 + dc_off * f_bin)
+                    const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
+                    const float fr = (float)(turns - rint(turns));
+                    float sn, cs;
+                    sincos_turns(fr, &sn, &cs);
+                    // H = exp(-i phase)
+                    val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
+                }
+            }
+            dst[x] = val;
         }
-        dst[x] = val;
+        __syncthreads();  // every lane is done with this row in LDS before the next one is loaded
     }
 }
 
@@ -298,7 +302,7 @@ hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s) {
     ensure_window(s);
     const size_t lds = sizeof(float2) * (size_t)a.nrv;
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rcmc_mf, dim3(a.nbins), dim3(kRcmcThreads), lds, s, a);
+    hipLaunchKernelGGL(k_rcmc_mf, dim3(std::min(a.nbins, 2048)), dim3(kRcmcThreads), lds, s, a);
     return hipGetLastError();
 }
 
