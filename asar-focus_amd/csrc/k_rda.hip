@@ -137,8 +137,12 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(7)
         for (int x = threadIdx.x; x < nrv; x += kRcmcThreads) {
             const ColConst cc = a.col[x];
             const double fn = alias_about_dc(fb, cc.dc, a.prf);
-            float2 val;
-            if (a.mode != 2) {
+            // outside the azimuth band the matched filter zeroes the pixel: skip its interpolation
+            const bool out_band =
+                a.mode != 1 && (fn < -half * a.beta + cc.dc_off || fn > half * a.beta + cc.dc_off);
+            float2 val = make_float2(0.f, 0.f);
+            if (out_band) {
+            } else if (a.mode != 2) {
                 const double shift = cc.shift_k * fn * fn;
                 const double p = floor(shift + 0.5);  // round half away from zero: shift >= 0
                 const int cx = x + (int)p;
@@ -158,25 +162,19 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(7)
             } else {
                 val = row[x];
             }
-            if (a.mode != 1) {
-                const double hi = half * a.beta + cc.dc_off;
-                const double lo = -half * a.beta + cc.dc_off;
-                if (fn < lo || fn > hi) {
-                    val = make_float2(0.f, 0.f);
-                } else {
-                    double fc = fb - cc.dc_off;
-                    if (fc > half)
-                        fc -= a.prf;
-                    else if (fc < -half)
-                        fc += a.prf;
-                    // phase / (2 pi) = inv_ka * (fc^2 / 2 + dc_off * f_bin)
-                    const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
-                    const float fr = (float)(turns - rint(turns));
-                    float sn, cs;
-                    sincos_turns(fr, &sn, &cs);
-                    // H = exp(-i phase)
-                    val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
-                }
+            if (a.mode != 1 && !out_band) {
+                double fc = fb - cc.dc_off;
+                if (fc > half)
+                    fc -= a.prf;
+                else if (fc < -half)
+                    fc += a.prf;
+                // phase / (2 pi) = inv_ka * (fc^2 / 2 + dc_off * f_bin)
+                const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
+                const float fr = (float)(turns - rint(turns));
+                float sn, cs;
+                sincos_turns(fr, &sn, &cs);
+                // H = exp(-i phase)
+                val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
             }
             dst[x] = val;
         }
@@ -238,8 +236,11 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
         const int k = (a.n2 == 1) ? q : (q / a.n2 + a.n1 * (q % a.n2));
         const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
         const double fn = alias_about_dc(fb, cc.dc, a.prf);
-        float2 val;
-        if (a.mode != 2) {
+        // outside the azimuth band the matched filter zeroes the pixel: skip its interpolation
+        const bool out_band = a.mode != 1 && (fn < band_lo || fn > band_hi);
+        float2 val = make_float2(0.f, 0.f);
+        if (out_band) {
+        } else if (a.mode != 2) {
             const double shift = cc.shift_k * fn * fn;
             const double p = floor(shift + 0.5);  // round half away from zero: shift >= 0
             const int pi = (int)fmin(p, 4096.0);
@@ -264,21 +265,17 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
         } else {
             val = seg[buf][t + 8];
         }
-        if (a.mode != 1) {
-            if (fn < band_lo || fn > band_hi) {
-                val = make_float2(0.f, 0.f);
-            } else {
-                double fc = fb - cc.dc_off;
-                if (fc > half)
-                    fc -= a.prf;
-                else if (fc < -half)
-                    fc += a.prf;
-                const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
-                const float fr = (float)(turns - rint(turns));
-                float sn, cs;
-                sincos_turns(fr, &sn, &cs);
-                val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
-            }
+        if (a.mode != 1 && !out_band) {
+            double fc = fb - cc.dc_off;
+            if (fc > half)
+                fc -= a.prf;
+            else if (fc < -half)
+                fc += a.prf;
+            const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
+            const float fr = (float)(turns - rint(turns));
+            float sn, cs;
+            sincos_turns(fr, &sn, &cs);
+            val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
         }
         if (xok) a.out[(long)q * a.stride + x] = val;
         __syncthreads();
