@@ -27,6 +27,10 @@ __global__ __launch_bounds__(16 * COLS) void k_col_static(ColFftArgs a) {
     const int col = tile * COLS + b;
     const bool col_ok = col < a.ncols;
     const TwLds tw_at = tw_fill(lds + N * P, a.sub.tw, N, threadIdx.x, T);
+    // inter-pass twiddles W_n^(g k) of the full length n = groups * N from a second two-level table in LDS
+    const int n_full = a.groups * N;
+    TwLds tw_full{};
+    if (a.tw_full) tw_full = tw_fill(lds + N * P + tw_lds_elems(N), a.tw_full, n_full, threadIdx.x, T);
     float2 v[E];
     const float2* in = a.in + (col_ok ? col : 0);
     const long in_base = (long)g * a.in_g;
@@ -57,13 +61,13 @@ __global__ __launch_bounds__(16 * COLS) void k_col_static(ColFftArgs a) {
     float2* out = a.out + (col_ok ? col : 0);
     const long out_base = (long)g * a.out_g;
     const float sc = a.scale;
-    const float2* twf = a.tw_full;
+    const bool twf = a.tw_full != nullptr;
     auto out_write = [&](int k, float2& x, bool ok) {
         const long r = out_base + (long)k * a.out_m;
         if (!ok || !col_ok || r >= a.out_rows) return;
         float2 y = x;
         if (twf) {
-            const float2 w = twf[(long)g * k];
+            const float2 w = tw_full(g * k);  // g * k < n_full
             y = INV ? cmulc(y, w) : cmul(y, w);
         }
         out[r * a.out_stride] = cscale(y, sc);
@@ -80,7 +84,8 @@ void set_col_static_width(int cols) { g_col_width = (cols == 16 || cols == 32) ?
 
 template <int N, class Seq, int COLS>
 static hipError_t launch_cs_w(const ColFftArgs& a, bool inverse, hipStream_t s) {
-    const size_t lds = sizeof(float2) * (size_t)(N * (COLS + 1) + tw_lds_elems(N));
+    const size_t lds =
+        sizeof(float2) * (size_t)(N * (COLS + 1) + tw_lds_elems(N) + (a.tw_full ? tw_lds_elems(a.groups * N) : 0));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     const long ntiles = (a.ncols + COLS - 1) / COLS;
     const dim3 grid((unsigned)(ntiles * a.groups));
