@@ -628,7 +628,9 @@ static ColFftArgs with_mfma(int mfma, ColFftArgs a) {
 }
 
 // Forward column transform into permuted storage: row k1*n2 + k2 holds bin k1 + n1*k2. pass 0 = A (strided
-// length-n1 transforms with the inter-pass twiddle), pass 1 = B (contiguous length-n2 transforms).
+// length-n1 transforms with the inter-pass twiddle), pass 1 = B (contiguous length-n2 transforms). Each pass reads
+// `in` (row pitch in_stride; pass 0 reads rows >= in_rows as zero, pass 1 reads all n rows) and writes img; in
+// and img may be the same buffer (in place) or two buffers (out of place, which streams slightly faster).
 int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
                      float2* img, long stride, int ncols, int mfma) {
     const int n = cs.n;
@@ -645,7 +647,7 @@ int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* i
                                       cs.full->d_tw, 1.f, cs.p1);
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
     } else {
-        const ColFftArgs a = col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr, 1.f,
+        const ColFftArgs a = col_args(in, in_stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr, 1.f,
                                       cs.p2);
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
     }
@@ -731,7 +733,7 @@ extern "C" int asarf_range_doppler(asarf_ctx* c, const asarf_rda_params* p, asar
     auto* o = static_cast<float2*>(out->data);
     const long so = out->x_stride;
     ST_CHECK(col_forward_pass(c, cs, 0, s, src->x_stride, na, o, so, nrv));
-    ST_CHECK(col_forward_pass(c, cs, 1, s, src->x_stride, na, o, so, nrv));
+    ST_CHECK(col_forward_pass(c, cs, 1, o, so, nap, o, so, nrv));
     RcmcArgs ra{};
     ra.img = o;
     ra.stride = so;

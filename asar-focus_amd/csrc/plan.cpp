@@ -43,7 +43,7 @@ struct asarf_plan {
     ColSplit cs;
     // device buffers
     float2* img = nullptr;
-    float2* img2 = nullptr;  // out-of-place target of the RCMC stage
+    float2* img2 = nullptr;  // second image: out-of-place target of the column passes and of the RCMC stage
     bool fused = false;      // fused RCMC + matched filter + inverse pass B' (opt-in)
     float2* hconj = nullptr;
     ColConst* col = nullptr;
@@ -163,7 +163,8 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
         clear_error();
     }
     if ((st = plan_alloc(pl, (void**)&pl->img, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
-    // second image: out-of-place target of the RCMC stage (the column-tiled RCMC kernel and the fused core)
+    // second image: the passes ping-pong between the two (A img -> img2, B img2 -> img, RCMC img -> img2,
+    // B' img2 -> img, A' img -> img2), since out-of-place passes stream faster than in-place ones
     pl->fused = fz_on && az_fused_lds_bytes(pl->cs.n2) <= c->max_lds;
     if ((st = plan_alloc(pl, (void**)&pl->img2, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->hconj, sizeof(float2) * (size_t)nrp))) return fail(st);
@@ -355,13 +356,19 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     }
     HIP_CHECK(launch_range_compress(ra, p.raw_format, s));
     ST_CHECK(mark(pl, "range_compression"));
-    // 3. azimuth FFT (permuted bins), rows >= Na read as zero
-    ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv, pl->fft_mfma));
-    ST_CHECK(mark(pl, "azimuth_fft_a"));
+    // 3. azimuth FFT (permuted bins), rows >= Na read as zero; with two passes, A img -> img2 and B img2 -> img
+    // (out of place streams faster than in place)
     if (pl->cs.n2 > 1) {
-        ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv,
+        ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img2, pl->stride, pl->nrv,
+                                  pl->fft_mfma));
+        ST_CHECK(mark(pl, "azimuth_fft_a"));
+        ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img2, pl->stride, pl->nap, pl->img, pl->stride, pl->nrv,
                                   pl->fft_mfma));
         ST_CHECK(mark(pl, "azimuth_fft_b"));
+    } else {
+        ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv,
+                                  pl->fft_mfma));
+        ST_CHECK(mark(pl, "azimuth_fft_a"));
     }
     return ASARF_OK;
 }
@@ -375,6 +382,7 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
     pl->ev_names.clear();
     ST_CHECK(mark(pl, "start"));
     ST_CHECK(enqueue_to_spectrum(pl, raw));
+    float2* slc_buf = pl->img;  // where the SLC ends (img2 after an out-of-place last pass)
     RcmcArgs rc{};
     rc.img = pl->img;
     rc.stride = pl->stride;
@@ -461,16 +469,20 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
             ib.mfma = pl->fft_mfma;
             HIP_CHECK(launch_col_fft(ib, true, s));
             ST_CHECK(mark(pl, "azimuth_ifft_b"));
-            ST_CHECK(col_inverse_pass(c, cs, 1, pl->img, pl->stride, pl->nrv, p.na, 1.0f / (float)pl->nap,
-                                      pl->fft_mfma));
+            // inverse pass A' (img -> img2, out of place), 1/Nap, rows < Na: the SLC ends in img2
+            ColFftArgs ia = col_args(pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1, cs.n2,
+                                     p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
+            ia.mfma = pl->fft_mfma;
+            HIP_CHECK(launch_col_fft(ia, true, s));
             ST_CHECK(mark(pl, "azimuth_ifft_a"));
+            slc_buf = pl->img2;
         }
     }
-    *slc = asarf_image{pl->img, pl->nrv, p.na, (int)pl->stride, pl->nap, ASARF_CF32};
+    *slc = asarf_image{slc_buf, pl->nrv, p.na, (int)pl->stride, pl->nap, ASARF_CF32};
     if (p.imp) {
         // 6. crop (results window), multilook x RSL, SRGR
         MultilookArgs ma{};
-        ma.slc = pl->img;
+        ma.slc = slc_buf;
         ma.stride = pl->stride;
         ma.row0_dev = &pl->res->crop_front;
         ma.col0 = pl->near;
