@@ -104,6 +104,9 @@ SIGNATURES = {
     "asarf_range_doppler": (_ci, [_vp, _P(RdaParams), _P(Image), _P(Image), _P(Workspace)]),
     "asarf_rcmc_window_size": (_ci, []),
     "asarf_az_multilook": (_ci, [_vp, _P(Image), _cd, _P(Workspace), _P(_ci), _P(_cd)]),
+    "asarf_ingest_ers_im": (_ci, [_vp, _vp, ctypes.c_long, _ci, _ci, _vp, _ci, _P(Image)]),
+    "asarf_ingest_asar_fbaq4": (_ci, [_vp, _vp, ctypes.c_long, _ci, _vp, _ci, _vp, _vp, _ci, _P(Image)]),
+    "asarf_fbaq4_index": (_ci, [_ci, _ci]),
     "asarf_range_spreading_loss": (_ci, [_vp, _vp, _ci, _ci, _cd, _cd, _cd]),
     "asarf_srgr": (_ci, [_vp, _vp, _ci, _ci, _vp, _cd, _cd, _ci, _vp]),
     "asarf_srgr_fit_spherical": (_ci, [_cd, _cd, _ci, _cd, _cd, _cd, _vp, _P(_ci)]),

@@ -99,6 +99,39 @@ def range_doppler(ctx: Context, params: L.RdaParams, src: torch.Tensor, x_size: 
     return out
 
 
+def ingest_ers_im(ctx: Context, samples: torch.Tensor, samples_per_line: int, swst: torch.Tensor, min_swst: int,
+                  out: torch.Tensor, x_size: int) -> L.Image:
+    """ConvertErsImSamplesToComplex + NaN fill + padded copy (envisat_format_kernels.cu:116-158,
+    main_flow.cc:537-540, main.cc:233-239): samples [lines, line_bytes] uint8, swst [lines] int16/uint16 codes,
+    out [y_stride, x_stride] complex64, all on the device; out's first `lines` rows are the logical region."""
+    assert samples.is_cuda and samples.dtype == torch.uint8 and samples.dim() == 2 and samples.is_contiguous()
+    assert swst.is_cuda and swst.dtype in (torch.int16, torch.uint16) and swst.numel() == samples.shape[0]
+    img = image(out, x_size, samples.shape[0])
+    L.check(L.lib().asarf_ingest_ers_im(ctx.h, samples.data_ptr(), samples.shape[1], samples_per_line,
+                                        samples.shape[0], swst.data_ptr(), min_swst, ctypes.byref(img)))
+    return img
+
+
+def ingest_asar_fbaq4(ctx: Context, records: torch.Tensor, swst: torch.Tensor, min_swst: int, i_lut: torch.Tensor,
+                      q_lut: torch.Tensor, out: torch.Tensor, x_size: int) -> L.Image:
+    """ConvertAsarImBlocksToComplex + Fbaq4IndexCalc + NaN fill + padded copy (envisat_format_kernels.cu:160-247):
+    records [n, record_bytes] uint8, swst [n] codes, i_lut / q_lut float32 tables, out complex64 [y_stride,
+    x_stride], all on the device."""
+    assert records.is_cuda and records.dtype == torch.uint8 and records.dim() == 2 and records.is_contiguous()
+    assert swst.is_cuda and swst.dtype in (torch.int16, torch.uint16) and swst.numel() == records.shape[0]
+    assert i_lut.is_cuda and q_lut.is_cuda and i_lut.dtype == torch.float32 and i_lut.numel() == q_lut.numel()
+    img = image(out, x_size, records.shape[0])
+    L.check(L.lib().asarf_ingest_asar_fbaq4(ctx.h, records.data_ptr(), records.shape[1], records.shape[0],
+                                            swst.data_ptr(), min_swst, i_lut.data_ptr(), q_lut.data_ptr(),
+                                            i_lut.numel(), ctypes.byref(img)))
+    return img
+
+
+def fbaq4_index(block: int, code: int) -> int:
+    """Fbaq4IndexCalc (envisat_parse_util.cuh:75-84), from the library."""
+    return int(L.lib().asarf_fbaq4_index(block, code))
+
+
 def az_multilook(ctx: Context, slc: torch.Tensor, x_size: int, y_size: int, az_spacing: float):
     """AzimuthMultiLook (sar/az_multilook.cu:82-101) -> (float tensor [rows, x_size], new azimuth spacing)."""
     img = image(slc, x_size, y_size)

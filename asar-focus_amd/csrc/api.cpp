@@ -760,6 +760,68 @@ extern "C" int asarf_range_doppler(asarf_ctx* c, const asarf_rda_params* p, asar
     return clear_error();
 }
 
+// Level-0 decode (k_ingest.hip): validation shared by both formats.
+static int check_ingest(asarf_ctx* c, const void* bytes, long line_bytes, int lines, const unsigned short* swst,
+                        asarf_image* out, const char* who) {
+    if (!c || !bytes || !swst || !out) return set_error(ASARF_EINVAL, "%s: null argument", who);
+    ST_CHECK(check_cf32(out, who));
+    if (lines <= 0 || line_bytes <= 0) return set_error(ASARF_ESHAPE, "%s: no lines", who);
+    if (out->y_size != lines) return set_error(ASARF_ESHAPE, "%s: image has %d lines, input %d", who, out->y_size, lines);
+    return ASARF_OK;
+}
+
+static IngestArgs ingest_args(const void* bytes, long line_bytes, int lines, const unsigned short* swst, int min_swst,
+                              const asarf_image* out) {
+    IngestArgs a{};
+    a.bytes = static_cast<const uint8_t*>(bytes);
+    a.line_bytes = line_bytes;
+    a.lines = lines;
+    a.swst = swst;
+    a.min_swst = min_swst;
+    a.out = static_cast<float2*>(out->data);
+    a.x_size = out->x_size;
+    a.x_stride = out->x_stride;
+    a.y_stride = out->y_stride;
+    return a;
+}
+
+extern "C" int asarf_ingest_ers_im(asarf_ctx* c, const unsigned char* samples, long line_bytes, int samples_per_line,
+                                   int lines, const unsigned short* swst_codes, int min_swst, asarf_image* out) {
+    ST_CHECK(check_ingest(c, samples, line_bytes, lines, swst_codes, out, "asarf_ingest_ers_im"));
+    if (samples_per_line <= 0 || 2L * samples_per_line > line_bytes)
+        return set_error(ASARF_ESHAPE, "asarf_ingest_ers_im: %d sample pairs do not fit %ld-byte lines",
+                         samples_per_line, line_bytes);
+    ST_CHECK(ctx_init(c));
+    IngestArgs a = ingest_args(samples, line_bytes, lines, swst_codes, min_swst, out);
+    a.samples_per_line = samples_per_line;
+    a.swst_mult = 4;  // ERS SWST code step in samples (envisat_format_kernels.cu:126)
+    HIP_CHECK(launch_ingest(a, 0, c->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_ingest_asar_fbaq4(asarf_ctx* c, const unsigned char* records, long record_bytes, int records_n,
+                                       const unsigned short* swst_codes, int min_swst, const float* i_lut,
+                                       const float* q_lut, int lut_items, asarf_image* out) {
+    ST_CHECK(check_ingest(c, records, record_bytes, records_n, swst_codes, out, "asarf_ingest_asar_fbaq4"));
+    if (!i_lut || !q_lut || lut_items <= 0) return set_error(ASARF_EINVAL, "asarf_ingest_asar_fbaq4: no tables");
+    if (record_bytes < 3) return set_error(ASARF_ESHAPE, "asarf_ingest_asar_fbaq4: records too short");
+    ST_CHECK(ctx_init(c));
+    IngestArgs a = ingest_args(records, record_bytes, records_n, swst_codes, min_swst, out);
+    a.swst_mult = 1;
+    a.i_lut = i_lut;
+    a.q_lut = q_lut;
+    a.lut_items = lut_items;
+    if (lut_items >= 4096) {
+        void* lt = nullptr;
+        ST_CHECK(ctx_scratch(c, sizeof(float2) * 4096, &lt));
+        a.lut_t = static_cast<const float2*>(lt);
+    }
+    HIP_CHECK(launch_ingest(a, 1, c->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_fbaq4_index(int block, int code) { return fbaq4_index_host(block, code); }
+
 extern "C" int asarf_az_multilook(asarf_ctx* c, const asarf_image* slc, double az_spacing, asarf_workspace* out,
                                   int* out_rows, double* new_spacing) {
     if (!c || !out || !out_rows) return set_error(ASARF_EINVAL, "asarf_az_multilook: null argument");

@@ -213,6 +213,24 @@ struct SrgrArgs {
     float* out;
 };
 
+// Level-0 sample decode into a padded cf32 image (k_ingest.hip).
+struct IngestArgs {
+    const uint8_t* bytes;     // line / record y at bytes + y * line_bytes
+    long line_bytes;
+    int lines;                // decoded lines (rows of the logical region)
+    int samples_per_line;     // ERS: I/Q byte pairs per line
+    const uint16_t* swst;     // per-line sampling-window-start codes
+    int min_swst;
+    int swst_mult;            // columns per SWST code step (ERS 4, ASAR 1)
+    const float* i_lut;       // FBAQ-4 reconstruction tables
+    const float* q_lut;
+    int lut_items;
+    const float2* lut_t;      // 4096-entry scratch for the block-major table copy (nullptr: read i_lut / q_lut)
+    float2* out;
+    int x_size, x_stride, y_stride;  // logical width, row pitch and padded height of the image
+    int vec16;                // set by the launcher: column pairs stored with 16-byte stores
+};
+
 // ---- launchers (k_*.hip) ----
 hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s);
@@ -245,6 +263,8 @@ hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s);
 hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_t s);
 hipError_t launch_srgr(const SrgrArgs& a, hipStream_t s);
 hipError_t launch_zero_cols(float2* img, long stride, int rows, int c0, int c1, hipStream_t s);
+hipError_t launch_ingest(const IngestArgs& a, int kind, hipStream_t s);  // kind 0 = ERS IM, 1 = ASAR FBAQ-4
+int fbaq4_index_host(int block, int code);
 
 // Max dynamic LDS bytes per workgroup the launchers may request (set by the context at initialisation).
 void set_max_lds_bytes(size_t bytes);

@@ -70,6 +70,47 @@ def cpu_baseline(cfg, raw, min_seconds=10.0, max_reps=3):
     }
 
 
+def ingest_probe(reps=20):
+    """Level-0 decoders at full scene size (synthetic bytes): ERS IM 30000 lines of 5616 u8 I/Q pairs and ASAR IM
+    30000 FBAQ-4 records of 5616 samples, each into a 6400 x 30240 padded cf32 image (NaN fill and zero padding
+    included). GB/s counts the input bytes plus every padded output element once."""
+    import torch
+    from asarf import focus
+    ctx = focus.Context(0)
+    lines, spl = 30000, 5616
+    x_size, x_stride, y_stride = spl + 4 * 66, 6400, 30240
+    g = torch.Generator(device="cuda").manual_seed(5)
+    out = torch.empty((y_stride, x_stride), dtype=torch.complex64, device="cuda")
+    swst = (700 + 22 * torch.randint(0, 4, (lines,), device="cuda", generator=g)).to(torch.int16)
+    ers = torch.randint(0, 32, (lines, 2 * spl), device="cuda", generator=g, dtype=torch.uint8)
+    blocks = (spl + 62) // 63
+    rec = torch.randint(0, 256, (lines, 64 * blocks + 2), device="cuda", generator=g, dtype=torch.uint8)
+    length = 64 * blocks
+    rec[:, 0] = length & 0xFF
+    rec[:, 1] = length >> 8
+    i_lut = torch.randn(4096, device="cuda", generator=g)
+    q_lut = torch.randn(4096, device="cuda", generator=g)
+    res = {}
+    for name, run, in_bytes in (
+            ("ers_im_u8", lambda: focus.ingest_ers_im(ctx, ers, spl, swst, 700, out, x_size), ers.numel()),
+            ("asar_im_fbaq4", lambda: focus.ingest_asar_fbaq4(ctx, rec, swst, 700, i_lut, q_lut, out, x_size),
+             rec.numel())):
+        run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = in_bytes + 8 * x_stride * y_stride
+        res[name] = {"ms": round(ms, 4), "gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "msamples_per_s": round(lines * spl / (ms * 1e-3) / 1e6, 1)}
+    ctx.close()
+    return {"ingest_probe": res, "lines": lines, "samples_per_line": spl, "padded_image": [x_stride, y_stride]}
+
+
 def run_batch(args):
     """Heterogeneous batch of 64 ERS/ASAR blocks: per-block sequential launches against one captured graph."""
     import numpy as np
@@ -157,6 +198,8 @@ def main():
     ap.add_argument("--judge-threshold", type=float, default=1e-3)
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (range and azimuth transform schedules) before the plan is built")
+    ap.add_argument("--ingest-probe", action="store_true",
+                    help="time the level-0 decoders (ERS u8, ASAR FBAQ-4) at full scene size and print their rates")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (5 times): the known-size reference that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of the plan's kernels")
@@ -171,6 +214,9 @@ def main():
 
     if args.config == "batch64":
         run_batch(args)
+        return
+    if args.ingest_probe:
+        print(json.dumps(ingest_probe()))
         return
     if args.copy_probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")

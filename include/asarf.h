@@ -179,6 +179,27 @@ int asarf_srgr(asarf_ctx* ctx, const float* in, int in_range_size, int azimuth_s
 int asarf_srgr_fit_spherical(double slant_range_first, double range_spacing, int range_size, double orbit_radius,
                              double earth_radius, double ground_spacing, double* grsr5, int* out_range_size);
 
+/* ----------------------------------------------------------------------------------- level-0 sample decode */
+/* ConvertErsImSamplesToComplex (envisat_format/src/envisat_format_kernels.cu:116-158) fused with the NaN fill and
+ * the padded copy that follow it (main_flow.cc:537-540, main.cc:233-239), in one pass over `out` (cf32, device):
+ * line y of `lines` holds samples_per_line (I, Q) u8 pairs at samples + y * line_bytes (device); sample s lands at
+ * column s + 4 * (swst_codes[y] - min_swst) (swst_codes on the device). Logical elements that no sample reaches
+ * are NaN, padding elements 0; samples shifted past the logical row are dropped. out->y_size must equal lines. */
+int asarf_ingest_ers_im(asarf_ctx* ctx, const unsigned char* samples, long line_bytes, int samples_per_line,
+                        int lines, const unsigned short* swst_codes, int min_swst, asarf_image* out);
+/* ConvertAsarImBlocksToComplex + Fbaq4IndexCalc (envisat_format_kernels.cu:160-247,
+ * envisat_format/include/envisat_parse_util.cuh:75-84), fused likewise: record y at records + y * record_bytes
+ * is a 2-byte little-endian length L and L bytes of 64-byte blocks (a quantiser block id, then 63 codes with the I
+ * code in the high nibble); code c of block b reconstructs to lut[256 * (c > 7 ? 15 - c : 8 + c) + b] from
+ * i_lut / q_lut (lut_items floats each, device; 4096 in the reference). Sample s (the s-th code of the record)
+ * lands at column s + swst_codes[y] - min_swst. */
+int asarf_ingest_asar_fbaq4(asarf_ctx* ctx, const unsigned char* records, long record_bytes, int records_n,
+                            const unsigned short* swst_codes, int min_swst, const float* i_lut, const float* q_lut,
+                            int lut_items, asarf_image* out);
+/* Fbaq4IndexCalc (envisat_parse_util.cuh:75-84) on the host: the reconstruction-table index of code (0..15) in
+ * quantiser block `block` (0..255). */
+int asarf_fbaq4_index(int block, int code);
+
 /* Batched complex FFT (test hook; replaces the cuFFT plans of cufft_plan.cc:34-91). ROWS: `batch` transforms of
  * length n at data + i*dist, contiguous. COLUMNS: transforms of length n down `batch` columns of a row-major
  * array with row stride `dist` elements. Unnormalised unless scale != 1. In place. */

@@ -286,3 +286,61 @@ def sinc8_weights(d: float) -> np.ndarray:
     out = np.zeros(8, np.float64)
     lib().or_sinc8_weights(float(d), out.ctypes.data)
     return out
+
+
+# ------------------------------------------------------------------- level-0 sample decode (numpy restatement)
+# These follow the reference's scatter kernels literally: a NaN-filled range_size-wide buffer receives every input
+# sample at its shifted column (main_flow.cc:537-540 and the kernels below), and is then copied into a zeroed padded
+# image (main.cc:233-239). Samples whose column is outside [0, range_size) are dropped (the reference does not bound
+# that write).
+
+def fbaq4_index(block: int, code: int) -> int:
+    """Fbaq4IndexCalc (envisat_format/include/envisat_parse_util.cuh:75-84)."""
+    conv = 15 - code if code > 7 else 8 + code
+    return 256 * conv + block
+
+
+def _padded(buf: np.ndarray, x_stride: int, y_stride: int) -> np.ndarray:
+    out = np.zeros((y_stride, x_stride), np.complex64)
+    out[: buf.shape[0], : buf.shape[1]] = buf
+    return out
+
+
+def ingest_ers_im(samples: np.ndarray, samples_per_line: int, swst: np.ndarray, min_swst: int, range_size: int,
+                  x_stride: int, y_stride: int) -> np.ndarray:
+    """ConvertErsImSamplesToComplexKernel (envisat_format/src/envisat_format_kernels.cu:116-133): byte x of line y
+    is component x % 2 of column x / 2 + 4 * (swst[y] - min_swst)."""
+    lines = samples.shape[0]
+    buf = np.full((lines, range_size), np.nan + 1j * np.nan, np.complex64)
+    for y in range(lines):
+        off = 4 * (int(swst[y]) - min_swst)
+        b = samples[y, : 2 * samples_per_line].astype(np.float32)
+        cols = np.arange(samples_per_line) + off
+        ok = (cols >= 0) & (cols < range_size)
+        buf[y, cols[ok]] = (b[0::2] + 1j * b[1::2])[ok]
+    return _padded(buf, x_stride, y_stride)
+
+
+def ingest_asar_fbaq4(records: np.ndarray, swst: np.ndarray, min_swst: int, i_lut: np.ndarray, q_lut: np.ndarray,
+                      range_size: int, x_stride: int, y_stride: int) -> np.ndarray:
+    """ConvertAsarImBlocksToComplexKernel (envisat_format/src/envisat_format_kernels.cu:160-210): record = 2-byte
+    little-endian length L, then blocks of 64 bytes (block id, 63 codes); byte x >= 2 of record y with k = x - 2 <
+    L and k % 64 != 0 goes to column k - (k / 64 + 1) + swst[y] - min_swst, I and Q from the tables."""
+    n, item = records.shape
+    lut_items = i_lut.size
+    buf = np.full((n, range_size), np.nan + 1j * np.nan, np.complex64)
+    for y in range(n):
+        rec = records[y]
+        length = int(rec[0]) | (int(rec[1]) << 8)
+        k = np.arange(0, item - 2)
+        k = k[(k < length) & (k % 64 != 0)]
+        blk = k // 64
+        block_id = rec[2 + 64 * blk].astype(np.int64)
+        code = rec[2 + k].astype(np.int64)
+        ci, cq = code >> 4, code & 15
+        ii = 256 * np.where(ci > 7, 15 - ci, 8 + ci) + block_id
+        qi = 256 * np.where(cq > 7, 15 - cq, 8 + cq) + block_id
+        cols = k - (blk + 1) + (int(swst[y]) - min_swst)
+        ok = (ii < lut_items) & (qi < lut_items) & (cols >= 0) & (cols < range_size)
+        buf[y, cols[ok]] = i_lut[ii[ok]] + 1j * q_lut[qi[ok]]
+    return _padded(buf, x_stride, y_stride)
