@@ -107,6 +107,9 @@ SIGNATURES = {
     "asarf_ingest_ers_im": (_ci, [_vp, _vp, ctypes.c_long, _ci, _ci, _vp, _ci, _P(Image)]),
     "asarf_ingest_asar_fbaq4": (_ci, [_vp, _vp, ctypes.c_long, _ci, _vp, _ci, _vp, _vp, _ci, _P(Image)]),
     "asarf_fbaq4_index": (_ci, [_ci, _ci]),
+    "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
+    "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
+    "asarf_image_statistics": (_ci, [_vp, _P(Image), _ci, _P(_cd)]),
     "asarf_range_spreading_loss": (_ci, [_vp, _vp, _ci, _ci, _cd, _cd, _cd]),
     "asarf_srgr": (_ci, [_vp, _vp, _ci, _ci, _vp, _cd, _cd, _ci, _vp]),
     "asarf_srgr_fit_spherical": (_ci, [_cd, _cd, _ci, _cd, _cd, _cd, _vp, _P(_ci)]),

@@ -127,6 +127,34 @@ def ingest_asar_fbaq4(ctx: Context, records: torch.Tensor, swst: torch.Tensor, m
     return img
 
 
+def condition_slc(ctx: Context, slc: torch.Tensor, x_size: int, y_size: int, calibration: float,
+                  header_bytes: int) -> torch.Tensor:
+    """ConditionResultsSLC (envisat_format_kernels.cu:28-68): uint8 tensor [y_size, header + 4 x_size]."""
+    img = image(slc, x_size, y_size)
+    out = torch.empty((y_size, header_bytes + 4 * x_size), dtype=torch.uint8, device=slc.device)
+    L.check(L.lib().asarf_condition_slc(ctx.h, ctypes.byref(img), calibration, header_bytes, out.data_ptr(),
+                                        out.numel()))
+    return out
+
+
+def condition_detected(ctx: Context, img: torch.Tensor, calibration: float, header_bytes: int) -> torch.Tensor:
+    """ConditionResultsDetected (envisat_format_kernels.cu:70-98): uint8 tensor [rows, header + 2 cols]."""
+    assert img.is_cuda and img.dtype == torch.float32 and img.dim() == 2 and img.is_contiguous()
+    rows, cols = img.shape
+    out = torch.empty((rows, header_bytes + 2 * cols), dtype=torch.uint8, device=img.device)
+    L.check(L.lib().asarf_condition_detected(ctx.h, img.data_ptr(), cols, rows, calibration, header_bytes,
+                                             out.data_ptr(), out.numel()))
+    return out
+
+
+def image_statistics(ctx: Context, t: torch.Tensor, x_size: int, y_size: int, imaged: bool = False) -> list:
+    """CalculateStatistics / CalculateStatisticsImaged (main_flow.cc:666-676)."""
+    img = image(t, x_size, y_size)
+    out = (ctypes.c_double * 4)()
+    L.check(L.lib().asarf_image_statistics(ctx.h, ctypes.byref(img), int(imaged), out))
+    return list(out)
+
+
 def fbaq4_index(block: int, code: int) -> int:
     """Fbaq4IndexCalc (envisat_parse_util.cuh:75-84), from the library."""
     return int(L.lib().asarf_fbaq4_index(block, code))

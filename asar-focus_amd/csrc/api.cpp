@@ -822,6 +822,93 @@ extern "C" int asarf_ingest_asar_fbaq4(asarf_ctx* c, const unsigned char* record
 
 extern "C" int asarf_fbaq4_index(int block, int code) { return fbaq4_index_host(block, code); }
 
+// Output conditioning (k_output.hip).
+extern "C" int asarf_condition_slc(asarf_ctx* c, const asarf_image* slc, float calibration, int header_bytes,
+                                   void* dest, size_t dest_bytes) {
+    if (!c || !dest) return set_error(ASARF_EINVAL, "asarf_condition_slc: null argument");
+    ST_CHECK(check_cf32(slc, "asarf_condition_slc"));
+    if (header_bytes < 0 || header_bytes > slc->x_size)
+        return set_error(ASARF_ESHAPE, "asarf_condition_slc: header of %d bytes", header_bytes);
+    const long rec = (long)header_bytes + 4L * slc->x_size;
+    if (dest_bytes < (size_t)rec * slc->y_size)
+        return set_error(ASARF_EWORKSPACE, "asarf_condition_slc: output %zu < %ld bytes", dest_bytes,
+                         rec * slc->y_size);
+    ST_CHECK(ctx_init(c));
+    OutputArgs a{};
+    a.slc = static_cast<const float2*>(slc->data);
+    a.x_stride = slc->x_stride;
+    a.x_size = slc->x_size;
+    a.y_size = slc->y_size;
+    a.calibration = calibration;
+    a.header = header_bytes;
+    a.record_bytes = rec;
+    a.dest = static_cast<unsigned char*>(dest);
+    HIP_CHECK(launch_condition(a, 0, c->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_condition_detected(asarf_ctx* c, const float* img, int range_size, int azimuth_size,
+                                        float calibration, int header_bytes, void* dest, size_t dest_bytes) {
+    if (!c || !img || !dest) return set_error(ASARF_EINVAL, "asarf_condition_detected: null argument");
+    if (range_size <= 0 || azimuth_size <= 0) return set_error(ASARF_ESHAPE, "asarf_condition_detected: empty image");
+    if (header_bytes < 0 || header_bytes > range_size)
+        return set_error(ASARF_ESHAPE, "asarf_condition_detected: header of %d bytes", header_bytes);
+    const long rec = (long)header_bytes + 2L * range_size;
+    if (dest_bytes < (size_t)rec * azimuth_size)
+        return set_error(ASARF_EWORKSPACE, "asarf_condition_detected: output %zu < %ld bytes", dest_bytes,
+                         rec * azimuth_size);
+    ST_CHECK(ctx_init(c));
+    OutputArgs a{};
+    a.det = img;
+    a.x_stride = range_size;
+    a.x_size = range_size;
+    a.y_size = azimuth_size;
+    a.calibration = calibration;
+    a.header = header_bytes;
+    a.record_bytes = rec;
+    a.dest = static_cast<unsigned char*>(dest);
+    HIP_CHECK(launch_condition(a, 1, c->stream));
+    return clear_error();
+}
+
+extern "C" int asarf_image_statistics(asarf_ctx* c, const asarf_image* img, int imaged, double* out4) {
+    if (!c || !out4) return set_error(ASARF_EINVAL, "asarf_image_statistics: null argument");
+    ST_CHECK(check_cf32(img, "asarf_image_statistics"));
+    ST_CHECK(ctx_init(c));
+    const int nwg = std::min(img->y_size, 1024);
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(double) * (size_t)(nwg + 1) * 12, &scratch));
+    double* partials = static_cast<double*>(scratch);
+    double* totals = partials + (size_t)nwg * 12;
+    auto* f2 = static_cast<const float2*>(img->data);
+    HIP_CHECK(launch_moments_cf32(f2, img->x_stride, img->x_size, img->y_size, img->x_stride, partials, nwg, c->stream));
+    HIP_CHECK(launch_reduce_f64(partials, nwg, 12, totals, c->stream));
+    void* hp = nullptr;
+    ST_CHECK(ctx_pinned(c, sizeof(double) * 12, &hp));
+    HIP_CHECK(hipMemcpyAsync(hp, totals, sizeof(double) * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    double t[12];
+    std::memcpy(t, hp, sizeof(t));
+    // CalcMean / CalcStdDev (device_padded_image.cu:331-414): NaN-skipping sums over the logical region, divided
+    // by its full sample count
+    const double n = (double)img->x_size * (double)img->y_size;
+    const double nL = t[0], mi = t[1] / n, mq = t[2] / n;
+    const double si = std::sqrt(std::max(0.0, (t[3] - 2.0 * mi * t[1] + nL * mi * mi) / n));
+    const double sq = std::sqrt(std::max(0.0, (t[4] - 2.0 * mq * t[2] + nL * mq * mq) / n));
+    if (imaged) {  // CalculateStatisticsImaged (main_flow.cc:672-676)
+        out4[0] = (mi + mq) / 2;
+        out4[1] = 0.0;
+        out4[2] = (si + sq) / 2;
+        out4[3] = 0.0;
+    } else {  // CalculateStatistics (main_flow.cc:666-670)
+        out4[0] = mi;
+        out4[1] = mq;
+        out4[2] = si;
+        out4[3] = sq;
+    }
+    return clear_error();
+}
+
 extern "C" int asarf_az_multilook(asarf_ctx* c, const asarf_image* slc, double az_spacing, asarf_workspace* out,
                                   int* out_rows, double* new_spacing) {
     if (!c || !out || !out_rows) return set_error(ASARF_EINVAL, "asarf_az_multilook: null argument");

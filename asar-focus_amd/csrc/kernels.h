@@ -231,6 +231,18 @@ struct IngestArgs {
     int vec16;                // set by the launcher: column pairs stored with 16-byte stores
 };
 
+// Output conditioning into MDS records (k_output.hip).
+struct OutputArgs {
+    const float2* slc;        // SLC (cf32), or
+    const float* det;         // detected image (f32)
+    long x_stride;            // elements per input row
+    int x_size, y_size;
+    float calibration;
+    int header;               // record header bytes (zeroed)
+    long record_bytes;        // header + x_size * bytes per pixel
+    unsigned char* dest;
+};
+
 // ---- launchers (k_*.hip) ----
 hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s);
@@ -264,6 +276,7 @@ hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_
 hipError_t launch_srgr(const SrgrArgs& a, hipStream_t s);
 hipError_t launch_zero_cols(float2* img, long stride, int rows, int c0, int c1, hipStream_t s);
 hipError_t launch_ingest(const IngestArgs& a, int kind, hipStream_t s);  // kind 0 = ERS IM, 1 = ASAR FBAQ-4
+hipError_t launch_condition(const OutputArgs& a, int detected, hipStream_t s);
 int fbaq4_index_host(int block, int code);
 
 // Max dynamic LDS bytes per workgroup the launchers may request (set by the context at initialisation).

@@ -107,6 +107,26 @@ def ingest_probe(reps=20):
         nbytes = in_bytes + 8 * x_stride * y_stride
         res[name] = {"ms": round(ms, 4), "gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
                      "msamples_per_s": round(lines * spl / (ms * 1e-3) / 1e6, 1)}
+    del ers, rec
+    # output conditioning at the ASAR scene's product sizes: SLC 30000 x 5095 (pitch 5104) to int16 records,
+    # detected 10128 x 7033 to uint16 records, 17-byte record headers
+    slc = torch.randn((30240, 5104), dtype=torch.complex64, device="cuda", generator=g) * 1000
+    det = torch.rand((10128, 7033), device="cuda", generator=g) * 1e4
+    for name, run, nbytes in (
+            ("condition_slc", lambda: focus.condition_slc(ctx, slc, 5095, 30000, 1.0, 17),
+             30000 * 5095 * 8 + 30000 * (17 + 4 * 5095)),
+            ("condition_detected", lambda: focus.condition_detected(ctx, det, 1.0, 17),
+             det.numel() * 4 + 10128 * (17 + 2 * 7033))):
+        run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": round(ms, 4), "gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
     ctx.close()
     return {"ingest_probe": res, "lines": lines, "samples_per_line": spl, "padded_image": [x_stride, y_stride]}
 
@@ -199,7 +219,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the autotuner (range and azimuth transform schedules) before the plan is built")
     ap.add_argument("--ingest-probe", action="store_true",
-                    help="time the level-0 decoders (ERS u8, ASAR FBAQ-4) at full scene size and print their rates")
+                    help="time the level-0 decoders (ERS u8, ASAR FBAQ-4) and the output conditioning at full scene "
+                         "size and print their rates")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (5 times): the known-size reference that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of the plan's kernels")

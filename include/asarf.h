@@ -200,6 +200,23 @@ int asarf_ingest_asar_fbaq4(asarf_ctx* ctx, const unsigned char* records, long r
  * quantiser block `block` (0..255). */
 int asarf_fbaq4_index(int block, int code);
 
+/* ------------------------------------------------------------------------------------- output conditioning */
+/* ConditionResultsSLC / CalibrateClampToBigEndian (envisat_format/src/envisat_format_kernels.cu:28-68, 100-111):
+ * writes slc->y_size records to dest (device) of header_bytes zero bytes then slc->x_size pixels of big-endian
+ * int16 I and Q = clamp(value * calibration) truncated toward zero. dest_bytes >= y_size * (header + 4 x_size). */
+int asarf_condition_slc(asarf_ctx* ctx, const asarf_image* slc, float calibration, int header_bytes, void* dest,
+                        size_t dest_bytes);
+/* ConditionResultsDetected / CalibrateClampToBigEndianDetected (envisat_format_kernels.cu:70-98, 113-121): records
+ * of header_bytes zero bytes then range_size uint16 of 10 log10(value * calibration) (single precision), saturated
+ * to [0, 65535] with negative and NaN values 0 (survey quirk Q17 made defined), stored low byte first as the
+ * reference's kernel does. img is azimuth_size x range_size floats (device, contiguous). */
+int asarf_condition_detected(asarf_ctx* ctx, const float* img, int range_size, int azimuth_size, float calibration,
+                             int header_bytes, void* dest, size_t dest_bytes);
+/* CalculateStatistics / CalculateStatisticsImaged (main_flow.cc:666-676): out4 = {mean I, mean Q, std I, std Q}
+ * of the logical region (NaN samples skipped, sums divided by x_size * y_size), or with imaged != 0
+ * {(mean I + mean Q) / 2, 0, (std I + std Q) / 2, 0}. Synchronises. */
+int asarf_image_statistics(asarf_ctx* ctx, const asarf_image* img, int imaged, double* out4);
+
 /* Batched complex FFT (test hook; replaces the cuFFT plans of cufft_plan.cc:34-91). ROWS: `batch` transforms of
  * length n at data + i*dist, contiguous. COLUMNS: transforms of length n down `batch` columns of a row-major
  * array with row stride `dist` elements. Unnormalised unless scale != 1. In place. */

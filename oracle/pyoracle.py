@@ -344,3 +344,53 @@ def ingest_asar_fbaq4(records: np.ndarray, swst: np.ndarray, min_swst: int, i_lu
         ok = (ii < lut_items) & (qi < lut_items) & (cols >= 0) & (cols < range_size)
         buf[y, cols[ok]] = i_lut[ii[ok]] + 1j * q_lut[qi[ok]]
     return _padded(buf, x_stride, y_stride)
+
+
+# ---------------------------------------------------------------- output conditioning (numpy restatement)
+def _sat_trunc(v: np.ndarray, lo: float, hi: float) -> np.ndarray:
+    """float32 -> integer as the reference's device casts behave: clamp, truncate toward zero, NaN -> 0."""
+    v = np.asarray(v, np.float32)
+    out = np.trunc(np.clip(np.nan_to_num(v, nan=0.0, posinf=hi, neginf=lo), lo, hi))
+    return out.astype(np.int64)
+
+
+def condition_slc(slc: np.ndarray, x_size: int, y_size: int, calibration: float, header: int) -> np.ndarray:
+    """CalibrateClampToBigEndian (envisat_format/src/envisat_format_kernels.cu:28-68): per row, header zero bytes then
+    big-endian int16 I and Q of clamp(value * calibration) (float32 product)."""
+    v = np.asarray(slc[:y_size, :x_size], np.complex64)
+    cal = np.float32(calibration)
+    iq = np.empty((y_size, 2 * x_size), ">i2")  # big-endian storage
+    iq[:, 0::2] = _sat_trunc(v.real * cal, -32768.0, 32767.0)
+    iq[:, 1::2] = _sat_trunc(v.imag * cal, -32768.0, 32767.0)
+    out = np.zeros((y_size, header + 4 * x_size), np.uint8)
+    out[:, header:] = iq.view(np.uint8).reshape(y_size, 4 * x_size)
+    return out
+
+
+def condition_detected(img: np.ndarray, calibration: float, header: int) -> np.ndarray:
+    """CalibrateClampToBigEndianDetected (envisat_format_kernels.cu:70-98): per row, header zero bytes then uint16 of
+    10 log10(value * calibration) in float32, saturated to [0, 65535], low byte first (the kernel does not swap)."""
+    rows, cols = img.shape
+    with np.errstate(divide="ignore", invalid="ignore"):
+        val = np.float32(10.0) * np.log10(np.asarray(img, np.float32) * np.float32(calibration))
+    u = np.empty((rows, cols), "<u2")  # little-endian storage
+    u[:] = _sat_trunc(val, 0.0, 65535.0)
+    out = np.zeros((rows, header + 2 * cols), np.uint8)
+    out[:, header:] = u.view(np.uint8).reshape(rows, 2 * cols)
+    return out
+
+
+def image_statistics(img: np.ndarray, x_size: int, y_size: int, imaged: bool = False) -> list:
+    """CalculateStatistics(Imaged) (main_flow.cc:666-676) over CalcMean / CalcStdDev
+    (cuda_util/device_padded_image.cu:331-414): a sample with a NaN in either component is skipped whole (:179,
+    :230), sums are divided by x_size * y_size."""
+    v = np.asarray(img[:y_size, :x_size], np.complex128)
+    n = float(x_size * y_size)
+    ok = ~(np.isnan(v.real) | np.isnan(v.imag))
+    re, im = v.real[ok], v.imag[ok]
+    mi, mq = re.sum() / n, im.sum() / n
+    si = np.sqrt(np.sum((re - mi) ** 2) / n)
+    sq = np.sqrt(np.sum((im - mq) ** 2) / n)
+    if imaged:
+        return [(mi + mq) / 2, 0.0, (si + sq) / 2, 0.0]
+    return [mi, mq, si, sq]
