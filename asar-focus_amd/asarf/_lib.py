@@ -78,6 +78,20 @@ class FocusResults(ctypes.Structure):
 
 _vp, _ci, _cd, _cf, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
 _P = ctypes.POINTER
+class Osv(ctypes.Structure):
+    _fields_ = [("t", ctypes.c_double), ("pos", ctypes.c_double * 3), ("vel", ctypes.c_double * 3)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("osv", ctypes.POINTER(Osv)), ("n_osv", ctypes.c_int), ("first_line_time", ctypes.c_double),
+                ("line_time_interval", ctypes.c_double), ("azimuth_size", ctypes.c_int),
+                ("range_size", ctypes.c_int), ("chirp_samples", ctypes.c_int),
+                ("slant_range_first", ctypes.c_double), ("range_spacing", ctypes.c_double),
+                ("carrier_frequency", ctypes.c_double), ("prf", ctypes.c_double),
+                ("az_bandwidth_fraction", ctypes.c_double), ("vr_poly", ctypes.c_double * 3),
+                ("dc_poly", ctypes.c_double * 3), ("center_point", ctypes.c_double * 3)]
+
+
 SIGNATURES = {
     "asarf_ctx_create": (_ci, [_ci, _vp, _P(_vp)]),
     "asarf_ctx_destroy": (_ci, [_vp]),
@@ -107,6 +121,14 @@ SIGNATURES = {
     "asarf_ingest_ers_im": (_ci, [_vp, _vp, ctypes.c_long, _ci, _ci, _vp, _ci, _P(Image)]),
     "asarf_ingest_asar_fbaq4": (_ci, [_vp, _vp, ctypes.c_long, _ci, _vp, _ci, _vp, _vp, _ci, _P(Image)]),
     "asarf_fbaq4_index": (_ci, [_ci, _ci]),
+    "asarf_interpolate_orbit": (_ci, [_P(Osv), _ci, _cd, _P(Osv)]),
+    "asarf_geolocate": (_ci, [_P(_cd), _P(_cd), _P(_cd), _cd, _P(_cd)]),
+    "asarf_incidence_angle": (_cd, [_P(_cd), _P(_cd)]),
+    "asarf_estimate_processing_velocity": (_ci, [_P(Geometry), _P(_cd), _P(_cd), _P(_cd), _P(_ci)]),
+    "asarf_srgr_fit": (_ci, [_P(Geometry), _cd, _P(_cd), _P(_ci)]),
+    "asarf_geo_to_xyz": (_ci, [_cd, _cd, _cd, _P(_cd)]),
+    "asarf_xyz_to_geo": (_ci, [_P(_cd), _P(_cd)]),
+    "asarf_scene_center": (_ci, [_P(Geometry), _P(_cd), _P(_cd), _P(_cd)]),
     "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_image_statistics": (_ci, [_vp, _P(Image), _ci, _P(_cd)]),

@@ -198,6 +198,80 @@ def srgr_fit_spherical(r_first, dr, range_size, orbit_radius, earth_radius, gs=1
     return g, n.value
 
 
+# ----------------------------------------------------------------------------------- FP64 host geometry ----
+def _osv_array(ts, pos, vel):
+    arr = (L.Osv * len(ts))()
+    for i, t in enumerate(ts):
+        arr[i].t = float(t)
+        for k in range(3):
+            arr[i].pos[k] = float(pos[i][k])
+            arr[i].vel[k] = float(vel[i][k])
+    return arr
+
+
+def _vec3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+def interpolate_orbit(ts, pos, vel, t: float):
+    """InterpolateOrbit (sar/orbit_state_vector.cc:23-48): (position, velocity) at time t."""
+    arr = _osv_array(ts, pos, vel)
+    o = L.Osv()
+    L.check(L.lib().asarf_interpolate_orbit(arr, len(ts), float(t), ctypes.byref(o)))
+    return np.array(o.pos[:]), np.array(o.vel[:])
+
+
+def geolocate(sensor_pos, sensor_vel, guess, slant_range: float) -> np.ndarray:
+    """RangeDopplerGeoLocate (util/src/geo_tools.cc:27-85): zero-Doppler WGS84 point at the slant range."""
+    out = _vec3([0, 0, 0])
+    L.check(L.lib().asarf_geolocate(_vec3(sensor_pos), _vec3(sensor_vel), _vec3(guess), float(slant_range), out))
+    return np.array(out[:])
+
+
+def incidence_angle(earth_point, sensor_pos) -> float:
+    """CalcIncidenceAngle (util/src/geo_tools.cc:87-106), degrees."""
+    return float(L.lib().asarf_incidence_angle(_vec3(earth_point), _vec3(sensor_pos)))
+
+
+def geometry(g: dict):
+    """asarf_geometry from a dict with the keys of the struct (orbit as osv_t / osv_pos / osv_vel); returns the
+    struct and the state-vector array it points to (keep both alive)."""
+    arr = _osv_array(g["osv_t"], g["osv_pos"], g["osv_vel"])
+    s = L.Geometry()
+    s.osv = ctypes.cast(arr, ctypes.POINTER(L.Osv))
+    s.n_osv = len(g["osv_t"])
+    for k in ("first_line_time", "line_time_interval", "slant_range_first", "range_spacing", "carrier_frequency",
+              "prf", "az_bandwidth_fraction"):
+        setattr(s, k, float(g[k]))
+    for k in ("azimuth_size", "range_size", "chirp_samples"):
+        setattr(s, k, int(g[k]))
+    for k in ("vr_poly", "dc_poly", "center_point"):
+        getattr(s, k)[:] = [float(v) for v in g[k]]
+    return s, arr
+
+
+def estimate_processing_velocity(g: dict):
+    """EstimateProcessingVelocity (sar/processing_velocity_estimation.cc:98-121): (Vr polynomial, Vr samples,
+    range indices of the samples)."""
+    s, keep = geometry(g)
+    poly, vrs, xs, n = np.zeros(3), np.zeros(16), np.zeros(16), ctypes.c_int()
+    L.check(L.lib().asarf_estimate_processing_velocity(ctypes.byref(s), poly.ctypes.data_as(L._P(ctypes.c_double)),
+                                                        vrs.ctypes.data_as(L._P(ctypes.c_double)),
+                                                        xs.ctypes.data_as(L._P(ctypes.c_double)), ctypes.byref(n)))
+    del keep
+    return poly, vrs[:n.value], xs[:n.value]
+
+
+def srgr_fit(g: dict, ground_spacing: float = 12.5):
+    """Slant-to-ground polynomial and output width from the orbit (sar/srgr.cu:73-107)."""
+    s, keep = geometry(g)
+    poly, n = np.zeros(5), ctypes.c_int()
+    L.check(L.lib().asarf_srgr_fit(ctypes.byref(s), float(ground_spacing), poly.ctypes.data_as(L._P(ctypes.c_double)),
+                                   ctypes.byref(n)))
+    del keep
+    return poly, n.value
+
+
 def fft(ctx: Context, t: torch.Tensor, layout: int = L.FFT_ROWS, inverse: bool = False, scale: float = 1.0,
         n: int | None = None, batch: int | None = None, split_bf16: bool = False):
     """Batched C2C FFT in place (rows: transforms along the last axis; columns: along the first). split_bf16
@@ -467,3 +541,26 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def geo_to_xyz(latitude: float, longitude: float, altitude: float = 0.0) -> np.ndarray:
+    """Geo2xyzWgs84 (util/include/geo_tools.h:52-74)."""
+    out = _vec3([0, 0, 0])
+    L.check(L.lib().asarf_geo_to_xyz(float(latitude), float(longitude), float(altitude), out))
+    return np.array(out[:])
+
+
+def xyz_to_geo(xyz):
+    """xyz2geoWGS84 (util/include/geo_tools.h:76-95): (latitude, longitude) in degrees."""
+    out = (ctypes.c_double * 2)()
+    L.check(L.lib().asarf_xyz_to_geo(_vec3(xyz), out))
+    return out[0], out[1]
+
+
+def scene_center(g: dict, guess_xyz):
+    """Scene centre and azimuth spacing (main_flow.cc:499-522)."""
+    s, keep = geometry(g)
+    c, sp = _vec3([0, 0, 0]), ctypes.c_double()
+    L.check(L.lib().asarf_scene_center(ctypes.byref(s), _vec3(guess_xyz), c, ctypes.byref(sp)))
+    del keep
+    return np.array(c[:]), sp.value

@@ -200,6 +200,56 @@ int asarf_ingest_asar_fbaq4(asarf_ctx* ctx, const unsigned char* records, long r
  * quantiser block `block` (0..255). */
 int asarf_fbaq4_index(int block, int code);
 
+/* ------------------------------------------------------------------------------ FP64 host geometry (host) */
+/* Orbit state vector (sar/orbit_state_vector.h): ECEF position [m] and velocity [m/s] at time t [s]. Times are on
+ * any epoch near the data (a double keeps 0.1 us at 1e8 s) and are used at the reference's whole-microsecond
+ * resolution. */
+typedef struct {
+    double t;
+    double pos[3];
+    double vel[3];
+} asarf_osv;
+
+/* What the geometry functions read of SARMetadata (sar/sar_metadata.h): orbit, line timing, image and radar. */
+typedef struct {
+    const asarf_osv* osv;        /* state vectors, increasing t */
+    int n_osv;
+    double first_line_time;      /* s, same epoch as osv[].t */
+    double line_time_interval;   /* s */
+    int azimuth_size, range_size, chirp_samples;
+    double slant_range_first, range_spacing;                   /* m */
+    double carrier_frequency, prf, az_bandwidth_fraction;      /* Hz, Hz, fraction */
+    double vr_poly[3], dc_poly[3];  /* current Vr and Doppler polynomials over the range index (aperture length) */
+    double center_point[3];         /* ECEF initial guess of the geolocation */
+} asarf_geometry;
+
+/* InterpolateOrbit (sar/orbit_state_vector.cc:23-48): Lagrange interpolation through all n state vectors. */
+int asarf_interpolate_orbit(const asarf_osv* osv, int n, double t, asarf_osv* out);
+/* RangeDopplerGeoLocate (util/src/geo_tools.cc:27-85): the zero-Doppler point at slant_range on the WGS84 ellipsoid
+ * (Newton from init_guess until the step is < 1e-6 m; ASARF_EINVAL after 101 steps). */
+int asarf_geolocate(const double* sensor_pos, const double* sensor_vel, const double* init_guess, double slant_range,
+                    double* out_xyz);
+/* CalcIncidenceAngle (util/src/geo_tools.cc:87-106), degrees. */
+double asarf_incidence_angle(const double* earth_point, const double* sensor_pos);
+/* EstimateProcessingVelocity (sar/processing_velocity_estimation.cc:56-121): Vr at range indices 0, step, 2 step, ...
+ * below range_size - chirp_samples (step = that / 10; 10 or 11 positions) from the orbit (hyperbolic range fit over
+ * 9 aperture samples) and their degree-2 fit, highest power first. Optional outputs: the samples (up to 16) and
+ * their range indices. */
+int asarf_estimate_processing_velocity(const asarf_geometry* geometry, double* vr_poly3, double* vr_samples,
+                                       double* range_samples, int* n_samples);
+/* Host part of SlantRangeToGroundRangeInterpolation (sar/srgr.cu:73-107) from the orbit: grsr[5] (degree 4, highest
+ * power first, slant - R_first as a function of ground distance) and the output range size. */
+int asarf_srgr_fit(const asarf_geometry* geometry, double ground_spacing, double* grsr5, int* out_range_size);
+/* Geo2xyzWgs84 (util/include/geo_tools.h:52-74): geodetic latitude/longitude [deg] and height [m] to ECEF. */
+int asarf_geo_to_xyz(double latitude, double longitude, double altitude, double* out_xyz);
+/* xyz2geoWGS84 (util/include/geo_tools.h:76-95): ECEF to geodetic [latitude, longitude] in degrees (Bowring). */
+int asarf_xyz_to_geo(const double* xyz, double* out_lat_lon);
+/* Scene centre and azimuth pixel spacing (main_flow.cc:499-522): the mid-line, mid-swath zero-Doppler point from
+ * guess_xyz, and its distance to the same range's point one line later (azimuth_spacing may be NULL). Reads every
+ * field of geometry except vr_poly, dc_poly and center_point. */
+int asarf_scene_center(const asarf_geometry* geometry, const double* guess_xyz, double* out_center_xyz,
+                       double* azimuth_spacing);
+
 /* ------------------------------------------------------------------------------------- output conditioning */
 /* ConditionResultsSLC / CalibrateClampToBigEndian (envisat_format/src/envisat_format_kernels.cu:28-68, 100-111):
  * writes slc->y_size records to dest (device) of header_bytes zero bytes then slc->x_size pixels of big-endian

@@ -394,3 +394,133 @@ def image_statistics(img: np.ndarray, x_size: int, y_size: int, imaged: bool = F
     if imaged:
         return [(mi + mq) / 2, 0.0, (si + sq) / 2, 0.0]
     return [mi, mq, si, sq]
+
+
+# ------------------------------------------------------------------- FP64 host geometry (numpy restatement)
+WGS84_A, WGS84_B, SPEED_OF_LIGHT = 6378137.0, 6356752.314245, 299792458.0
+
+
+def interpolate_orbit(ts, pos, vel, t):
+    """InterpolateOrbit (sar/orbit_state_vector.cc:23-48): Lagrange through every state vector, times relative to the
+    first in whole microseconds."""
+    ts = np.asarray(ts, np.float64)
+    rel = lambda x: np.round((x - ts[0]) * 1e6) * 1e-6
+    x, xs = rel(t), rel(ts)
+    p, v = np.zeros(3), np.zeros(3)
+    for i in range(len(ts)):
+        m = 1.0
+        for j in range(len(ts)):
+            if i != j:
+                m *= (x - xs[j]) / (xs[i] - xs[j])
+        p += m * np.asarray(pos[i], np.float64)
+        v += m * np.asarray(vel[i], np.float64)
+    return p, v
+
+
+def geolocate(sensor_pos, sensor_vel, guess, slant_range):
+    """RangeDopplerGeoLocate (util/src/geo_tools.cc:27-85): Newton on the Doppler, range and WGS84 equations."""
+    s, v, p = (np.asarray(a, np.float64) for a in (sensor_pos, sensor_vel, guess))
+    p = p.copy()
+    a2, b2 = WGS84_A ** 2, WGS84_B ** 2
+    for _ in range(101):
+        d = p - s
+        A = np.array([v, 2.0 * d, [2 * p[0] / a2, 2 * p[1] / a2, 2 * p[2] / b2]])
+        B = np.array([v @ d, d @ d - slant_range ** 2, p[0] ** 2 / a2 + p[1] ** 2 / a2 + p[2] ** 2 / b2 - 1.0])
+        dx = np.linalg.solve(A, -B)
+        p += dx
+        if np.linalg.norm(dx) < 1e-6:
+            return p
+    raise RuntimeError("geolocation did not converge")
+
+
+def incidence_angle(earth_point, sensor_pos):
+    """CalcIncidenceAngle (util/src/geo_tools.cc:87-106), degrees (full-precision b, util/include/geo_tools.h:36)."""
+    e = np.asarray(earth_point, np.float64)
+    n = np.array([e[0] / WGS84_A ** 2, e[1] / WGS84_A ** 2, e[2] / WGS84_B_FULL ** 2])
+    d = np.asarray(sensor_pos, np.float64) - e
+    return float(np.degrees(np.arccos(n @ d / (np.linalg.norm(n) * np.linalg.norm(d)))))
+
+
+def _line_time(g, idx):
+    """CalcAzimuthTime (sar/sar_metadata.h:90-93): whole microseconds after the first line."""
+    return g["first_line_time"] + np.trunc(g["line_time_interval"] * idx * 1e6) * 1e-6
+
+
+def estimate_processing_velocity(g):
+    """EstimateProcessingVelocity (sar/processing_velocity_estimation.cc:56-121) on a geometry dict."""
+    az = g["azimuth_size"] // 2
+    range_end = g["range_size"] - g["chirp_samples"]
+    step = range_end // 10
+    xs, vrs = [], []
+    for x in range(0, range_end, step):
+        r0 = g["slant_range_first"] + x * g["range_spacing"]
+        vr = np.polyval(g["vr_poly"], x)
+        ka = 2.0 * g["carrier_frequency"] * vr * vr / (SPEED_OF_LIGHT * r0)
+        fmax = np.polyval(g["dc_poly"], x) + g["prf"] * g["az_bandwidth_fraction"] / 2.0
+        aperture = int(np.round(fmax / ka * g["prf"]))
+        tc = _line_time(g, az)
+        cp, cv = interpolate_orbit(g["osv_t"], g["osv_pos"], g["osv_vel"], tc)
+        ep = geolocate(cp, cv, g["center_point"], r0)
+        astep = aperture // 8
+        ts, ys = [], []
+        for i in range(9):
+            ti = _line_time(g, az + (i - 4) * astep)
+            op, _ = interpolate_orbit(g["osv_t"], g["osv_pos"], g["osv_vel"], ti)
+            ys.append(np.sum((op - ep) ** 2) - r0 * r0)
+            ts.append(np.round((tc - ti) * 1e6) * 1e-6)
+        A = np.stack([np.ones(9), np.asarray(ts) ** 2], axis=1)
+        sol = np.linalg.lstsq(A, np.asarray(ys), rcond=None)[0]
+        xs.append(float(x))
+        vrs.append(float(np.sqrt(sol[1])))
+    return np.polyfit(xs, vrs, 2), np.asarray(vrs), np.asarray(xs)
+
+
+def srgr_fit(g, ground_spacing):
+    """SlantRangeToGroundRangeInterpolation host part (sar/srgr.cu:73-107) on a geometry dict."""
+    p, v = interpolate_orbit(g["osv_t"], g["osv_pos"], g["osv_vel"], _line_time(g, g["azimuth_size"] // 2))
+    first = geolocate(p, v, g["center_point"], g["slant_range_first"])
+    gd, sr = [], []
+    for i in range(100):
+        r = g["slant_range_first"] + (i * (g["range_size"] // 100)) * g["range_spacing"]
+        q = geolocate(p, v, g["center_point"], r)
+        gd.append(np.linalg.norm(q - first))
+        sr.append(r - g["slant_range_first"])
+    return np.polyfit(gd, sr, 4), int(gd[-1] / ground_spacing)
+
+
+WGS84_B_FULL = 6356752.3142451794975639665996337
+_E2 = 2.0 * ((WGS84_A - WGS84_B_FULL) / WGS84_A) - ((WGS84_A - WGS84_B_FULL) / WGS84_A) ** 2
+_EP2 = _E2 / (1.0 - _E2)
+
+
+def geo_to_xyz(lat, lon, alt=0.0):
+    """Geo2xyzWgs84 (util/include/geo_tools.h:52-74)."""
+    la, lo = np.radians(lat), np.radians(lon)
+    n = WGS84_A / np.sqrt(1.0 - _E2 * np.sin(la) ** 2)
+    return np.array([(n + alt) * np.cos(la) * np.cos(lo), (n + alt) * np.cos(la) * np.sin(lo),
+                     (n * (1.0 - _E2) + alt) * np.sin(la)])
+
+
+def xyz_to_geo(xyz):
+    """xyz2geoWGS84 (util/include/geo_tools.h:76-95): Bowring's closed form; (latitude, longitude) degrees."""
+    x, y, z = (float(v) for v in xyz)
+    s = np.hypot(x, y)
+    th = np.arctan(z * WGS84_A / (s * WGS84_B_FULL))
+    lon = np.degrees(np.arctan(y / x))
+    if lon < 0 and y >= 0:
+        lon += 180.0
+    elif lon > 0 and y < 0:
+        lon -= 180.0
+    lat = np.degrees(np.arctan((z + _EP2 * WGS84_B_FULL * np.sin(th) ** 3) / (s - _E2 * WGS84_A * np.cos(th) ** 3)))
+    return float(lat), float(lon)
+
+
+def scene_center(g, guess):
+    """Scene centre and azimuth spacing (main_flow.cc:499-522)."""
+    mid = g["azimuth_size"] // 2
+    r = g["slant_range_first"] + ((g["range_size"] - g["chirp_samples"]) // 2) * g["range_spacing"]
+    p, v = interpolate_orbit(g["osv_t"], g["osv_pos"], g["osv_vel"], _line_time(g, mid))
+    c = geolocate(p, v, guess, r)
+    p1, v1 = interpolate_orbit(g["osv_t"], g["osv_pos"], g["osv_vel"], _line_time(g, mid + 1))
+    c1 = geolocate(p1, v1, c, r)
+    return c, float(np.linalg.norm(c1 - c))
