@@ -82,6 +82,12 @@ class Osv(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("pos", ctypes.c_double * 3), ("vel", ctypes.c_double * 3)]
 
 
+class ResultsWindow(ctypes.Structure):
+    _fields_ = [("lines_before_sensing_start", ctypes.c_long), ("lines_after_sensing_start", ctypes.c_long),
+                ("lines_before_sensing_stop", ctypes.c_long), ("lines_after_sensing_stop", ctypes.c_long),
+                ("near_range_pixels", ctypes.c_int), ("far_range_pixels", ctypes.c_int)]
+
+
 class Geometry(ctypes.Structure):
     _fields_ = [("osv", ctypes.POINTER(Osv)), ("n_osv", ctypes.c_int), ("first_line_time", ctypes.c_double),
                 ("line_time_interval", ctypes.c_double), ("azimuth_size", ctypes.c_int),
@@ -129,6 +135,8 @@ SIGNATURES = {
     "asarf_geo_to_xyz": (_ci, [_cd, _cd, _cd, _P(_cd)]),
     "asarf_xyz_to_geo": (_ci, [_P(_cd), _P(_cd)]),
     "asarf_scene_center": (_ci, [_P(Geometry), _P(_cd), _P(_cd), _P(_cd)]),
+    "asarf_aperture_pixels": (_ci, [_P(Geometry), _P(_ci), _ci, _P(_ci)]),
+    "asarf_results_window": (_ci, [_cd, ctypes.c_long, ctypes.c_long, ctypes.c_long, _ci, _P(ResultsWindow)]),
     "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_image_statistics": (_ci, [_vp, _P(Image), _ci, _P(_cd)]),

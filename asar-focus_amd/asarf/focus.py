@@ -564,3 +564,21 @@ def scene_center(g: dict, guess_xyz):
     L.check(L.lib().asarf_scene_center(ctypes.byref(s), _vec3(guess_xyz), c, ctypes.byref(sp)))
     del keep
     return np.array(c[:]), sp.value
+
+
+def aperture_pixels(g: dict) -> list:
+    """CalcAperturePixels (sar/sar_metadata.h:117-141)."""
+    s, keep = geometry(g)
+    out, n = (ctypes.c_int * 16)(), ctypes.c_int()
+    L.check(L.lib().asarf_aperture_pixels(ctypes.byref(s), out, 16, ctypes.byref(n)))
+    del keep
+    return list(out[:n.value])
+
+
+def results_window(dc_constant: float, lines_before_start: int, lines_after_stop: int, max_aperture: int,
+                   rcmc_window: int = 16) -> dict:
+    """CalcResultsWindow (main_flow.cc:155-201)."""
+    w = L.ResultsWindow()
+    L.check(L.lib().asarf_results_window(float(dc_constant), int(lines_before_start), int(lines_after_stop),
+                                         int(max_aperture), int(rcmc_window), ctypes.byref(w)))
+    return {k: getattr(w, k) for k, _ in L.ResultsWindow._fields_}

@@ -287,3 +287,47 @@ extern "C" int asarf_scene_center(const asarf_geometry* g, const double* guess_x
     std::memcpy(center_xyz, c, sizeof(c));
     return clear_error();
 }
+
+extern "C" int asarf_aperture_pixels(const asarf_geometry* g, int* out, int capacity, int* n_out) {
+    // CalcAperturePixels (sar/sar_metadata.h:117-141): round(fmax / Ka * PRF) at the range indices 0, step, ...
+    // below range_size - chirp_samples (step = that / 10), with Ka = 2 f0 Vr^2 / (c R0) and fmax = fDC + PRF * B / 2
+    if (!g || !out || !n_out) return set_error(ASARF_EINVAL, "asarf_aperture_pixels: null");
+    const int range_end = g->range_size - g->chirp_samples;
+    const int step = range_end / 10;
+    if (step <= 0 || !(g->range_spacing > 0) || !(g->slant_range_first > 0))
+        return set_error(ASARF_EINVAL, "asarf_aperture_pixels: bad image geometry");
+    int n = 0;
+    for (int x = 0; x < range_end; x += step) {
+        if (n == capacity) return set_error(ASARF_EINVAL, "asarf_aperture_pixels: capacity %d too small", capacity);
+        const double r0 = g->slant_range_first + (double)x * g->range_spacing;
+        const double vr = polyval(g->vr_poly, 3, (double)x);
+        const double ka = 2.0 * g->carrier_frequency * vr * vr / (kSpeedOfLight * r0);
+        const double fmax = polyval(g->dc_poly, 3, (double)x) + g->prf * g->az_bandwidth_fraction / 2.0;
+        out[n++] = (int)std::round(fmax / ka * g->prf);
+    }
+    *n_out = n;
+    return clear_error();
+}
+
+extern "C" int asarf_results_window(double dc_constant, long lines_before_start, long lines_after_stop,
+                                    long max_aperture, int rcmc_window, asarf_results_window_t* out) {
+    // CalcResultsWindow (main_flow.cc:155-201): the lines read beyond the requested sensing window are removed;
+    // when they are fewer than the longest aperture, the shortfall is removed inside the window too, at the start
+    // for a positive Doppler centroid constant term and at the stop otherwise; half the RCMC window is removed at
+    // near and at far range
+    if (!out) return set_error(ASARF_EINVAL, "asarf_results_window: null");
+    if (lines_before_start < 0 || lines_after_stop < 0 || max_aperture < 0 || rcmc_window < 0)
+        return set_error(ASARF_EINVAL, "asarf_results_window: negative size");
+    asarf_results_window_t w{};
+    w.lines_before_sensing_start = lines_before_start;
+    w.lines_after_sensing_stop = lines_after_stop;
+    if (dc_constant > 0) {
+        if (lines_before_start < max_aperture) w.lines_after_sensing_start = max_aperture - lines_before_start;
+    } else if (lines_after_stop < max_aperture) {
+        w.lines_before_sensing_stop = max_aperture - lines_after_stop;
+    }
+    w.near_range_pixels = rcmc_window / 2;
+    w.far_range_pixels = rcmc_window / 2;
+    *out = w;
+    return clear_error();
+}

@@ -249,6 +249,23 @@ int asarf_xyz_to_geo(const double* xyz, double* out_lat_lon);
  * field of geometry except vr_poly, dc_poly and center_point. */
 int asarf_scene_center(const asarf_geometry* geometry, const double* guess_xyz, double* out_center_xyz,
                        double* azimuth_spacing);
+/* CalcAperturePixels (sar/sar_metadata.h:117-141): the synthetic aperture in lines at the range indices 0, step, ...
+ * below range_size - chirp_samples (step = that / 10) from vr_poly and dc_poly; *n_out entries (10 or 11) are
+ * written, ASARF_EINVAL when capacity is smaller. */
+int asarf_aperture_pixels(const asarf_geometry* geometry, int* out, int capacity, int* n_out);
+
+/* AzimuthRangeWindow (main_flow.h): lines and range pixels removed from the focused image before output. */
+typedef struct {
+    long lines_before_sensing_start;  /* lines read before the requested start */
+    long lines_after_sensing_start;   /* aperture shortfall removed after the start */
+    long lines_before_sensing_stop;   /* aperture shortfall removed before the stop */
+    long lines_after_sensing_stop;    /* lines read after the requested stop */
+    int near_range_pixels, far_range_pixels;
+} asarf_results_window_t;
+/* CalcResultsWindow (main_flow.cc:155-201). The subset itself is asarf_image_subset_copy with
+ * y0 = lines_before_sensing_start + lines_after_sensing_start, x0 = near_range_pixels. */
+int asarf_results_window(double dc_constant, long lines_before_start, long lines_after_stop, long max_aperture,
+                         int rcmc_window, asarf_results_window_t* out);
 
 /* ------------------------------------------------------------------------------------- output conditioning */
 /* ConditionResultsSLC / CalibrateClampToBigEndian (envisat_format/src/envisat_format_kernels.cu:28-68, 100-111):

@@ -524,3 +524,29 @@ def scene_center(g, guess):
     p1, v1 = interpolate_orbit(g["osv_t"], g["osv_pos"], g["osv_vel"], _line_time(g, mid + 1))
     c1 = geolocate(p1, v1, c, r)
     return c, float(np.linalg.norm(c1 - c))
+
+
+def aperture_pixels(g):
+    """CalcAperturePixels (sar/sar_metadata.h:117-141): aperture in lines at range indices 0, step, ..."""
+    range_end = g["range_size"] - g["chirp_samples"]
+    out = []
+    for x in range(0, range_end, range_end // 10):
+        r0 = g["slant_range_first"] + x * g["range_spacing"]
+        vr = np.polyval(g["vr_poly"], x)
+        ka = 2.0 * g["carrier_frequency"] * vr * vr / (SPEED_OF_LIGHT * r0)
+        fmax = np.polyval(g["dc_poly"], x) + g["prf"] * g["az_bandwidth_fraction"] / 2.0
+        out.append(int(np.round(fmax / ka * g["prf"])))
+    return out
+
+
+def results_window(dc_constant, lines_before_start, lines_after_stop, max_aperture, rcmc_window=16):
+    """CalcResultsWindow (main_flow.cc:155-201)."""
+    w = dict(lines_before_sensing_start=lines_before_start, lines_after_sensing_start=0,
+             lines_before_sensing_stop=0, lines_after_sensing_stop=lines_after_stop,
+             near_range_pixels=rcmc_window // 2, far_range_pixels=rcmc_window // 2)
+    if dc_constant > 0:
+        if lines_before_start < max_aperture:
+            w["lines_after_sensing_start"] = max_aperture - lines_before_start
+    elif lines_after_stop < max_aperture:
+        w["lines_before_sensing_stop"] = max_aperture - lines_after_stop
+    return w

@@ -297,3 +297,54 @@ def test_scene_center_straight_line_closed_form():
     assert abs(spacing - np.linalg.norm(point(vz * 0.001) - point(0.0))) < 1e-6
     rc, rsp = po.scene_center(g, [A * 0.99, 3e5, 0.0])
     assert np.abs(c - rc).max() < 1e-6 and abs(spacing - rsp) < 1e-9
+
+
+# ----------------------------------------------------------------------------- aperture and results window ----
+def test_aperture_pixels_match_the_doppler_history():
+    # straight-line pass at speed V: the Doppler of a target at closest range R0 is f(t) = -(2 / lambda) V^2 t / R(t)
+    # with R(t) = sqrt(R0^2 + V^2 t^2); the aperture is the time (in lines) for |f| to reach fDC + PRF * B / 2,
+    # which to first order in V t / R0 is fmax / Ka
+    g, v = straight_line_scene()
+    speed = np.linalg.norm(v)
+    g = dict(g, vr_poly=[0.0, 0.0, speed], dc_poly=[0.0, 0.0, 95.0])
+    got = focus.aperture_pixels(g)
+    assert got == po.aperture_pixels(g)
+    range_end = g["range_size"] - g["chirp_samples"]
+    xs = list(range(0, range_end, range_end // 10))
+    assert len(got) == len(xs) == 11
+    lam = po.SPEED_OF_LIGHT / g["carrier_frequency"]
+    fmax = 95.0 + g["prf"] * g["az_bandwidth_fraction"] / 2.0
+    for x, n in zip(xs, got):
+        r0 = g["slant_range_first"] + x * g["range_spacing"]
+        # invert |f(t)| = fmax exactly: (2 V^2 t / lambda)^2 = fmax^2 (R0^2 + V^2 t^2)
+        t = fmax * r0 / (speed * math.sqrt((2 * speed / lam) ** 2 - fmax ** 2))
+        assert abs(n / (t * g["prf"]) - 1.0) < 2e-3, (x, n, t * g["prf"])
+
+
+def test_aperture_pixels_capacity_and_geometry_errors():
+    g, _ = straight_line_scene()
+    s, keep = focus.geometry(g)
+    import ctypes
+    out, n = (ctypes.c_int * 4)(), ctypes.c_int()
+    with pytest.raises(L.AsarfError):
+        L.check(L.lib().asarf_aperture_pixels(ctypes.byref(s), out, 4, ctypes.byref(n)))
+    with pytest.raises(L.AsarfError):
+        focus.aperture_pixels(dict(g, range_size=530))  # fewer than 10 range samples after the chirp
+
+
+@pytest.mark.parametrize("dc,before,after,aperture,want_after_start,want_before_stop", [
+    (120.0, 3000, 3000, 700, 0, 0),      # enough lines cached either side
+    (120.0, 500, 3000, 700, 200, 0),     # positive centroid: shortfall at the start
+    (-80.0, 3000, 100, 700, 0, 600),     # negative centroid: shortfall at the stop
+    (0.0, 0, 0, 700, 0, 700),            # zero counts as not positive
+    (1e-9, 0, 0, 700, 700, 0),
+])
+def test_results_window_cases(dc, before, after, aperture, want_after_start, want_before_stop):
+    w = focus.results_window(dc, before, after, aperture, 16)
+    assert w == po.results_window(dc, before, after, aperture, 16)
+    assert w["lines_before_sensing_start"] == before and w["lines_after_sensing_stop"] == after
+    assert w["lines_after_sensing_start"] == want_after_start
+    assert w["lines_before_sensing_stop"] == want_before_stop
+    assert w["near_range_pixels"] == 8 and w["far_range_pixels"] == 8
+    with pytest.raises(L.AsarfError):
+        focus.results_window(dc, -1, after, aperture, 16)
