@@ -82,6 +82,28 @@ class Osv(ctypes.Structure):
     _fields_ = [("t", ctypes.c_double), ("pos", ctypes.c_double * 3), ("vel", ctypes.c_double * 3)]
 
 
+class Mjd(ctypes.Structure):
+    _fields_ = [("days", ctypes.c_int), ("seconds", ctypes.c_uint), ("micros", ctypes.c_uint)]
+
+    def astuple(self):
+        return (self.days, self.seconds, self.micros)
+
+
+class Patc(ctypes.Structure):
+    _fields_ = [("cyclic_counter", ctypes.c_ushort), ("mission", ctypes.c_char * 2), ("orbit_number", ctypes.c_uint),
+                ("epoch_1950", ctypes.c_int), ("milliseconds", ctypes.c_int), ("sbt_counter", ctypes.c_uint),
+                ("sbt_period", ctypes.c_uint)]
+
+
+class PatcReference(ctypes.Structure):
+    _fields_ = [("time_us", ctypes.c_longlong), ("sbt_counter", ctypes.c_uint), ("sbt_period_ns", ctypes.c_uint)]
+
+
+PRODUCT_SAR_IM0, PRODUCT_ASA_IM0, PRODUCT_SAR_IMS, PRODUCT_SAR_IMP, PRODUCT_ASA_IMS, PRODUCT_ASA_IMP, \
+    PRODUCT_UNIDENTIFIED = range(7)
+INSTRUMENT_SAR, INSTRUMENT_ASAR = 0, 1
+
+
 class ResultsWindow(ctypes.Structure):
     _fields_ = [("lines_before_sensing_start", ctypes.c_long), ("lines_after_sensing_start", ctypes.c_long),
                 ("lines_before_sensing_stop", ctypes.c_long), ("lines_after_sensing_stop", ctypes.c_long),
@@ -136,6 +158,15 @@ SIGNATURES = {
     "asarf_xyz_to_geo": (_ci, [_P(_cd), _P(_cd)]),
     "asarf_scene_center": (_ci, [_P(Geometry), _P(_cd), _P(_cd), _P(_cd)]),
     "asarf_aperture_pixels": (_ci, [_P(Geometry), _P(_ci), _ci, _P(_ci)]),
+    "asarf_mjd_compare": (_ci, [_P(Mjd), _P(Mjd)]),
+    "asarf_mjd_add_micros": (_ci, [_P(Mjd), ctypes.c_longlong, _P(Mjd)]),
+    "asarf_counter_gap": (ctypes.c_ulonglong, [ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong]),
+    "asarf_parse_patc": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(Patc)]),
+    "asarf_patc_register": (_ci, [_P(Patc), _P(PatcReference)]),
+    "asarf_adjust_isp_sensing_time": (_ci, [_P(PatcReference), _P(Mjd), ctypes.c_uint, _ci]),
+    "asarf_product_type_from_name": (_ci, [ctypes.c_char_p]),
+    "asarf_target_product": (_ci, [_ci, ctypes.c_char_p, _P(_ci)]),
+    "asarf_instrument": (_ci, [_ci, _P(_ci)]),
     "asarf_results_window": (_ci, [_cd, ctypes.c_long, ctypes.c_long, ctypes.c_long, _ci, _P(ResultsWindow)]),
     "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),

@@ -1,0 +1,180 @@
+// Level-0 host utilities (the reference's envisat_format helpers that its unit tests pin): MJD time arithmetic,
+// packet-counter gaps, the ERS satellite-binary-time (SBT) correction from a PATC record, and product-type
+// identification. Host-only integer work; the packets they describe are decoded on the device by k_ingest.
+#include <cctype>
+#include <cstdint>
+#include <cstring>
+
+#include "asarf.h"
+#include "internal.h"
+
+using namespace asarf;
+
+namespace {
+
+constexpr int64_t kMicrosPerDay = 86400LL * 1000000LL;
+// days from 1950-01-01 to 2000-01-01 (the MJD2000 epoch): 50 years with the 12 leap days 1952 ... 1996
+constexpr int64_t kDays1950To2000 = 18262;
+// days from 1950-01-01 to 1991-01-01; a PATC epoch before that is rejected (ers_sbt.cc:48-54)
+constexpr int32_t kMinPatcDays1950 = 14975;
+
+int64_t mjd_to_micros(const asarf_mjd& m) {
+    return (int64_t)m.days * kMicrosPerDay + (int64_t)m.seconds * 1000000LL + (int64_t)m.micros;
+}
+
+// PtimeToMjd (envisat_utils.h:50-59): days = floor(t / day) from 2000-01-01, then whole seconds of the day and the
+// microseconds within the second
+asarf_mjd micros_to_mjd(int64_t us) {
+    int64_t days = us / kMicrosPerDay;
+    int64_t rem = us % kMicrosPerDay;
+    if (rem < 0) {
+        rem += kMicrosPerDay;
+        --days;
+    }
+    asarf_mjd r;
+    r.days = (int32_t)days;
+    r.seconds = (uint32_t)(rem / 1000000LL);
+    r.micros = (uint32_t)(rem % 1000000LL);
+    return r;
+}
+
+// std::stoul on a fixed-width ASCII field: optional leading blanks, then at least one decimal digit
+bool ascii_field(const unsigned char* p, int n, uint32_t* out) {
+    int i = 0;
+    while (i < n && std::isspace(p[i])) ++i;
+    if (i == n || !std::isdigit(p[i])) return false;
+    uint64_t v = 0;
+    for (; i < n && std::isdigit(p[i]); ++i) v = v * 10 + (uint64_t)(p[i] - '0');
+    *out = (uint32_t)v;
+    return true;
+}
+
+bool iequals(const char* a, const char* b) {
+    for (; *a && *b; ++a, ++b)
+        if (std::tolower((unsigned char)*a) != std::tolower((unsigned char)*b)) return false;
+    return *a == *b;
+}
+
+}  // namespace
+
+extern "C" int asarf_mjd_compare(const asarf_mjd* a, const asarf_mjd* b) {
+    // mjd ordering (envisat_types.h:42-73): lexicographic on (days, seconds, micros)
+    if (a->days != b->days) return a->days < b->days ? -1 : 1;
+    if (a->seconds != b->seconds) return a->seconds < b->seconds ? -1 : 1;
+    if (a->micros != b->micros) return a->micros < b->micros ? -1 : 1;
+    return 0;
+}
+
+extern "C" int asarf_mjd_add_micros(const asarf_mjd* in, long long micros, asarf_mjd* out) {
+    // MjdAddMicros (envisat_utils.h:74-78)
+    if (!in || !out) return set_error(ASARF_EINVAL, "asarf_mjd_add_micros: null");
+    *out = micros_to_mjd(mjd_to_micros(*in) + (int64_t)micros);
+    return clear_error();
+}
+
+extern "C" unsigned long long asarf_counter_gap(unsigned long long last, unsigned long long current,
+                                                unsigned long long limit) {
+    // CounterGap (parse_util.h:33-45): steps from last to current of a counter that wraps after limit
+    // (limit = 2^bits - 1 for a bits-wide field)
+    if (last + 1 == current) return 1;
+    if (current < last) return limit - last + current + 1;
+    return current - last;
+}
+
+extern "C" int asarf_parse_patc(const unsigned char* bytes, long n_bytes, asarf_patc* out) {
+    // ParsePatc / ConstructPatc (patc.cc:25-57): a 54-byte record; ASCII cyclic counter at 15 (4 characters),
+    // mission at 20-21, ASCII orbit number at 30 (5 characters), then little-endian int32 days since 1950 at 38,
+    // int32 milliseconds at 42, uint32 SBT counter at 46 and uint32 SBT period [ns] at 50
+    if (!bytes || !out) return set_error(ASARF_EINVAL, "asarf_parse_patc: null");
+    if (n_bytes != 54) return set_error(ASARF_EINVAL, "asarf_parse_patc: record is %ld bytes, not 54", n_bytes);
+    asarf_patc p{};
+    uint32_t cc = 0, orbit = 0;
+    if (!ascii_field(bytes + 15, 4, &cc) || !ascii_field(bytes + 30, 5, &orbit))
+        return set_error(ASARF_EINVAL, "asarf_parse_patc: counter or orbit field is not a number");
+    p.cyclic_counter = (uint16_t)cc;
+    p.mission[0] = (char)bytes[20];
+    p.mission[1] = (char)bytes[21];
+    p.orbit_number = orbit;
+    auto le32 = [&](int at) {
+        return (uint32_t)bytes[at] | ((uint32_t)bytes[at + 1] << 8) | ((uint32_t)bytes[at + 2] << 16) |
+               ((uint32_t)bytes[at + 3] << 24);
+    };
+    p.epoch_1950 = (int32_t)le32(38);
+    p.milliseconds = (int32_t)le32(42);
+    p.sbt_counter = le32(46);
+    p.sbt_period = le32(50);
+    *out = p;
+    return clear_error();
+}
+
+extern "C" int asarf_patc_register(const asarf_patc* patc, asarf_patc_reference* out) {
+    // RegisterPatc (ers_sbt.cc:40-63): validates the PATC epoch and keeps its time point (days since 1950 plus
+    // milliseconds) with the SBT counter and period. The reference keeps this in a process-wide variable; here the
+    // caller holds it.
+    if (!patc || !out) return set_error(ASARF_EINVAL, "asarf_patc_register: null");
+    if (patc->epoch_1950 < 0) return set_error(ASARF_EINVAL, "asarf_patc_register: negative epoch days since 1950");
+    if (patc->milliseconds < 0) return set_error(ASARF_EINVAL, "asarf_patc_register: negative epoch milliseconds");
+    if (patc->epoch_1950 < kMinPatcDays1950)
+        return set_error(ASARF_EINVAL, "asarf_patc_register: epoch %d days after 1950-01-01 is before 1991",
+                         patc->epoch_1950);
+    out->time_us = ((int64_t)patc->epoch_1950 - kDays1950To2000) * kMicrosPerDay + (int64_t)patc->milliseconds * 1000;
+    out->sbt_counter = patc->sbt_counter;
+    out->sbt_period_ns = patc->sbt_period;
+    return clear_error();
+}
+
+extern "C" int asarf_adjust_isp_sensing_time(const asarf_patc_reference* ref, asarf_mjd* isp_time, unsigned sbt,
+                                             int overflow) {
+    // AdjustIspSensingTime (ers_sbt.cc:65-86): without a registered PATC the time is left as it is; otherwise it
+    // becomes the PATC time plus (SBT ticks since the PATC counter) x period, in whole microseconds truncated
+    // toward zero. A counter below the PATC's counts backwards unless overflow is set, in which case it is taken
+    // as wrapped past 2^32 - 1
+    if (!isp_time) return set_error(ASARF_EINVAL, "asarf_adjust_isp_sensing_time: null");
+    if (!ref) return clear_error();
+    int64_t delta;
+    if (sbt < ref->sbt_counter && !overflow)
+        delta = (int64_t)sbt - (int64_t)ref->sbt_counter;
+    else
+        delta = (int64_t)asarf_counter_gap(ref->sbt_counter, sbt, 0xFFFFFFFFull);
+    const int64_t ns = (int64_t)ref->sbt_period_ns * delta;
+    *isp_time = micros_to_mjd(ref->time_us + ns / 1000);
+    return clear_error();
+}
+
+extern "C" int asarf_product_type_from_name(const char* product_name) {
+    // GetProductTypeFrom (asar_constants.cc:58-66): by the name's prefix
+    if (!product_name) return ASARF_PRODUCT_UNIDENTIFIED;
+    if (std::strncmp(product_name, "ASA_IM__0", 9) == 0) return ASARF_PRODUCT_ASA_IM0;
+    if (std::strncmp(product_name, "SAR_IM__0", 9) == 0) return ASARF_PRODUCT_SAR_IM0;
+    return ASARF_PRODUCT_UNIDENTIFIED;
+}
+
+extern "C" int asarf_target_product(int in_product, const char* target, int* out_product) {
+    // TryDetermineTargetProductFrom (asar_constants.cc:68-90): "IMS" or "IMP", any case, from an IM level-0 input
+    if (!target || !out_product) return set_error(ASARF_EINVAL, "asarf_target_product: null");
+    const bool ims = iequals(target, "IMS"), imp = iequals(target, "IMP");
+    if (in_product == ASARF_PRODUCT_SAR_IM0 && (ims || imp)) {
+        *out_product = ims ? ASARF_PRODUCT_SAR_IMS : ASARF_PRODUCT_SAR_IMP;
+        return clear_error();
+    }
+    if (in_product == ASARF_PRODUCT_ASA_IM0 && (ims || imp)) {
+        *out_product = ims ? ASARF_PRODUCT_ASA_IMS : ASARF_PRODUCT_ASA_IMP;
+        return clear_error();
+    }
+    return set_error(ASARF_EINVAL, "asarf_target_product: focussing '%s' from this input is not supported", target);
+}
+
+extern "C" int asarf_instrument(int product, int* out_instrument) {
+    // GetInstrumentFrom (asar_constants.cc:92-102): SAR for SAR_IM0 / SAR_IMS, ASAR for ASA_IM0 / ASA_IMS, an error
+    // for every other type (the detected products included, as the reference)
+    if (!out_instrument) return set_error(ASARF_EINVAL, "asarf_instrument: null");
+    if (product == ASARF_PRODUCT_SAR_IM0 || product == ASARF_PRODUCT_SAR_IMS) {
+        *out_instrument = ASARF_INSTRUMENT_SAR;
+        return clear_error();
+    }
+    if (product == ASARF_PRODUCT_ASA_IM0 || product == ASARF_PRODUCT_ASA_IMS) {
+        *out_instrument = ASARF_INSTRUMENT_ASAR;
+        return clear_error();
+    }
+    return set_error(ASARF_EINVAL, "asarf_instrument: unsupported product type %d", product);
+}

@@ -421,6 +421,64 @@ int asarf_plan_rcmc_pairs(asarf_plan* plan, const void* raw, long n_pairs, unsig
  * 16-tap sinc everywhere, and the plan runs that kernel without the per-pixel table lookups. */
 int asarf_plan_set_rcmc(asarf_plan* plan, int interpolator, const asarf_rcmc_net* net);
 
+/* --------------------------------------------------------------------------- level-0 host utilities (host) */
+/* mjd (envisat_types.h:37-75): days since 2000-01-01 (negative before), whole seconds of the day, microseconds. */
+typedef struct {
+    int days;
+    unsigned seconds;
+    unsigned micros;
+} asarf_mjd;
+/* mjd ordering (envisat_types.h:42-73): -1, 0 or 1 as a is before, equal to or after b. */
+int asarf_mjd_compare(const asarf_mjd* a, const asarf_mjd* b);
+/* MjdAddMicros (envisat_utils.h:74-78); micros may be negative. */
+int asarf_mjd_add_micros(const asarf_mjd* in, long long micros, asarf_mjd* out);
+/* CounterGap (parse_util.h:33-45): steps from last to current of a counter that wraps after limit
+ * (MaxValueForBits: limit = 2^bits - 1). */
+unsigned long long asarf_counter_gap(unsigned long long last, unsigned long long current, unsigned long long limit);
+
+/* Patc (patc.h): the ERS time-correlation record. */
+typedef struct {
+    unsigned short cyclic_counter;
+    char mission[2];
+    unsigned orbit_number;
+    int epoch_1950;      /* days since 1950-01-01 */
+    int milliseconds;    /* of that day */
+    unsigned sbt_counter;
+    unsigned sbt_period; /* ns per SBT tick */
+} asarf_patc;
+/* ParsePatc (patc.cc:25-57) from the 54 bytes of a PATC file. */
+int asarf_parse_patc(const unsigned char* bytes, long n_bytes, asarf_patc* out);
+/* A registered PATC: its time point in microseconds since 2000-01-01, SBT counter and period. */
+typedef struct {
+    long long time_us;
+    unsigned sbt_counter;
+    unsigned sbt_period_ns;
+} asarf_patc_reference;
+/* RegisterPatc (ers_sbt.cc:40-63): validates (no negative fields, not before 1991) into a reference the caller
+ * keeps (the reference holds it in a process-wide variable). */
+int asarf_patc_register(const asarf_patc* patc, asarf_patc_reference* out);
+/* AdjustIspSensingTime (ers_sbt.cc:65-86): ISP time from the SBT counter; ref == NULL leaves it unchanged (no PATC
+ * registered). */
+int asarf_adjust_isp_sensing_time(const asarf_patc_reference* ref, asarf_mjd* isp_time, unsigned sbt, int overflow);
+
+/* ProductTypes / Instrument (asar_constants.h:33-35), same order. */
+enum {
+    ASARF_PRODUCT_SAR_IM0 = 0,
+    ASARF_PRODUCT_ASA_IM0 = 1,
+    ASARF_PRODUCT_SAR_IMS = 2,
+    ASARF_PRODUCT_SAR_IMP = 3,
+    ASARF_PRODUCT_ASA_IMS = 4,
+    ASARF_PRODUCT_ASA_IMP = 5,
+    ASARF_PRODUCT_UNIDENTIFIED = 6
+};
+enum { ASARF_INSTRUMENT_SAR = 0, ASARF_INSTRUMENT_ASAR = 1 };
+/* GetProductTypeFrom (asar_constants.cc:58-66): by the product name's prefix. */
+int asarf_product_type_from_name(const char* product_name);
+/* TryDetermineTargetProductFrom (asar_constants.cc:68-90): target "IMS" / "IMP" (any case); ASARF_EINVAL otherwise. */
+int asarf_target_product(int in_product, const char* target, int* out_product);
+/* GetInstrumentFrom (asar_constants.cc:92-102); ASARF_EINVAL for detected and unidentified products. */
+int asarf_instrument(int product, int* out_instrument);
+
 #ifdef __cplusplus
 }
 #endif
