@@ -77,8 +77,17 @@ __global__ __launch_bounds__(kIngestThreads) void k_ingest(IngestArgs a) {
                 const uint8_t* rec = bytes + (long)y * a.line_bytes;
                 len = (int)rec[0] | ((int)rec[1] << 8);
             }
-            if (x < a.x_size) v0 = sample(x, y, shift, len);
-            if (x + 1 < a.x_size) v1 = sample(x + 1, y, shift, len);
+            const int s0 = x - shift;
+            if (FMT == 0 && a.w32 && x + 1 < a.x_size && s0 >= 0 && s0 + 1 < a.samples_per_line) {
+                // both samples inside the line: one aligned 4-byte load (I0, Q0, I1, Q1). x is even and the ERS
+                // shift a multiple of 4, so 2 s0 is a multiple of 4; w32 says line pitch and base are too
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(bytes + (long)y * a.line_bytes + 2 * s0);
+                v0 = make_float2((float)(w & 0xFF), (float)((w >> 8) & 0xFF));
+                v1 = make_float2((float)((w >> 16) & 0xFF), (float)(w >> 24));
+            } else {
+                if (x < a.x_size) v0 = sample(x, y, shift, len);
+                if (x + 1 < a.x_size) v1 = sample(x + 1, y, shift, len);
+            }
         }
         float2* row = out + (long)y * a.x_stride;
         if (pair && a.vec16) {
@@ -103,6 +112,9 @@ hipError_t launch_ingest(const IngestArgs& args, int kind, hipStream_t s) {
     IngestArgs a = args;
     // 16-byte stores of column pairs when every pair is 16-byte aligned
     a.vec16 = (a.x_stride % 2 == 0) && (reinterpret_cast<uintptr_t>(a.out) & 15) == 0;
+    // 4-byte ERS sample-pair loads when every lane's pair is 4-byte aligned: the shift step is even (4 for ERS),
+    // the line pitch and the base are multiples of 4
+    a.w32 = (a.swst_mult % 2 == 0) && (a.line_bytes % 4 == 0) && (reinterpret_cast<uintptr_t>(a.bytes) & 3) == 0;
     const dim3 grid((a.x_stride + 2 * kIngestThreads - 1) / (2 * kIngestThreads),
                     (a.y_stride + kIngestRows - 1) / kIngestRows);
     if (kind == 0) {

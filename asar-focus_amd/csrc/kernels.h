@@ -229,6 +229,7 @@ struct IngestArgs {
     float2* out;
     int x_size, x_stride, y_stride;  // logical width, row pitch and padded height of the image
     int vec16;                // set by the launcher: column pairs stored with 16-byte stores
+    int w32;                  // set by the launcher: ERS sample pairs read with aligned 4-byte loads
 };
 
 // Output conditioning into MDS records (k_output.hip).

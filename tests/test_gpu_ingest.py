@@ -25,11 +25,13 @@ def ers_lines(rng, lines, spl, line_bytes):
     return samples, swst
 
 
-@pytest.mark.parametrize("lines,spl,range_size,x_stride,y_stride",
-                         [(37, 300, 340, 352, 48), (200, 5616, 5616 + 4 * 66, 6400, 210)])
-def test_ers_decode_matches_oracle(ctx, lines, spl, range_size, x_stride, y_stride):
-    rng = np.random.default_rng(lines)
-    samples, swst = ers_lines(rng, lines, spl, line_bytes=2 * spl + 6)
+# line pitch 2 spl + pad: a multiple of 4 takes the 4-byte sample-pair loads (odd spl: the last pair is split)
+@pytest.mark.parametrize("lines,spl,range_size,x_stride,y_stride,pad",
+                         [(37, 300, 340, 352, 48, 6), (37, 300, 340, 352, 48, 8), (37, 301, 341, 352, 48, 6),
+                          (200, 5616, 5616 + 4 * 66, 6400, 210, 6), (200, 5616, 5616 + 4 * 66, 6400, 210, 8)])
+def test_ers_decode_matches_oracle(ctx, lines, spl, range_size, x_stride, y_stride, pad):
+    rng = np.random.default_rng(lines + pad)
+    samples, swst = ers_lines(rng, lines, spl, line_bytes=2 * spl + pad)
     min_swst = int(swst.min())
     out = torch.full((y_stride, x_stride), 7 + 7j, dtype=torch.complex64, device="cuda")
     focus.ingest_ers_im(ctx, torch.from_numpy(samples).cuda(), spl, torch.from_numpy(swst.view(np.int16)).cuda(),
