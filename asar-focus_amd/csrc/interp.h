@@ -28,6 +28,15 @@ __device__ __forceinline__ float2 sinc16_interp(const float2* tap, const float (
     return make_float2(acc.x * inv, acc.y * inv);
 }
 
+// Azimuth band test of the matched filter (sar/range_doppler_algorithm.cu:238-246): a bin is dropped when its
+// aliased frequency fn lies more than beta * prf / 2 from the bin-quantised centroid dc_off. Both are whole
+// multiples of the bin spacing, so the test runs on bin offsets, |rint(fn / step) - rint(dc_off / step)| against
+// half_bins = beta * nbins / 2: a bin exactly on the edge is kept (the reference's strict inequality in exact
+// arithmetic) however the edge would round in Hz, and every kernel decides every bin the same way.
+__device__ __forceinline__ bool outside_band(double fn, double m_dc, double inv_step, double half_bins) {
+    return fabs(rint(fn * inv_step) - m_dc) > half_bins + 1e-9;
+}
+
 // sin(2 pi x) and cos(2 pi x) for |x| <= 0.5 from the hardware sine/cosine, whose argument is in turns
 // (measured on gfx950 against FP64 over [-0.5, 0.5]: max abs error 1.3e-7).
 __device__ __forceinline__ void sincos_turns(float x, float* s, float* c) {

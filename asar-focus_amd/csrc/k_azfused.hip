@@ -88,8 +88,9 @@ __global__ __launch_bounds__(kThreads) void k_az_fused(AzFusedArgs a) {
     float ws[16];  // signed window (-1)^i w[i]
 #pragma unroll
     for (int i = 0; i < 16; ++i) ws[i] = (i & 1) ? -a.window[i] : a.window[i];
-    const double band_hi = half * a.beta + cc.dc_off;
-    const double band_lo = -half * a.beta + cc.dc_off;
+    const double inv_step = (double)a.nbins / a.prf;
+    const double half_bins = 0.5 * a.beta * (double)a.nbins;
+    const double m_dc = rint(cc.dc_off * inv_step);
     auto rcmc_load = [&](int m) -> float2 {
         const int k = k1 + a.n1 * m;
         const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(kThreads) void k_az_fused(AzFusedArgs a) {
             const float d = (float)(shift - p);
             val = (d == 0.0f) ? tap[8] : sinc16_interp(tap, ws, d);
         }
-        if (fn < band_lo || fn > band_hi) return make_float2(0.f, 0.f);
+        if (outside_band(fn, m_dc, inv_step, half_bins)) return make_float2(0.f, 0.f);
         double fc = fb - cc.dc_off;
         if (fc > half)
             fc -= a.prf;

@@ -122,6 +122,8 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(7)
     if (a.tile_gate && *a.tile_gate <= kRcmcTileReach) return;
     const int nrv = a.nrv;
     const double step = a.prf / (double)a.nbins;
+    const double inv_step = (double)a.nbins / a.prf;
+    const double half_bins = 0.5 * a.beta * (double)a.nbins;
     const double half = 0.5 * a.prf;
     float w[16];  // signed window (-1)^i w[i]
 #pragma unroll
@@ -138,8 +140,7 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(7)
             const ColConst cc = a.col[x];
             const double fn = alias_about_dc(fb, cc.dc, a.prf);
             // outside the azimuth band the matched filter zeroes the pixel: skip its interpolation
-            const bool out_band =
-                a.mode != 1 && (fn < -half * a.beta + cc.dc_off || fn > half * a.beta + cc.dc_off);
+            const bool out_band = a.mode != 1 && outside_band(fn, rint(cc.dc_off * inv_step), inv_step, half_bins);
             float2 val = make_float2(0.f, 0.f);
             if (out_band) {
             } else if (a.mode != 2) {
@@ -205,9 +206,10 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
     const bool xok = x < nrv;
     const ColConst cc = a.col[xok ? x : nrv - 1];
     const double step = a.prf / (double)a.nbins;
+    const double inv_step = (double)a.nbins / a.prf;
+    const double half_bins = 0.5 * a.beta * (double)a.nbins;
+    const double m_dc = rint(cc.dc_off * inv_step);  // the column's quantised centroid in bins
     const double half = 0.5 * a.prf;
-    const double band_hi = half * a.beta + cc.dc_off;
-    const double band_lo = -half * a.beta + cc.dc_off;
     float w[16];  // signed window (-1)^i w[i]
 #pragma unroll
     for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
         const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
         const double fn = alias_about_dc(fb, cc.dc, a.prf);
         // outside the azimuth band the matched filter zeroes the pixel: skip its interpolation
-        const bool out_band = a.mode != 1 && (fn < band_lo || fn > band_hi);
+        const bool out_band = a.mode != 1 && outside_band(fn, m_dc, inv_step, half_bins);
         float2 val = make_float2(0.f, 0.f);
         if (out_band) {
         } else if (a.mode != 2) {

@@ -385,9 +385,10 @@ void or_az_matched_filter(or_zc* img, int xs, int nbins, int xstr, const or_rda_
             const double dc_off = round(dc / step) * step;
             const double fb = bin_frequency(y, nbins, step);
             const double fa = alias_about_dc(fb, dc, p->prf);
-            const double hi = half * p->az_bandwidth_fraction + dc_off;
-            const double lo = -half * p->az_bandwidth_fraction + dc_off;
-            if (fa < lo || fa > hi) {
+            /* the band test on bin offsets: fa and dc_off are whole multiples of step, so a bin exactly on
+             * the edge is kept (the reference's strict inequality in exact arithmetic) however the edge rounds */
+            const double jb = rint(fa / step) - rint(dc_off / step);
+            if (fabs(jb) > 0.5 * p->az_bandwidth_fraction * (double)nbins + 1e-9) {
                 row[x] = 0.0;
                 continue;
             }

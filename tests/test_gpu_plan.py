@@ -169,3 +169,17 @@ def test_plan_statistics_load_width(ctx, nr, offset):
         assert abs(res["iq"][k] - v) <= 1e-9 * max(1.0, abs(v)), k
     xs = np.arange(0, cfg.nr - cfg.nc)
     assert np.abs(np.polyval(res["dc_poly"], xs) - np.polyval(rres["dc_poly"], xs)).max() < 1e-6
+
+
+@pytest.mark.parametrize("na", [1100, 1120, 2500])
+def test_plan_band_edge_sizes(ctx, na):
+    """Azimuth lengths (1120, 1125, 2520) for which 0.8 x nbins / 2 is an integer put bins exactly on the band
+    limits of the matched filter: the column-tiled RCMC kernel (which skips out-of-band pixels) and the oracle
+    must keep the same bins."""
+    cfg = configs.ers_block().scaled(na=na, nr=900, n_targets_az=1, n_targets_rg=1)
+    raw, slc, res, _ = run_plan(ctx, cfg)
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    err = rel_rms(slc, ref)
+    print(f"na={na}: slc rel rms {err:.3e}, shift bound {res['rcmc_shift_bound']}")
+    assert res["rcmc_shift_bound"] <= 24  # the tiled kernel ran
+    assert err < SLC_TOL

@@ -275,6 +275,21 @@ def test_matched_filter_is_unit_modulus_inside_band():
     assert abs(inside.sum() - 0.8 * nbins) <= 2
 
 
+@pytest.mark.parametrize("nbins,beta,dc", [(1120, 0.8, -300.0), (1120, 0.8, 211.7), (1125, 0.8, 35.0),
+                                           (1008, 0.8, -300.0), (2520, 0.8, 640.0), (1000, 0.7, 0.0)])
+def test_matched_filter_band_edges(nbins, beta, dc):
+    # bins are whole multiples of prf / nbins and the kept band is centred on the bin-quantised centroid, so
+    # exactly 2 floor(beta nbins / 2) + 1 bins survive in every column: when beta nbins / 2 is an integer the
+    # two edge bins lie exactly on the band limits and are kept
+    nrv = 8
+    f = np.ones((nbins, nrv), np.complex128)
+    p = rcmc_params(prf=1679.9, dc=dc)
+    p.az_bandwidth_fraction = beta
+    po.az_matched_filter(f, nrv, p)
+    kept = (np.abs(f) > 0.5).sum(axis=0)
+    assert np.all(kept == 2 * int(np.floor(beta * nbins / 2 + 1e-9)) + 1), kept
+
+
 # ------------------------------------------------------------------------------------------ whole path ----
 @pytest.mark.parametrize("dc", [0.0, 150.0, -150.0])
 def test_point_target_focuses_at_zero_doppler_position(dc):
