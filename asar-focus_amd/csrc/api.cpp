@@ -654,6 +654,21 @@ int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* i
     return ASARF_OK;
 }
 
+// Out-of-place inverse pass of a four-step split (n2 > 1): pass 0 = B' (in -> out, inverse inter-pass twiddle),
+// pass 1 = A' (in -> out, scale, rows < out_rows written).
+int col_inverse_pass_to(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, float2* out,
+                        long out_stride, int ncols, int out_rows, float scale) {
+    const int n = cs.n;
+    if (cs.n2 == 1) return set_error(ASARF_EINVAL, "col_inverse_pass_to: no four-step split");
+    const ColFftArgs a =
+        pass == 0 ? col_args(in, in_stride, out, out_stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, cs.full->d_tw,
+                             1.f, cs.p2)
+                  : col_args(in, in_stride, out, out_stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows, nullptr,
+                             scale, cs.p1);
+    HIP_CHECK(launch_col_fft(a, true, c->stream));
+    return ASARF_OK;
+}
+
 // Inverse column transform from permuted storage; writes rows < out_rows scaled by `scale`.
 // pass 0 = B' (contiguous, inverse inter-pass twiddle), pass 1 = A' (strided, scale and row limit).
 int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,
@@ -717,7 +732,10 @@ extern "C" int asarf_range_doppler(asarf_ctx* c, const asarf_rda_params* p, asar
     ST_CHECK(ctx_scratch(c, sizeof(ColConst) * (size_t)nrv + 64, &scratch));
     auto* col = static_cast<ColConst*>(scratch);
     auto* d_dc = reinterpret_cast<double*>(static_cast<char*>(scratch) + sizeof(ColConst) * (size_t)nrv);
-    HIP_CHECK(hipMemcpyAsync(d_dc, p->dc_poly, sizeof(double) * 3, hipMemcpyHostToDevice, c->stream));
+    // the Doppler polynomial and a zeroed slot for the device-side RCMC shift bound, in one copy
+    const double h_dc[4] = {p->dc_poly[0], p->dc_poly[1], p->dc_poly[2], 0.0};
+    int* d_reach = reinterpret_cast<int*>(d_dc + 3);
+    HIP_CHECK(hipMemcpyAsync(d_dc, h_dc, sizeof(h_dc), hipMemcpyHostToDevice, c->stream));
     ColConstArgs ca{};
     ca.col = col;
     ca.nrv = nrv;
@@ -728,10 +746,45 @@ extern "C" int asarf_range_doppler(asarf_ctx* c, const asarf_rda_params* p, asar
     ca.r_first = p->slant_range_first;
     ca.prf = p->prf;
     ca.nbins = nap;
+    ca.reach = d_reach;
     HIP_CHECK(launch_col_const(ca, c->stream));
     auto* s = static_cast<float2*>(src->data);
     auto* o = static_cast<float2*>(out->data);
     const long so = out->x_stride;
+    if (cs.n2 > 1 && src->x_stride == out->x_stride && s != o) {
+        // src is consumed (the reference trades it away as the new workspace), so it serves as the second buffer
+        // and every pass runs out of place, as in the fused plan: A s -> o, B o -> s, RCMC + matched filter
+        // s -> o (column-tiled kernel when the device-side shift bound fits its halo, else the row kernel; both
+        // are enqueued and the bound makes exactly one of them work), B' o -> s, A' s -> o
+        ST_CHECK(col_forward_pass(c, cs, 0, s, so, na, o, so, nrv));
+        ST_CHECK(col_forward_pass(c, cs, 1, o, so, nap, s, so, nrv));
+        RcmcArgs rt{};
+        rt.img = s;
+        rt.out = o;
+        rt.stride = so;
+        rt.nrv = nrv;
+        rt.nbins = nap;
+        rt.n1 = cs.n1;
+        rt.n2 = cs.n2;
+        rt.col = col;
+        rt.prf = p->prf;
+        rt.beta = p->az_bandwidth_fraction;
+        rt.mode = 0;
+        rt.tile_gate = d_reach;
+        HIP_CHECK(launch_rcmc_tile(rt, c->stream));
+        HIP_CHECK(launch_rcmc_mf(rt, c->stream));
+        ST_CHECK(col_inverse_pass_to(c, cs, 0, o, so, s, so, nrv, na, 1.f));
+        ST_CHECK(col_inverse_pass_to(c, cs, 1, s, so, o, so, nrv, na, 1.0f / (float)nap));
+        HIP_CHECK(launch_zero_cols(o, so, out->y_stride, nrv, out->x_stride, c->stream));
+        out->x_size = nrv;
+        out->y_size = na;
+        if (adopt) {
+            ws->data = src->data;
+            ws->bytes = asarf_image_bytes(src);
+            *src = asarf_image{nullptr, 0, 0, 0, 0, ASARF_CF32};
+        }
+        return clear_error();
+    }
     ST_CHECK(col_forward_pass(c, cs, 0, s, src->x_stride, na, o, so, nrv));
     ST_CHECK(col_forward_pass(c, cs, 1, o, so, nap, o, so, nrv));
     RcmcArgs ra{};

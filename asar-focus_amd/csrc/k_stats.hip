@@ -16,28 +16,45 @@
 namespace asarf {
 
 // ---------------------------------------------------------------------------------- cf32 moments ----
-// partials[wg][12]: logical {n, SI, SQ, SII, SQQ, SIQ}, padding {n, SI, SQ, SII, SQQ, SIQ}
+// partials[wg][12]: logical {n, SI, SQ, SII, SQQ, SIQ}, padding {n, SI, SQ, SII, SQQ, SIQ}. A sample with a NaN in
+// either component is skipped whole. The logical columns [0, xs) and the padding columns [xs, xstr) run as two
+// loops with their own accumulators (a runtime index into one accumulator array would put it in scratch memory),
+// each with four loads in flight per lane.
+struct Mom6 {
+    double n = 0.0, si = 0.0, sq = 0.0, sii = 0.0, sqq = 0.0, siq = 0.0;
+    __device__ __forceinline__ void add(float2 v) {
+        const bool ok = !(isnan(v.x) || isnan(v.y));
+        const double i_ = ok ? (double)v.x : 0.0, q_ = ok ? (double)v.y : 0.0;
+        n += ok ? 1.0 : 0.0;
+        si += i_;
+        sq += q_;
+        sii = fma(i_, i_, sii);
+        sqq = fma(q_, q_, sqq);
+        siq = fma(i_, q_, siq);
+    }
+    __device__ __forceinline__ void span(const float2* row, int x0, int x1) {
+        int x = x0 + (int)threadIdx.x;
+        for (; x + 3 * kThreads < x1; x += 4 * kThreads) {
+            const float2 v0 = row[x], v1 = row[x + kThreads], v2 = row[x + 2 * kThreads], v3 = row[x + 3 * kThreads];
+            add(v0);
+            add(v1);
+            add(v2);
+            add(v3);
+        }
+        for (; x < x1; x += kThreads) add(row[x]);
+    }
+};
+
 __global__ __launch_bounds__(kThreads) void k_moments_cf32(const float2* img, long stride, int xs, int ys,
                                                           int xstr, double* partials) {
-    double acc[12];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) acc[i] = 0.0;
+    Mom6 lg, pd;
     for (int y = blockIdx.x; y < ys; y += gridDim.x) {
         const float2* row = img + (long)y * stride;
-        for (int x = threadIdx.x; x < xstr; x += kThreads) {
-            const float2 v = row[x];
-            if (isnan(v.x) || isnan(v.y)) continue;
-            const int o = (x < xs) ? 0 : 6;
-            const double i_ = v.x, q_ = v.y;
-            acc[o + 0] += 1.0;
-            acc[o + 1] += i_;
-            acc[o + 2] += q_;
-            acc[o + 3] += i_ * i_;
-            acc[o + 4] += q_ * q_;
-            acc[o + 5] += i_ * q_;
-        }
+        lg.span(row, 0, xs);
+        pd.span(row, xs, xstr);
     }
     __shared__ double sh[12][kThreads];
+    const double acc[12] = {lg.n, lg.si, lg.sq, lg.sii, lg.sqq, lg.siq, pd.n, pd.si, pd.sq, pd.sii, pd.sqq, pd.siq};
 #pragma unroll
     for (int i = 0; i < 12; ++i) sh[i][threadIdx.x] = acc[i];
     __syncthreads();

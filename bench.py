@@ -131,6 +131,54 @@ def ingest_probe(reps=20):
     return {"ingest_probe": res, "lines": lines, "samples_per_line": spl, "padded_image": [x_stride, y_stride]}
 
 
+def stages_probe(reps=5):
+    """The reference's call sequence (main.cc:243-283) through the stage entry points at the ASAR scene size:
+    RawDataCorrection, CalculateDopplerCentroid, RangeCompression and RangeDopplerAlgorithm on one padded cf32
+    image (30240 x 6144) made from the scene's raw bytes. Each stage is timed with events on the context's stream,
+    the host synchronisations of the statistics stages included; the padded image is restored (untimed) before
+    each repetition because the stages work in place."""
+    import numpy as np
+    import torch
+    from asarf import configs, focus, scene
+    cfg = configs.asar_scene()
+    raw, _ = scene.generate(cfg)
+    na, nr = cfg.na, cfg.nr
+    nrp, nap = focus.padding_size(nr + cfg.nc), focus.padding_size(na)
+    ctx = focus.Context(0)
+    img0 = torch.zeros((nap, nrp), dtype=torch.complex64, device="cuda")
+    r = torch.from_numpy(raw).cuda().float()
+    img0[:na, :nr] = torch.complex(r[..., 0], r[..., 1])
+    del r
+    t = torch.empty_like(img0)
+    out = torch.empty_like(img0)
+    chirp = focus.generate_chirp(cfg, nrp)
+    names = ("iq_correction", "doppler_centroid", "range_compression", "range_doppler")
+    acc = {k: [] for k in names}
+    for rep in range(reps + 1):
+        t.copy_(img0)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev[0].record()
+        focus.iq_correction(ctx, t, nr, na)
+        ev[1].record()
+        dc_poly, _, _ = focus.doppler_centroid(ctx, t, nr, na, cfg.prf)
+        ev[2].record()
+        nrv = focus.range_compression(ctx, t, nr, na, chirp, cfg.nc)
+        ev[3].record()
+        focus.range_doppler(ctx, focus.rda_params(cfg, dc_poly), t, nrv, na, out=out)
+        ev[4].record()
+        torch.cuda.synchronize()
+        if rep:  # the first repetition warms up
+            for i, k in enumerate(names):
+                acc[k].append(ev[i].elapsed_time(ev[i + 1]))
+    ctx.close()
+    ms = {k: round(float(np.median(v)), 4) for k, v in acc.items()}
+    total = sum(ms.values())
+    return {"stages_probe": {"workload": "asar_im_30000x5616", "padded_image": [nrp, nap], "stages_ms": ms,
+                             "total_ms": round(total, 4),
+                             "msamples_per_s": round(na * nr / (total * 1e-3) / 1e6, 1)}}
+
+
 def run_batch(args):
     """Heterogeneous batch of 64 ERS/ASAR blocks: per-block sequential launches against one captured graph."""
     import numpy as np
@@ -221,6 +269,9 @@ def main():
     ap.add_argument("--ingest-probe", action="store_true",
                     help="time the level-0 decoders (ERS u8, ASAR FBAQ-4) and the output conditioning at full scene "
                          "size and print their rates")
+    ap.add_argument("--stages-probe", action="store_true",
+                    help="time the reference's stage sequence (I/Q correction, Doppler centroid, range compression, "
+                         "range-Doppler) through the stage entry points at the ASAR scene size")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (5 times): the known-size reference that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of the plan's kernels")
@@ -238,6 +289,9 @@ def main():
         return
     if args.ingest_probe:
         print(json.dumps(ingest_probe()))
+        return
+    if args.stages_probe:
+        print(json.dumps(stages_probe()))
         return
     if args.copy_probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")

@@ -109,8 +109,11 @@ def test_range_compression_matches_oracle(ctx, nr, na):
 
 
 # ------------------------------------------------------------------------------------- range-Doppler ----
-@pytest.mark.parametrize("na,nrv,dc", [(200, 300, 150.0), (600, 700, -300.0), (1100, 500, 40.0)])
-def test_range_doppler_matches_oracle(ctx, na, nrv, dc):
+# nap <= 1204: one column transform per pass; nap = 1500: four-step split, every pass out of place through the
+# consumed source buffer, column-tiled RCMC, or (8x the wavelength: migration beyond the tile halo) the row kernel
+@pytest.mark.parametrize("na,nrv,dc,wl", [(200, 300, 150.0, 1), (600, 700, -300.0, 1), (1100, 500, 40.0, 1),
+                                          (1500, 400, 80.0, 1), (1500, 400, -120.0, 8)])
+def test_range_doppler_matches_oracle(ctx, na, nrv, dc, wl):
     cfg = configs.ers_block()
     nap = po.next_fft_size(na)
     stride = nrv + 13
@@ -118,7 +121,8 @@ def test_range_doppler_matches_oracle(ctx, na, nrv, dc):
     src[:na, :nrv] = rand_c((na, nrv), na + nrv)
     dcp = [2e-5, -0.01, dc]
     p = focus.rda_params(cfg, dcp)
-    op = po.rda_params(cfg.wavelength, cfg.range_spacing, cfg.r_first, cfg.prf, cfg.az_bandwidth_fraction, dcp,
+    p.wavelength = cfg.wavelength * wl
+    op = po.rda_params(cfg.wavelength * wl, cfg.range_spacing, cfg.r_first, cfg.prf, cfg.az_bandwidth_fraction, dcp,
                        cfg.vr_poly)
     ref = po.range_doppler(src.astype(np.complex128), nrv, na, op)
     out = focus.range_doppler(ctx, p, cuda(src), nrv, na)
@@ -165,3 +169,50 @@ def test_multilook_rsl_srgr_match_oracle(ctx, width):
     sr_g = focus.srgr(ctx, ml_g, grsr, 7.8, 12.5, out_nr)
     assert rel_rms(sr_g.cpu().numpy(), sr_r) < 1e-6
     del rng
+
+
+# ------------------------------------------------------------------------- the reference's stage sequence ----
+@pytest.mark.parametrize("which", ["ers", "asar", "ers1500"])
+def test_stage_sequence_matches_oracle_and_plan(ctx, which):
+    """main.cc:243-283 through the stage entry points (RawDataCorrection, CalculateDopplerCentroid,
+    RangeCompression, RangeDopplerAlgorithm) on one padded image equals the FP64 oracle and the fused plan
+    (ers1500: 1500 lines, a four-step azimuth split)."""
+    if which == "ers":
+        cfg = configs.ers_block().scaled(na=700, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6))
+    elif which == "ers1500":
+        cfg = configs.ers_block().scaled(na=1500, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6))
+    else:
+        cfg = configs.asar_scene().scaled(na=640, nr=1400, n_targets_random=6)
+    raw, _ = scene.generate(cfg)
+    na, nr = cfg.na, cfg.nr
+    nrp, nap = po.next_fft_size(nr + cfg.nc), po.next_fft_size(na)
+    img = np.zeros((nap, nrp), np.complex64)
+    img[:na, :nr] = scene.raw_to_complex(raw)
+    t = cuda(img)
+    iq = focus.iq_correction(ctx, t, nr, na)
+    dc_poly, _, _ = focus.doppler_centroid(ctx, t, nr, na, cfg.prf)
+    chirp = focus.generate_chirp(cfg, nrp)
+    nrv = focus.range_compression(ctx, t, nr, na, chirp, cfg.nc)
+    out = focus.range_doppler(ctx, focus.rda_params(cfg, dc_poly), t, nrv, na)
+    slc = out[:na, :nrv].cpu().numpy()
+
+    ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    assert nrv == rres["nrv"]
+    err = rel_rms(slc, ref)
+    print(f"{which}: stage sequence vs oracle rel rms {err:.3e}")
+    assert err < 1e-4
+    for k, v in rres["iq"].items():
+        assert abs(iq[k] - v) <= 1e-9 * max(1.0, abs(v)), k
+
+    plan = focus.Plan(ctx, cfg)
+    plan.execute(torch.from_numpy(raw).cuda())
+    pslc = plan.slc_tensor().cpu().numpy()
+    pres = plan.results()
+    plan.close()
+    # the stage sequence estimates the Doppler centroid on the FP32-corrected image, as the reference does; the
+    # plan (and the oracle) from exact moments of the raw bytes: they agree to FP32 rounding of the samples
+    xs = np.arange(0, nrv)
+    assert np.abs(np.polyval(pres["dc_poly"], xs) - np.polyval(dc_poly, xs)).max() < 1e-3
+    assert np.abs(np.polyval(rres["dc_poly"], xs) - np.polyval(dc_poly, xs)).max() < 1e-3
+    assert pslc.shape == slc.shape
+    assert rel_rms(slc, pslc) < 1e-5
