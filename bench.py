@@ -60,10 +60,18 @@ def cpu_baseline(cfg, raw, min_seconds=10.0, max_reps=3):
         po.focus_slc(x, cfg)
         times.append(time.perf_counter() - t0)
     dt = sum(times) / len(times)
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {
         "value": round(cfg.na * cfg.nr / dt / 1e6, 2),
         "unit": "Msamples/s",
         "cores": threads,
+        "host_nproc": os.cpu_count(),
+        "host_cpu": model,
         "kind": "port",
         "sample": f"the same {cfg.na} x {cfg.nr} scene, whole SLC chain (I/Q, DC, RC, RDA) in the FP64 oracle, "
                   f"{len(times)} run(s), {dt:.2f} s each on {threads} threads",
