@@ -731,15 +731,14 @@ extern "C" int asarf_range_doppler(asarf_ctx* c, const asarf_rda_params* p, asar
     void* scratch = nullptr;
     ST_CHECK(ctx_scratch(c, sizeof(ColConst) * (size_t)nrv + 64, &scratch));
     auto* col = static_cast<ColConst*>(scratch);
-    auto* d_dc = reinterpret_cast<double*>(static_cast<char*>(scratch) + sizeof(ColConst) * (size_t)nrv);
-    // the Doppler polynomial and a zeroed slot for the device-side RCMC shift bound, in one copy
-    const double h_dc[4] = {p->dc_poly[0], p->dc_poly[1], p->dc_poly[2], 0.0};
-    int* d_reach = reinterpret_cast<int*>(d_dc + 3);
-    HIP_CHECK(hipMemcpyAsync(d_dc, h_dc, sizeof(h_dc), hipMemcpyHostToDevice, c->stream));
+    // device-side bound on the RCMC shift (pixels), cleared and then maximised by the column-constant kernels
+    int* d_reach = reinterpret_cast<int*>(static_cast<char*>(scratch) + sizeof(ColConst) * (size_t)nrv);
     ColConstArgs ca{};
     ca.col = col;
     ca.nrv = nrv;
-    ca.dc_poly = d_dc;
+    ca.dc_poly = nullptr;  // by value
+    for (int i = 0; i < 3; ++i) ca.dc_val[i] = p->dc_poly[i];
+    ca.clear_reach = 1;
     for (int i = 0; i < 3; ++i) ca.vr_poly[i] = p->vr_poly[i];
     ca.wavelength = p->wavelength;
     ca.range_spacing = p->range_spacing;

@@ -38,7 +38,7 @@ __global__ void k_col_const(ColConstArgs a) {
     int r = 0;
     if (valid) {
         const double step = a.prf / (double)a.nbins;
-        const double dc = polyval(a.dc_poly, 3, (double)x);
+        const double dc = polyval(a.dc_poly ? a.dc_poly : a.dc_val, 3, (double)x);
         const double r0 = a.r_first + (double)x * a.range_spacing;
         const double vr = polyval(a.vr_poly, 3, (double)x);
         ColConst c;
@@ -60,7 +60,14 @@ __global__ void k_col_const(ColConstArgs a) {
     }
 }
 
+// Clears the RCMC shift bound before k_col_const takes its maximum (a one-lane vector store; no memset, so the
+// sequence can be captured in a graph).
+__global__ void k_clear_int(int* p) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = 0;
+}
+
 hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s) {
+    if (a.clear_reach && a.reach) hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, s, a.reach);
     hipLaunchKernelGGL(k_col_const, dim3((a.nrv + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }

@@ -179,11 +179,13 @@ struct FinalizeArgs {
 struct ColConstArgs {
     ColConst* col;
     int nrv;
-    const double* dc_poly;  // device
+    const double* dc_poly;  // device; nullptr: dc_val
+    double dc_val[3];       // the Doppler polynomial by value (host callers)
     double vr_poly[3];
     double wavelength, range_spacing, r_first, prf;
     int nbins;
-    int* reach;             // non-null: atomic max of the per-column RCMC shift bound (cleared by finalize)
+    int* reach;             // non-null: atomic max of the per-column RCMC shift bound (cleared beforehand)
+    int clear_reach;        // launch_col_const clears *reach first (when no earlier kernel of the stream does)
 };
 
 struct MultilookArgs {
