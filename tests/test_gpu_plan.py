@@ -183,3 +183,21 @@ def test_plan_band_edge_sizes(ctx, na):
     print(f"na={na}: slc rel rms {err:.3e}, shift bound {res['rcmc_shift_bound']}")
     assert res["rcmc_shift_bound"] <= 24  # the tiled kernel ran
     assert err < SLC_TOL
+
+
+# scene sizes spread over generic four-step splits, both formats, single- and two-pass azimuth transforms
+@pytest.mark.parametrize("which,na,nr", [("ers", 2770, 1204), ("asar", 376, 1915), ("ers", 2342, 2438),
+                                         ("asar", 2710, 2201), ("ers", 1119, 1980), ("asar", 2052, 2130),
+                                         ("asar", 565, 1518), ("asar", 2707, 724)])
+def test_plan_size_sweep(ctx, which, na, nr):
+    if which == "ers":
+        cfg = configs.ers_block().scaled(na=na, nr=nr, n_targets_az=1, n_targets_rg=2)
+    else:
+        cfg = configs.asar_scene().scaled(na=na, nr=nr, n_targets_random=4)
+    raw, slc, res, _ = run_plan(ctx, cfg)
+    ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    err = rel_rms(slc, ref)
+    print(f"{which} {na}x{nr}: slc rel rms {err:.3e}")
+    assert err < SLC_TOL
+    xs = np.arange(0, cfg.nr - cfg.nc)
+    assert np.abs(np.polyval(res["dc_poly"], xs) - np.polyval(rres["dc_poly"], xs)).max() < 1e-6
