@@ -27,80 +27,129 @@ constexpr int kIngestRows = 8;  // lines per workgroup; each lane writes 2 adjac
 // :160-210, FMT 2 the same from the block-major table copy). ERS line y holds samples_per_line (I, Q) byte pairs; sample s lands at column s + 4 (swst[y] -
 // min_swst). ASAR record y = 2-byte little-endian length L, then L bytes in blocks of 64: one block-id byte and
 // 63 code bytes (I code in the high nibble, Q in the low); sample s is code byte 64 (s / 63) + 1 + s % 63 (if
-// that is < L) and lands at column s + swst[y] - min_swst. The lines of a workgroup are unrolled so that their
-// (dependent: length, code, table) loads are in flight together.
+// that is < L) and lands at column s + swst[y] - min_swst.
 template <int FMT>
 __global__ __launch_bounds__(kIngestThreads) void k_ingest(IngestArgs a) {
     const uint8_t* __restrict__ bytes = a.bytes;
     const float* __restrict__ i_lut = a.i_lut;
     const float* __restrict__ q_lut = a.q_lut;
     float2* __restrict__ out = a.out;
+    constexpr int R = kIngestRows;
     const int x = (blockIdx.x * kIngestThreads + threadIdx.x) * 2;
     if (x >= a.x_stride) return;
-    const int y0 = blockIdx.y * kIngestRows;
+    const int y0 = blockIdx.y * R;
     const bool pair = x + 1 < a.x_stride;
-    // decoded value of logical element (xc, y): NaN when no sample lands there
-    auto sample = [&](int xc, int y, int shift, int len) -> float2 {
-        const int s = xc - shift;
-        if (FMT == 0) {
-            if (s < 0 || s >= a.samples_per_line) return make_float2(NAN, NAN);
-            const uint8_t* p = bytes + (long)y * a.line_bytes + 2 * s;
-            return make_float2((float)p[0], (float)p[1]);
-        } else {
-            if (s < 0) return make_float2(NAN, NAN);
-            const int blk = s / 63;
-            const int k = 64 * blk + 1 + (s - 63 * blk);  // byte index within the block data
-            if (k >= len || 2 + k >= a.line_bytes) return make_float2(NAN, NAN);
-            const uint8_t* rec = bytes + (long)y * a.line_bytes;
-            const int block_id = rec[2 + 64 * blk];
-            const int code = rec[2 + k];
-            if (FMT == 2) {
-                // block-major copy of the tables: the 16 entries of one quantiser block share a 128 B line
-                const float2* __restrict__ lt = a.lut_t + 16 * block_id;
-                const int ci = code >> 4, cq = code & 15;
-                return make_float2(lt[ci > 7 ? 15 - ci : 8 + ci].x, lt[cq > 7 ? 15 - cq : 8 + cq].y);
-            }
-            const int ii = fbaq4_index(block_id, code >> 4), qi = fbaq4_index(block_id, code & 15);
-            if (ii >= a.lut_items || qi >= a.lut_items) return make_float2(NAN, NAN);
-            return make_float2(i_lut[ii], q_lut[qi]);
-        }
-    };
+    const float2 nan2 = make_float2(NAN, NAN);
+    // The R lines are processed in phases, each issuing the loads of every line before any result is used, so
+    // that the dependent loads (SWST code and record length, then the sample or code bytes, then the table
+    // entries) of all R lines are in flight together.
+    // phase 1: each line's shift and (ASAR) record length
+    int shift[R], len[R];
 #pragma unroll
-    for (int u = 0; u < kIngestRows; ++u) {
+    for (int u = 0; u < R; ++u) {
         const int y = y0 + u;
-        if (y >= a.y_stride) break;
-        float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
+        shift[u] = 0;
+        len[u] = 0;
         if (y < a.lines) {
-            const int shift = a.swst_mult * ((int)a.swst[y] - a.min_swst);
-            int len = 0;
+            shift[u] = a.swst_mult * ((int)a.swst[y] - a.min_swst);
             if (FMT != 0) {
                 const uint8_t* rec = bytes + (long)y * a.line_bytes;
-                len = (int)rec[0] | ((int)rec[1] << 8);
-            }
-            const int s0 = x - shift;
-            if (FMT == 0 && a.w32 && x + 1 < a.x_size && s0 >= 0 && s0 + 1 < a.samples_per_line) {
-                // both samples inside the line: one aligned 4-byte load (I0, Q0, I1, Q1). x is even and the ERS
-                // shift a multiple of 4, so 2 s0 is a multiple of 4; w32 says line pitch and base are too
-                const uint32_t w = *reinterpret_cast<const uint32_t*>(bytes + (long)y * a.line_bytes + 2 * s0);
-                v0 = make_float2((float)(w & 0xFF), (float)((w >> 8) & 0xFF));
-                v1 = make_float2((float)((w >> 16) & 0xFF), (float)(w >> 24));
-            } else {
-                if (x < a.x_size) v0 = sample(x, y, shift, len);
-                if (x + 1 < a.x_size) v1 = sample(x + 1, y, shift, len);
+                len[u] = (int)rec[0] | ((int)rec[1] << 8);
             }
         }
+    }
+    // v[u][j]: element (x + j, y0 + u); 0 in the padding, NaN where no sample lands in the logical region
+    float2 v[R][2];
+    if constexpr (FMT == 0) {
+        // phase 2: the samples (one aligned 4-byte load per pair where the pitch allows)
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int y = y0 + u;
+            v[u][0] = v[u][1] = make_float2(0.f, 0.f);
+            if (y >= a.lines) continue;
+            const uint8_t* line = bytes + (long)y * a.line_bytes;
+            const int s0 = x - shift[u];
+            if (a.w32 && x + 1 < a.x_size && s0 >= 0 && s0 + 1 < a.samples_per_line) {
+                // x is even and the ERS shift a multiple of 4, so 2 s0 is a multiple of 4; w32 says line pitch
+                // and base are too
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(line + 2 * s0);
+                v[u][0] = make_float2((float)(w & 0xFF), (float)((w >> 8) & 0xFF));
+                v[u][1] = make_float2((float)((w >> 16) & 0xFF), (float)(w >> 24));
+            } else {
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int s = s0 + j;
+                    if (x + j >= a.x_size) continue;
+                    v[u][j] = (s < 0 || s >= a.samples_per_line)
+                                  ? nan2
+                                  : make_float2((float)line[2 * s], (float)line[2 * s + 1]);
+                }
+            }
+        }
+    } else {
+        // phase 2: block-id and code bytes of both samples of every line; sample s is code byte
+        // 64 (s / 63) + 1 + s % 63 of the record's data, valid below the record length
+        int bid[R][2], code[R][2];
+        bool ok[R][2];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int y = y0 + u;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int s = x + j - shift[u];
+                const int blk = s / 63;
+                const int k = 64 * blk + 1 + (s - 63 * blk);
+                ok[u][j] = y < a.lines && x + j < a.x_size && s >= 0 && k < len[u] && 2 + k < a.line_bytes;
+                bid[u][j] = code[u][j] = 0;
+                if (ok[u][j]) {
+                    const uint8_t* rec = bytes + (long)y * a.line_bytes;
+                    bid[u][j] = rec[2 + 64 * blk];
+                    code[u][j] = rec[2 + k];
+                }
+            }
+        }
+        // phase 3: table entries
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int y = y0 + u;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (y >= a.lines || x + j >= a.x_size) {
+                    v[u][j] = make_float2(0.f, 0.f);
+                } else if (!ok[u][j]) {
+                    v[u][j] = nan2;
+                } else if (FMT == 2) {
+                    // block-major copy of the tables: the 16 entries of one quantiser block share a 128 B line
+                    const float2* __restrict__ lt = a.lut_t + 16 * bid[u][j];
+                    const int ci = code[u][j] >> 4, cq = code[u][j] & 15;
+                    v[u][j] = make_float2(lt[ci > 7 ? 15 - ci : 8 + ci].x, lt[cq > 7 ? 15 - cq : 8 + cq].y);
+                } else {
+                    const int ii = fbaq4_index(bid[u][j], code[u][j] >> 4);
+                    const int qi = fbaq4_index(bid[u][j], code[u][j] & 15);
+                    v[u][j] = (ii >= a.lut_items || qi >= a.lut_items) ? nan2 : make_float2(i_lut[ii], q_lut[qi]);
+                }
+            }
+        }
+    }
+    // phase 4: stores
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int y = y0 + u;
+        if (y >= a.y_stride) continue;
         float2* row = out + (long)y * a.x_stride;
         if (pair && a.vec16) {
-            *reinterpret_cast<float4*>(row + x) = make_float4(v0.x, v0.y, v1.x, v1.y);
+            *reinterpret_cast<float4*>(row + x) = make_float4(v[u][0].x, v[u][0].y, v[u][1].x, v[u][1].y);
         } else {
-            row[x] = v0;
-            if (pair) row[x + 1] = v1;
+            row[x] = v[u][0];
+            if (pair) row[x + 1] = v[u][1];
         }
     }
 }
 
 // Block-major copy of the FBAQ-4 tables: lut_t[16 b + c] = (i_lut[256 c + b], q_lut[256 c + b]) for the 256
-// quantiser blocks b and the 16 converted codes c (the full 4096-entry tables only).
+// quantiser blocks b and the 16 converted codes c (the full 4096-entry tables only). (A product table keyed by
+// block and code byte, one 8-byte read per sample, measured slower: a wave's reads then spread over 16 cache
+// lines per block instead of one.)
 __global__ void k_fbaq4_lut_t(const float* i_lut, const float* q_lut, float2* lut_t) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= 4096) return;
