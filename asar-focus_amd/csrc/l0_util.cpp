@@ -59,6 +59,10 @@ bool iequals(const char* a, const char* b) {
 
 extern "C" int asarf_mjd_compare(const asarf_mjd* a, const asarf_mjd* b) {
     // mjd ordering (envisat_types.h:42-73): lexicographic on (days, seconds, micros)
+    if (!a || !b) {
+        set_error(ASARF_EINVAL, "asarf_mjd_compare: null");
+        return 0;
+    }
     if (a->days != b->days) return a->days < b->days ? -1 : 1;
     if (a->seconds != b->seconds) return a->seconds < b->seconds ? -1 : 1;
     if (a->micros != b->micros) return a->micros < b->micros ? -1 : 1;

@@ -428,7 +428,8 @@ typedef struct {
     unsigned seconds;
     unsigned micros;
 } asarf_mjd;
-/* mjd ordering (envisat_types.h:42-73): -1, 0 or 1 as a is before, equal to or after b. */
+/* mjd ordering (envisat_types.h:42-73): -1, 0 or 1 as a is before, equal to or after b (both non-NULL; NULL
+ * compares as equal and sets the last error). */
 int asarf_mjd_compare(const asarf_mjd* a, const asarf_mjd* b);
 /* MjdAddMicros (envisat_utils.h:74-78); micros may be negative. */
 int asarf_mjd_add_micros(const asarf_mjd* in, long long micros, asarf_mjd* out);
