@@ -104,6 +104,19 @@ PRODUCT_SAR_IM0, PRODUCT_ASA_IM0, PRODUCT_SAR_IMS, PRODUCT_SAR_IMP, PRODUCT_ASA_
 INSTRUMENT_SAR, INSTRUMENT_ASAR = 0, 1
 
 
+class AsarEcho(ctypes.Structure):
+    _uc, _us = ctypes.c_ubyte, ctypes.c_ushort
+    _fields_ = [("isp_sensing_time", Mjd), ("fep_time", Mjd), ("isp_length", _us), ("packet_id", _us),
+                ("sequence_count", _us), ("packet_length", _us), ("segmentation_flags", _uc), ("mode_id", _uc),
+                ("antenna_beam_set_no", _uc), ("comp_ratio", _uc), ("echo_flag", _uc), ("noise_flag", _uc),
+                ("cal_flag", _uc), ("cal_type", _uc), ("cycle_count", _us), ("mode_count", ctypes.c_uint),
+                ("onboard_time", ctypes.c_ulonglong), ("pri_code", _us), ("swst_code", _us),
+                ("echo_window_code", _us), ("upconverter_raw", _uc), ("downconverter_raw", _uc), ("tx_pol", _uc),
+                ("rx_pol", _uc), ("cal_row_number", _uc), ("tx_pulse_code", _us), ("beam_adj_delta", _uc),
+                ("chirp_pulse_bw_code", _uc), ("aux_tx_monitor_level", _uc), ("resampling_factor", _us),
+                ("sample_bytes", ctypes.c_int), ("samples", ctypes.c_int)]
+
+
 class ResultsWindow(ctypes.Structure):
     _fields_ = [("lines_before_sensing_start", ctypes.c_long), ("lines_after_sensing_start", ctypes.c_long),
                 ("lines_before_sensing_stop", ctypes.c_long), ("lines_after_sensing_stop", ctypes.c_long),
@@ -167,6 +180,7 @@ SIGNATURES = {
     "asarf_product_type_from_name": (_ci, [ctypes.c_char_p]),
     "asarf_target_product": (_ci, [_ci, ctypes.c_char_p, _P(_ci)]),
     "asarf_instrument": (_ci, [_ci, _P(_ci)]),
+    "asarf_parse_asar_im_packet": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(AsarEcho), ctypes.c_void_p, ctypes.c_long]),
     "asarf_results_window": (_ci, [_cd, ctypes.c_long, ctypes.c_long, ctypes.c_long, _ci, _P(ResultsWindow)]),
     "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),

@@ -182,3 +182,94 @@ extern "C" int asarf_instrument(int product, int* out_instrument) {
     }
     return set_error(ASARF_EINVAL, "asarf_instrument: unsupported product type %d", product);
 }
+
+namespace {
+uint16_t be16(const unsigned char* p) { return (uint16_t)(((unsigned)p[0] << 8) | p[1]); }
+uint32_t be32(const unsigned char* p) {
+    return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | (uint32_t)p[3];
+}
+asarf_mjd be_mjd(const unsigned char* p) {
+    asarf_mjd m;
+    m.days = (int32_t)be32(p);
+    m.seconds = be32(p + 4);
+    m.micros = be32(p + 8);
+    return m;
+}
+}  // namespace
+
+extern "C" int asarf_parse_asar_im_packet(const unsigned char* packet, long n_bytes, asarf_asar_echo* meta,
+                                          unsigned char* record, long record_bytes) {
+    // One ASAR IM level-0 data set record (envisat_im_parse.cc:433-600; PO-TN-MMS-SR-0248), all fields big-endian:
+    //   0 ISP sensing time (MJD: int32 days, uint32 seconds, uint32 microseconds), 12 FEP annotations (MJD, ISP
+    //   length, CRC error count, correction count, spare), 32 packet id, 34 sequence control (14-bit count),
+    //   36 packet length, 38 data field header length (29), 41 mode id (0x54 for IM), 42 40-bit on-board time,
+    //   48 24-bit mode packet count, 51 antenna beam set (6 bits) | compression ratio (2 bits), 52 echo / noise /
+    //   calibration / calibration-type flags and the 12-bit cycle packet count, 54 PRI code, 56 SWST code,
+    //   58 echo window code, 60 up/down-converter levels, polarisations and calibration row, 62 Tx pulse code and
+    //   beam adjustment, 64 chirp bandwidth code, 65 auxiliary Tx monitor level, 66 resampling factor,
+    //   68 sample blocks (packet length - 29 bytes).
+    // For an echo packet, record (when given) receives the sample blocks as the decoder's record: a 2-byte
+    // little-endian length, the bytes, and zeros up to record_bytes.
+    if (!packet || !meta) return set_error(ASARF_EINVAL, "asarf_parse_asar_im_packet: null");
+    if (n_bytes < 68) return set_error(ASARF_ESHAPE, "asarf_parse_asar_im_packet: %ld bytes < 68", n_bytes);
+    asarf_asar_echo m{};
+    m.isp_sensing_time = be_mjd(packet);
+    m.fep_time = be_mjd(packet + 12);
+    m.isp_length = be16(packet + 24);
+    m.packet_id = be16(packet + 32);
+    const uint16_t seq = be16(packet + 34);
+    m.segmentation_flags = (unsigned char)(seq >> 14);
+    m.sequence_count = (unsigned short)(seq & 0x3FFF);
+    m.packet_length = be16(packet + 36);
+    const uint16_t dfh = be16(packet + 38);
+    if (dfh != 29)
+        return set_error(ASARF_EINVAL, "asarf_parse_asar_im_packet: data field header length %u, not 29", dfh);
+    m.mode_id = packet[41];
+    if (m.mode_id != 0x54)
+        return set_error(ASARF_EINVAL, "asarf_parse_asar_im_packet: mode id 0x%x, not IM (0x54)", m.mode_id);
+    m.onboard_time = ((unsigned long long)packet[42] << 32) | ((unsigned long long)packet[43] << 24) |
+                     ((unsigned long long)packet[44] << 16) | ((unsigned long long)packet[45] << 8) | packet[46];
+    m.mode_count = ((unsigned)packet[48] << 16) | ((unsigned)packet[49] << 8) | packet[50];
+    m.antenna_beam_set_no = (unsigned char)(packet[51] >> 2);
+    m.comp_ratio = (unsigned char)(packet[51] & 0x3);
+    m.echo_flag = (packet[52] & 0x80) != 0;
+    m.noise_flag = (packet[52] & 0x40) != 0;
+    m.cal_flag = (packet[52] & 0x20) != 0;
+    m.cal_type = (packet[52] & 0x10) != 0;
+    m.cycle_count = (unsigned short)(((packet[52] & 0x0F) << 8) | packet[53]);
+    m.pri_code = be16(packet + 54);
+    m.swst_code = be16(packet + 56);
+    m.echo_window_code = be16(packet + 58);
+    const uint16_t conv = be16(packet + 60);
+    m.upconverter_raw = (unsigned char)(conv >> 12);
+    m.downconverter_raw = (unsigned char)((conv >> 7) & 0x1F);
+    m.tx_pol = (conv & 0x40) != 0;
+    m.rx_pol = (conv & 0x20) != 0;
+    m.cal_row_number = (unsigned char)(conv & 0x1F);
+    const uint16_t pulse = be16(packet + 62);
+    m.tx_pulse_code = (unsigned short)(pulse >> 6);
+    m.beam_adj_delta = (unsigned char)(pulse & 0x3F);
+    m.chirp_pulse_bw_code = packet[64];
+    m.aux_tx_monitor_level = packet[65];
+    m.resampling_factor = be16(packet + 66);
+    if (m.packet_length < 29) return set_error(ASARF_EINVAL, "asarf_parse_asar_im_packet: packet length < 29");
+    const long data_len = (long)m.packet_length - 29;
+    if (68 + data_len > n_bytes)
+        return set_error(ASARF_ESHAPE, "asarf_parse_asar_im_packet: %ld sample bytes beyond the %ld-byte record",
+                         data_len, n_bytes);
+    m.sample_bytes = (int)data_len;
+    // whole 64-byte blocks hold 63 samples; a partial block holds its length minus the block-id byte
+    // (envisat_im_parse.cc:588-590)
+    m.samples = (int)((data_len / 64) * 63 + data_len % 64 - (data_len % 64 > 0 ? 1 : 0));
+    if (record && m.echo_flag) {
+        if (record_bytes < 2 + data_len)
+            return set_error(ASARF_EWORKSPACE, "asarf_parse_asar_im_packet: record of %ld bytes < %ld", record_bytes,
+                             2 + data_len);
+        record[0] = (unsigned char)(data_len & 0xFF);
+        record[1] = (unsigned char)(data_len >> 8);
+        std::memcpy(record + 2, packet + 68, (size_t)data_len);
+        std::memset(record + 2 + data_len, 0, (size_t)(record_bytes - 2 - data_len));
+    }
+    *meta = m;
+    return clear_error();
+}

@@ -480,6 +480,29 @@ int asarf_target_product(int in_product, const char* target, int* out_product);
 /* GetInstrumentFrom (asar_constants.cc:92-102); ASARF_EINVAL for detected and unidentified products. */
 int asarf_instrument(int product, int* out_instrument);
 
+/* Echo metadata of one ASAR IM level-0 packet (EchoMeta, envisat_im_parse.cc:54-80, plus the packet header). */
+typedef struct {
+    asarf_mjd isp_sensing_time, fep_time;
+    unsigned short isp_length, packet_id, sequence_count, packet_length;
+    unsigned char segmentation_flags, mode_id, antenna_beam_set_no, comp_ratio;
+    unsigned char echo_flag, noise_flag, cal_flag, cal_type;
+    unsigned short cycle_count;
+    unsigned mode_count;
+    unsigned long long onboard_time;
+    unsigned short pri_code, swst_code, echo_window_code;
+    unsigned char upconverter_raw, downconverter_raw, tx_pol, rx_pol, cal_row_number;
+    unsigned short tx_pulse_code;
+    unsigned char beam_adj_delta, chirp_pulse_bw_code, aux_tx_monitor_level;
+    unsigned short resampling_factor;
+    int sample_bytes; /* packet length - 29 */
+    int samples;      /* 63 per whole 64-byte block, length - 1 of a partial one */
+} asarf_asar_echo;
+/* One ASAR IM level-0 data set record (envisat_im_parse.cc:433-600): its echo metadata, and for an echo packet
+ * (record != NULL) its sample blocks as asarf_ingest_asar_fbaq4's record (2-byte little-endian length, bytes,
+ * zeros to record_bytes). ASARF_EINVAL for a data field header length other than 29 or a mode other than IM. */
+int asarf_parse_asar_im_packet(const unsigned char* packet, long n_bytes, asarf_asar_echo* meta,
+                               unsigned char* record, long record_bytes);
+
 #ifdef __cplusplus
 }
 #endif

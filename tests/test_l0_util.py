@@ -1,6 +1,8 @@
 """CPU checks of the level-0 host utilities through the C ABI against the reference's own unit-test vectors
 (envisat_format/unit-test/envisat_types_test.cc, envisat_utils_test.cc, parse_util_test.cc, ers_sbt_test.cc,
-asar_constants_test.cc) and, for the time arithmetic, against Python's datetime on random values."""
+asar_constants_test.cc), the time arithmetic against Python's datetime on random values, and the ASAR IM packet
+parser on records laid out as the reference's parser reads them (envisat_im_parse.cc:433-600; the reference ships
+no level-0 files), whose sample records must decode through the oracle's FBAQ-4 decoder."""
 import ctypes
 import datetime as dt
 import struct
@@ -8,6 +10,7 @@ import struct
 import numpy as np
 import pytest
 
+import pyoracle as po
 from asarf import _lib as L
 
 
@@ -148,3 +151,88 @@ def test_product_types_reference_cases():
     for prod in (L.PRODUCT_SAR_IMP, L.PRODUCT_UNIDENTIFIED):
         with pytest.raises(L.AsarfError):
             L.check(L.lib().asarf_instrument(prod, ctypes.byref(out)))
+
+
+# ------------------------------------------------------------------------------------- ASAR IM packets ----
+def asar_packet(rng, blocks, tail, swst=700, mode=0x54, dfh=29, flags=0x80, extra=0):
+    """A data set record laid out as envisat_im_parse.cc:433-600 reads it (big-endian fields)."""
+    data_len = 64 * blocks + tail
+    data = bytearray(rng.integers(0, 256, data_len, dtype=np.uint8).tobytes())
+    for b in range(blocks + (1 if tail else 0)):
+        data[64 * b] = int(rng.integers(0, 256))  # block id
+    hdr = struct.pack(">iII", -3026, 77935, 870958)                          # ISP sensing time
+    hdr += struct.pack(">iIIHHHH", -3026, 77935, 870000, 1234, 0, 0, 0)      # FEP annotations
+    hdr += struct.pack(">HHHH", 0x0A11, (1 << 14) | 513, data_len + 29, dfh)  # packet id, sequence, lengths
+    hdr += bytes([0, mode]) + (0x0102030405).to_bytes(5, "big") + b"\x00"      # mode id, on-board time, spare
+    hdr += (0x0A0B0C).to_bytes(3, "big") + bytes([(17 << 2) | 2, flags | 0x3, 0x45])  # mode count, beam, flags
+    hdr += struct.pack(">HHH", 2112, swst, 5616)                              # PRI, SWST, echo window codes
+    hdr += struct.pack(">HH", (9 << 12) | (21 << 7) | 0x40 | 13, (700 << 6) | 33)  # converters, pulse code
+    hdr += bytes([7, 200]) + struct.pack(">H", 7777)                          # chirp bw, aux tx, resampling
+    assert len(hdr) == 68
+    return bytes(hdr) + bytes(data) + bytes(extra), bytes(data)
+
+
+def parse(pkt, record_bytes=0):
+    m = L.AsarEcho()
+    rec = np.full(record_bytes, 0xAB, np.uint8) if record_bytes else None
+    L.check(L.lib().asarf_parse_asar_im_packet(pkt, len(pkt), ctypes.byref(m),
+                                               rec.ctypes.data if rec is not None else None, record_bytes))
+    return m, rec
+
+
+def test_asar_packet_fields_and_record():
+    rng = np.random.default_rng(4)
+    pkt, data = asar_packet(rng, 3, 10, extra=5)
+    m, rec = parse(pkt, record_bytes=300)
+    assert m.isp_sensing_time.astuple() == (-3026, 77935, 870958) and m.fep_time.astuple() == (-3026, 77935, 870000)
+    assert (m.isp_length, m.packet_id, m.segmentation_flags, m.sequence_count) == (1234, 0x0A11, 1, 513)
+    assert m.packet_length == len(data) + 29 and m.mode_id == 0x54 and m.onboard_time == 0x0102030405
+    assert (m.mode_count, m.antenna_beam_set_no, m.comp_ratio) == (0x0A0B0C, 17, 2)
+    assert (m.echo_flag, m.noise_flag, m.cal_flag, m.cal_type, m.cycle_count) == (1, 0, 0, 0, 0x345)
+    assert (m.pri_code, m.swst_code, m.echo_window_code) == (2112, 700, 5616)
+    assert (m.upconverter_raw, m.downconverter_raw, m.tx_pol, m.rx_pol, m.cal_row_number) == (9, 21, 1, 0, 13)
+    assert (m.tx_pulse_code, m.beam_adj_delta, m.chirp_pulse_bw_code, m.aux_tx_monitor_level) == (700, 33, 7, 200)
+    assert m.resampling_factor == 7777
+    assert m.sample_bytes == 202 and m.samples == 3 * 63 + 9
+    assert rec[0] | (rec[1] << 8) == 202 and bytes(rec[2:204]) == data and not rec[204:].any()
+
+
+def test_asar_packet_records_feed_the_decoder():
+    # parsed records are the decoder's input: sample s of line y comes from code byte 64 (s / 63) + 1 + s % 63
+    rng = np.random.default_rng(9)
+    i_lut, q_lut = rng.standard_normal(4096).astype(np.float32), rng.standard_normal(4096).astype(np.float32)
+    swsts = [700, 722, 700, 744]
+    recs, datas = np.zeros((4, 2 + 64 * 6), np.uint8), []
+    for y, sw in enumerate(swsts):
+        pkt, data = asar_packet(rng, 5 + (y % 2), 0 if y % 2 else 20, swst=sw)
+        m, rec = parse(pkt, record_bytes=recs.shape[1])
+        assert m.swst_code == sw
+        recs[y], _ = rec, datas.append(data)
+    img = po.ingest_asar_fbaq4(recs, np.array(swsts, np.uint16), 700, i_lut, q_lut, 500, 512, 8)
+    for y, (sw, data) in enumerate(zip(swsts, datas)):
+        for s in (0, 1, 62, 63, 64, 125, 200, 300):
+            k = 64 * (s // 63) + 1 + s % 63
+            x = s + sw - 700
+            if k >= len(data) or x >= 500:
+                continue
+            bid, code = data[64 * (s // 63)], data[k]
+            want = (i_lut[po.fbaq4_index(bid, code >> 4)], q_lut[po.fbaq4_index(bid, code & 15)])
+            assert (img[y, x].real, img[y, x].imag) == want, (y, s)
+
+
+def test_asar_packet_errors_and_non_echo():
+    rng = np.random.default_rng(5)
+    for kw in (dict(dfh=28), dict(mode=0x55)):
+        pkt, _ = asar_packet(rng, 2, 0, **kw)
+        with pytest.raises(L.AsarfError):
+            parse(pkt)
+    pkt, _ = asar_packet(rng, 2, 0)
+    m = L.AsarEcho()
+    assert L.lib().asarf_parse_asar_im_packet(pkt[:-1], len(pkt) - 1, ctypes.byref(m), None, 0) == L.ESHAPE
+    assert L.lib().asarf_parse_asar_im_packet(pkt[:60], 60, ctypes.byref(m), None, 0) == L.ESHAPE
+    small = np.zeros(100, np.uint8)
+    assert L.lib().asarf_parse_asar_im_packet(pkt, len(pkt), ctypes.byref(m), small.ctypes.data, 100) == L.EWORKSPACE
+    # a calibration packet: metadata parsed, no record written
+    pkt, _ = asar_packet(rng, 2, 0, flags=0x20)
+    m, rec = parse(pkt, record_bytes=200)
+    assert (m.echo_flag, m.cal_flag) == (0, 1) and (rec == 0xAB).all()
