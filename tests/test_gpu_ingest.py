@@ -99,3 +99,31 @@ def test_decoded_ers_image_through_iq_correction(ctx):
     err = np.sqrt(np.sum(np.abs(got - ref) ** 2) / np.sum(np.abs(ref) ** 2))
     print(f"decoded ERS through I/Q correction: rel rms {err:.2e}, gain {gres['iq_gain']:.6f}")
     assert err < 1e-6
+
+
+def test_parsed_asar_packets_decode_on_the_device(ctx):
+    """ASAR IM data set records through the host packet parser into the device FBAQ-4 decoder, bit-identical to
+    the oracle's decoder on the same parsed records."""
+    import ctypes
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from asarf import _lib as L
+    from test_l0_util import asar_packet
+    rng = np.random.default_rng(21)
+    n, rec_bytes = 40, 2 + 64 * 90
+    swst = np.array([700 + 11 * (i % 4) for i in range(n)], np.uint16)
+    recs = np.zeros((n, rec_bytes), np.uint8)
+    for y in range(n):
+        pkt, _ = asar_packet(rng, 80 + y % 10, (7 * y) % 64, swst=int(swst[y]))
+        m = L.AsarEcho()
+        L.check(L.lib().asarf_parse_asar_im_packet(pkt, len(pkt), ctypes.byref(m), recs[y].ctypes.data, rec_bytes))
+        assert m.swst_code == swst[y]
+    i_lut = rng.standard_normal(4096).astype(np.float32)
+    q_lut = rng.standard_normal(4096).astype(np.float32)
+    range_size, x_stride, y_stride = 5616 + 33, 5664, 48
+    out = torch.full((y_stride, x_stride), 3 - 3j, dtype=torch.complex64, device="cuda")
+    focus.ingest_asar_fbaq4(ctx, torch.from_numpy(recs).cuda(), torch.from_numpy(swst.view(np.int16)).cuda(), 700,
+                            torch.from_numpy(i_lut).cuda(), torch.from_numpy(q_lut).cuda(), out, range_size)
+    ref = po.ingest_asar_fbaq4(recs, swst, 700, i_lut, q_lut, range_size, x_stride, y_stride)
+    assert same(out.cpu().numpy(), ref)
