@@ -117,6 +117,14 @@ class AsarEcho(ctypes.Structure):
                 ("sample_bytes", ctypes.c_int), ("samples", ctypes.c_int)]
 
 
+class ErsEcho(ctypes.Structure):
+    _uc = ctypes.c_ubyte
+    _fields_ = [("isp_sensing_time", Mjd), ("isp_length", ctypes.c_ushort), ("data_record_number", ctypes.c_uint),
+                ("packet_counter", _uc), ("subcommutation_counter", _uc), ("obrc_orbit", _uc), ("activity_task", _uc),
+                ("sample_flags", _uc), ("sbt", ctypes.c_uint), ("image_format_counter", ctypes.c_uint),
+                ("swst_code", ctypes.c_ushort), ("pri_code", ctypes.c_ushort)]
+
+
 class ResultsWindow(ctypes.Structure):
     _fields_ = [("lines_before_sensing_start", ctypes.c_long), ("lines_after_sensing_start", ctypes.c_long),
                 ("lines_before_sensing_stop", ctypes.c_long), ("lines_after_sensing_stop", ctypes.c_long),
@@ -181,6 +189,7 @@ SIGNATURES = {
     "asarf_target_product": (_ci, [_ci, ctypes.c_char_p, _P(_ci)]),
     "asarf_instrument": (_ci, [_ci, _P(_ci)]),
     "asarf_parse_asar_im_packet": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(AsarEcho), ctypes.c_void_p, ctypes.c_long]),
+    "asarf_parse_ers_im_packet": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(ErsEcho), ctypes.c_void_p]),
     "asarf_results_window": (_ci, [_cd, ctypes.c_long, ctypes.c_long, ctypes.c_long, _ci, _P(ResultsWindow)]),
     "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),

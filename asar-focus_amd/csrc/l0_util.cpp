@@ -273,3 +273,37 @@ extern "C" int asarf_parse_asar_im_packet(const unsigned char* packet, long n_by
     *meta = m;
     return clear_error();
 }
+
+extern "C" int asarf_parse_ers_im_packet(const unsigned char* packet, long n_bytes, asarf_ers_echo* meta,
+                                         unsigned char* line) {
+    // One ERS IM level-0 data set record of 11498 bytes (ers_im_parse.cc:366-548, ers_env_format.h:26-39), fields
+    // big-endian: 0 ISP sensing time (MJD), 24 ISP length (11465), 32 data record number, 36 packet counter,
+    // 37 subcommutation counter, 46 auxiliary section marker (0xAA), 47 OBRC flag / orbit bits, 48 satellite
+    // binary time, 52 activity task, 53 sample flags, 54 image format counter, 58 SWST code, 60 PRI code, and from
+    // 266 the 5616 (I, Q) byte pairs of the echo, which line (when given, 11232 bytes) receives as the ERS
+    // decoder's line. The ISP sensing time is returned as recorded: the caller applies the SBT correction
+    // (asarf_adjust_isp_sensing_time with sbt and overflow = sbt < the previous packet's), as the reference does.
+    if (!packet || !meta) return set_error(ASARF_EINVAL, "asarf_parse_ers_im_packet: null");
+    if (n_bytes < 11498) return set_error(ASARF_ESHAPE, "asarf_parse_ers_im_packet: %ld bytes < 11498", n_bytes);
+    asarf_ers_echo m{};
+    m.isp_sensing_time = be_mjd(packet);
+    m.isp_length = be16(packet + 24);
+    if (m.isp_length != 11465)
+        return set_error(ASARF_EINVAL, "asarf_parse_ers_im_packet: ISP length %u, not 11465", m.isp_length);
+    m.data_record_number = be32(packet + 32);
+    m.packet_counter = packet[36];
+    m.subcommutation_counter = packet[37];
+    if (packet[46] != 0xAA)
+        return set_error(ASARF_EINVAL, "asarf_parse_ers_im_packet: auxiliary section starts with 0x%x, not 0xAA",
+                         packet[46]);
+    m.obrc_orbit = packet[47];
+    m.sbt = be32(packet + 48);
+    m.activity_task = packet[52];
+    m.sample_flags = packet[53];
+    m.image_format_counter = be32(packet + 54);
+    m.swst_code = be16(packet + 58);
+    m.pri_code = be16(packet + 60);
+    if (line) std::memcpy(line, packet + 266, 11232);
+    *meta = m;
+    return clear_error();
+}

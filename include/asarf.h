@@ -503,6 +503,21 @@ typedef struct {
 int asarf_parse_asar_im_packet(const unsigned char* packet, long n_bytes, asarf_asar_echo* meta,
                                unsigned char* record, long record_bytes);
 
+/* Echo metadata of one ERS IM level-0 packet (EchoMeta of ers_im_parse.cc). */
+typedef struct {
+    asarf_mjd isp_sensing_time; /* as recorded, before the SBT correction */
+    unsigned short isp_length;
+    unsigned data_record_number;
+    unsigned char packet_counter, subcommutation_counter, obrc_orbit, activity_task, sample_flags;
+    unsigned sbt;               /* satellite binary time (1/256 s ticks) */
+    unsigned image_format_counter;
+    unsigned short swst_code, pri_code;
+} asarf_ers_echo;
+/* One 11498-byte ERS IM level-0 data set record (ers_im_parse.cc:366-548): its metadata, and (line != NULL) the
+ * 11232 bytes of its 5616 (I, Q) pairs as asarf_ingest_ers_im's line. ASARF_EINVAL for an ISP length other than
+ * 11465 or an auxiliary section not starting with 0xAA. */
+int asarf_parse_ers_im_packet(const unsigned char* packet, long n_bytes, asarf_ers_echo* meta, unsigned char* line);
+
 #ifdef __cplusplus
 }
 #endif

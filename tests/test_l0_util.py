@@ -236,3 +236,68 @@ def test_asar_packet_errors_and_non_echo():
     pkt, _ = asar_packet(rng, 2, 0, flags=0x20)
     m, rec = parse(pkt, record_bytes=200)
     assert (m.echo_flag, m.cal_flag) == (0, 1) and (rec == 0xAB).all()
+
+
+# -------------------------------------------------------------------------------------- ERS IM packets ----
+def ers_packet(rng, sensing, sbt, swst=700, isp_length=11465, aux=0xAA):
+    """An 11498-byte data set record laid out as ers_im_parse.cc:366-548 reads it (big-endian fields)."""
+    samples = rng.integers(0, 32, 11232, dtype=np.uint8).tobytes()
+    p = bytearray(11498)
+    p[0:12] = struct.pack(">iII", *sensing)
+    p[24:26] = struct.pack(">H", isp_length)
+    p[32:36] = struct.pack(">I", 987654)
+    p[36], p[37], p[46], p[47] = 17, 23, aux, 0x21
+    p[48:52] = struct.pack(">I", sbt)
+    p[52], p[53] = 0xAA, 0x11
+    p[54:62] = struct.pack(">IHH", 424242, swst, 2800)
+    p[266:] = samples
+    return bytes(p), samples
+
+
+def parse_ers(pkt):
+    m = L.ErsEcho()
+    line = np.zeros(11232, np.uint8)
+    L.check(L.lib().asarf_parse_ers_im_packet(pkt, len(pkt), ctypes.byref(m), line.ctypes.data))
+    return m, line
+
+
+def test_ers_packet_fields_line_and_sbt_time():
+    rng = np.random.default_rng(3)
+    # the TDS01 values of ers_sbt_test.cc:58-124 carried through packets: the recorded ISP time and SBT of each
+    # packet, corrected with the registered PATC, give the reference test's expected times
+    ref = register(15236, 77966950, 1440813490, 3906249)
+    last_sbt = None
+    for sensing, sbt, want in (((-3026, 77935, 870958), 1440805534, (-3026, 77935, 871883)),
+                               ((-3026, 77966, 960794), 1440813493, (-3026, 77966, 961718))):
+        pkt, samples = ers_packet(rng, sensing, sbt)
+        m, line = parse_ers(pkt)
+        assert m.isp_sensing_time.astuple() == sensing and m.sbt == sbt
+        assert (m.isp_length, m.data_record_number, m.packet_counter, m.subcommutation_counter) == (11465, 987654, 17, 23)
+        assert (m.obrc_orbit, m.activity_task, m.sample_flags, m.image_format_counter) == (0x21, 0xAA, 0x11, 424242)
+        assert (m.swst_code, m.pri_code) == (700, 2800) and bytes(line) == samples
+        t = m.isp_sensing_time
+        overflow = last_sbt is not None and sbt < last_sbt
+        L.check(L.lib().asarf_adjust_isp_sensing_time(ctypes.byref(ref), ctypes.byref(t), sbt, int(overflow)))
+        assert t.astuple() == want
+        last_sbt = sbt
+
+
+def test_ers_packet_lines_feed_the_decoder_and_errors():
+    rng = np.random.default_rng(8)
+    swsts = [700, 704, 700, 712]
+    lines = np.zeros((4, 11232), np.uint8)
+    for y, sw in enumerate(swsts):
+        pkt, _ = ers_packet(rng, (0, 0, 0), 1, swst=sw)
+        m, lines[y] = parse_ers(pkt)
+        assert m.swst_code == sw
+    img = po.ingest_ers_im(lines, 5616, np.array(swsts, np.uint16), 700, 5616 + 48, 5680, 8)
+    for y, sw in enumerate(swsts):
+        for s in (0, 1, 100, 5615):
+            x = s + 4 * (sw - 700)
+            assert (img[y, x].real, img[y, x].imag) == (lines[y, 2 * s], lines[y, 2 * s + 1])
+    m = L.ErsEcho()
+    for kw in (dict(isp_length=11464), dict(aux=0xAB)):
+        pkt, _ = ers_packet(rng, (0, 0, 0), 1, **kw)
+        assert L.lib().asarf_parse_ers_im_packet(pkt, len(pkt), ctypes.byref(m), None) == L.EINVAL
+    pkt, _ = ers_packet(rng, (0, 0, 0), 1)
+    assert L.lib().asarf_parse_ers_im_packet(pkt, 11497, ctypes.byref(m), None) == L.ESHAPE
