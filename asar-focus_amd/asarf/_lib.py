@@ -114,7 +114,7 @@ class AsarEcho(ctypes.Structure):
                 ("echo_window_code", _us), ("upconverter_raw", _uc), ("downconverter_raw", _uc), ("tx_pol", _uc),
                 ("rx_pol", _uc), ("cal_row_number", _uc), ("tx_pulse_code", _us), ("beam_adj_delta", _uc),
                 ("chirp_pulse_bw_code", _uc), ("aux_tx_monitor_level", _uc), ("resampling_factor", _us),
-                ("sample_bytes", ctypes.c_int), ("samples", ctypes.c_int)]
+                ("sample_bytes", ctypes.c_int), ("samples", ctypes.c_int), ("dsr_bytes", ctypes.c_int)]
 
 
 class ErsEcho(ctypes.Structure):
@@ -123,6 +123,12 @@ class ErsEcho(ctypes.Structure):
                 ("packet_counter", _uc), ("subcommutation_counter", _uc), ("obrc_orbit", _uc), ("activity_task", _uc),
                 ("sample_flags", _uc), ("sbt", ctypes.c_uint), ("image_format_counter", ctypes.c_uint),
                 ("swst_code", ctypes.c_ushort), ("pri_code", ctypes.c_ushort)]
+
+
+class Dsd(ctypes.Structure):
+    _fields_ = [("ds_name", ctypes.c_char * 64), ("ds_type", ctypes.c_char * 16), ("filename", ctypes.c_char * 96),
+                ("ds_offset", ctypes.c_ulonglong), ("ds_size", ctypes.c_ulonglong), ("num_dsr", ctypes.c_ulonglong),
+                ("dsr_size", ctypes.c_ulonglong)]
 
 
 class ResultsWindow(ctypes.Structure):
@@ -190,6 +196,8 @@ SIGNATURES = {
     "asarf_instrument": (_ci, [_ci, _P(_ci)]),
     "asarf_parse_asar_im_packet": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(AsarEcho), ctypes.c_void_p, ctypes.c_long]),
     "asarf_parse_ers_im_packet": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(ErsEcho), ctypes.c_void_p]),
+    "asarf_header_get": (_ci, [ctypes.c_char_p, ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, _ci]),
+    "asarf_level0_mdsr": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(Dsd)]),
     "asarf_results_window": (_ci, [_cd, ctypes.c_long, ctypes.c_long, ctypes.c_long, _ci, _P(ResultsWindow)]),
     "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),

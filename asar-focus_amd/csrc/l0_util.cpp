@@ -258,6 +258,7 @@ extern "C" int asarf_parse_asar_im_packet(const unsigned char* packet, long n_by
         return set_error(ASARF_ESHAPE, "asarf_parse_asar_im_packet: %ld sample bytes beyond the %ld-byte record",
                          data_len, n_bytes);
     m.sample_bytes = (int)data_len;
+    m.dsr_bytes = 39 + (int)m.isp_length;  // the next record's offset, as the reference steps (envisat_im_parse.cc:188)
     // whole 64-byte blocks hold 63 samples; a partial block holds its length minus the block-id byte
     // (envisat_im_parse.cc:588-590)
     m.samples = (int)((data_len / 64) * 63 + data_len % 64 - (data_len % 64 > 0 ? 1 : 0));
@@ -306,4 +307,94 @@ extern "C" int asarf_parse_ers_im_packet(const unsigned char* packet, long n_byt
     if (line) std::memcpy(line, packet + 266, 11232);
     *meta = m;
     return clear_error();
+}
+
+namespace {
+constexpr long kMphSize = 1247, kSphSize = 1956, kDsdSize = 280;
+
+// extract_bytes / extract_num (envisat_mph_sph_parser.h:86-107): from the first digit 1-9 after the '=' (signs and
+// leading zeros are skipped, so "-0000000001" reads as 1 and an all-zero field as 0), the decimal number
+unsigned long long dsd_number(const char* p, long n) {
+    long i = 0;
+    while (i < n && p[i] != '=') ++i;
+    for (; i < n; ++i)
+        if (p[i] >= '1' && p[i] <= '9') break;
+    unsigned long long v = 0;
+    for (; i < n && p[i] >= '0' && p[i] <= '9'; ++i) v = v * 10 + (unsigned long long)(p[i] - '0');
+    return v;
+}
+
+// the idx-th '\n'-separated token of [p, p + n): its start and length (false when there are fewer tokens)
+bool nth_line(const char* p, long n, int idx, const char** s, long* len) {
+    long start = 0;
+    for (int k = 0; k < idx; ++k) {
+        while (start < n && p[start] != '\n') ++start;
+        if (start >= n) return false;
+        ++start;
+    }
+    long end = start;
+    while (end < n && p[end] != '\n') ++end;
+    *s = p + start;
+    *len = end - start;
+    return true;
+}
+
+void copy_text(char* dst, int cap, const char* src, long len) {
+    if (!dst || cap <= 0) return;
+    const long k = len < cap - 1 ? len : cap - 1;
+    std::memcpy(dst, src, (size_t)k);
+    dst[k] = '\0';
+}
+}  // namespace
+
+extern "C" int asarf_header_get(const char* text, long n_bytes, const char* key, char* value, int capacity) {
+    // ProductHeader::Load / Get (envisat_mph_sph_parser.h:41-77): '\n'-separated KEY=value lines; the value of the
+    // first line whose key matches, without one surrounding pair of quotes
+    if (!text || !key || !value || capacity <= 0) return set_error(ASARF_EINVAL, "asarf_header_get: argument");
+    const long klen = (long)std::strlen(key);
+    long start = 0;
+    while (start < n_bytes) {
+        long end = start;
+        while (end < n_bytes && text[end] != '\n') ++end;
+        long eq = start;
+        while (eq < end && text[eq] != '=') ++eq;
+        if (eq < end && eq - start == klen && std::memcmp(text + start, key, (size_t)klen) == 0) {
+            long vs = eq + 1, ve = end;
+            if (ve > vs && text[ve - 1] == '"') --ve;
+            if (ve > vs && text[vs] == '"') ++vs;
+            copy_text(value, capacity, text + vs, ve - vs);
+            return clear_error();
+        }
+        start = end + 1;
+    }
+    return set_error(ASARF_EINVAL, "asarf_header_get: %s not found", key);
+}
+
+extern "C" int asarf_level0_mdsr(const unsigned char* file, long n_bytes, asarf_dsd* out) {
+    // ParseMdsr / ExtractDSDs (envisat_lvl0_parser.cc:98-102, envisat_mph_sph_parser.h:110-133): the first of the
+    // four 280-byte data set descriptors at the end of the 1956-byte SPH that follows the 1247-byte MPH; lines 0-2
+    // as they stand (name, type, filename), 3 offset, 4 size, 5 record count, 6 record size
+    if (!file || !out) return set_error(ASARF_EINVAL, "asarf_level0_mdsr: null");
+    if (n_bytes < kMphSize + kSphSize)
+        return set_error(ASARF_ESHAPE, "asarf_level0_mdsr: %ld bytes < MPH + SPH (%ld)", n_bytes, kMphSize + kSphSize);
+    const char* sph = reinterpret_cast<const char*>(file) + kMphSize;
+    for (int d = 0; d < 4; ++d) {
+        const char* dsd = sph + kSphSize - 4 * kDsdSize + d * kDsdSize;
+        const char* ln[7];
+        long len[7];
+        bool ok = true;
+        for (int i = 0; i < 7 && ok; ++i) ok = nth_line(dsd, kDsdSize, i, &ln[i], &len[i]);
+        if (!ok) continue;  // fewer tokens than a descriptor has: not a descriptor
+        asarf_dsd r{};
+        copy_text(r.ds_name, sizeof(r.ds_name), ln[0], len[0]);
+        copy_text(r.ds_type, sizeof(r.ds_type), ln[1], len[1]);
+        copy_text(r.filename, sizeof(r.filename), ln[2], len[2]);
+        r.ds_offset = dsd_number(ln[3], len[3]);
+        r.ds_size = dsd_number(ln[4], len[4]);
+        r.num_dsr = dsd_number(ln[5], len[5]);
+        r.dsr_size = dsd_number(ln[6], len[6]);
+        *out = r;
+        return clear_error();
+    }
+    return set_error(ASARF_EINVAL, "asarf_level0_mdsr: no data set descriptor");
 }

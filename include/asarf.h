@@ -496,6 +496,7 @@ typedef struct {
     unsigned short resampling_factor;
     int sample_bytes; /* packet length - 29 */
     int samples;      /* 63 per whole 64-byte block, length - 1 of a partial one */
+    int dsr_bytes;    /* 39 + FEP ISP length: the offset of the next data set record */
 } asarf_asar_echo;
 /* One ASAR IM level-0 data set record (envisat_im_parse.cc:433-600): its echo metadata, and for an echo packet
  * (record != NULL) its sample blocks as asarf_ingest_asar_fbaq4's record (2-byte little-endian length, bytes,
@@ -517,6 +518,19 @@ typedef struct {
  * 11232 bytes of its 5616 (I, Q) pairs as asarf_ingest_ers_im's line. ASARF_EINVAL for an ISP length other than
  * 11465 or an auxiliary section not starting with 0xAA. */
 int asarf_parse_ers_im_packet(const unsigned char* packet, long n_bytes, asarf_ers_echo* meta, unsigned char* line);
+
+/* ProductHeader::Get (envisat_mph_sph_parser.h:41-77): the value of KEY in '\n'-separated KEY=value text (MPH,
+ * SPH), one pair of surrounding quotes removed, NUL-terminated into value (truncated to capacity - 1). */
+int asarf_header_get(const char* text, long n_bytes, const char* key, char* value, int capacity);
+/* DSD_lvl0 (envisat_mph_sph_parser.h:30-38). Names are the descriptor's lines as they stand. */
+typedef struct {
+    char ds_name[64], ds_type[16], filename[96];
+    unsigned long long ds_offset, ds_size, num_dsr, dsr_size;
+} asarf_dsd;
+/* ParseMdsr (envisat_lvl0_parser.cc:98-102): the measurement data set descriptor of a level-0 file (the first
+ * complete one of the four at the end of the SPH). Numbers are read as the reference reads them: from the first
+ * digit 1-9 after '=', so a sign is dropped ("-0000000001" reads as 1). */
+int asarf_level0_mdsr(const unsigned char* file, long n_bytes, asarf_dsd* out);
 
 #ifdef __cplusplus
 }

@@ -161,7 +161,7 @@ def asar_packet(rng, blocks, tail, swst=700, mode=0x54, dfh=29, flags=0x80, extr
     for b in range(blocks + (1 if tail else 0)):
         data[64 * b] = int(rng.integers(0, 256))  # block id
     hdr = struct.pack(">iII", -3026, 77935, 870958)                          # ISP sensing time
-    hdr += struct.pack(">iIIHHHH", -3026, 77935, 870000, 1234, 0, 0, 0)      # FEP annotations
+    hdr += struct.pack(">iIIHHHH", -3026, 77935, 870000, data_len + 29, 0, 0, 0)  # FEP (ISP length)
     hdr += struct.pack(">HHHH", 0x0A11, (1 << 14) | 513, data_len + 29, dfh)  # packet id, sequence, lengths
     hdr += bytes([0, mode]) + (0x0102030405).to_bytes(5, "big") + b"\x00"      # mode id, on-board time, spare
     hdr += (0x0A0B0C).to_bytes(3, "big") + bytes([(17 << 2) | 2, flags | 0x3, 0x45])  # mode count, beam, flags
@@ -185,7 +185,8 @@ def test_asar_packet_fields_and_record():
     pkt, data = asar_packet(rng, 3, 10, extra=5)
     m, rec = parse(pkt, record_bytes=300)
     assert m.isp_sensing_time.astuple() == (-3026, 77935, 870958) and m.fep_time.astuple() == (-3026, 77935, 870000)
-    assert (m.isp_length, m.packet_id, m.segmentation_flags, m.sequence_count) == (1234, 0x0A11, 1, 513)
+    assert (m.isp_length, m.packet_id, m.segmentation_flags, m.sequence_count) == (len(data) + 29, 0x0A11, 1, 513)
+    assert m.dsr_bytes == 68 + len(data)  # extra bytes after the blocks belong to the next record
     assert m.packet_length == len(data) + 29 and m.mode_id == 0x54 and m.onboard_time == 0x0102030405
     assert (m.mode_count, m.antenna_beam_set_no, m.comp_ratio) == (0x0A0B0C, 17, 2)
     assert (m.echo_flag, m.noise_flag, m.cal_flag, m.cal_type, m.cycle_count) == (1, 0, 0, 0, 0x345)
@@ -301,3 +302,68 @@ def test_ers_packet_lines_feed_the_decoder_and_errors():
         assert L.lib().asarf_parse_ers_im_packet(pkt, len(pkt), ctypes.byref(m), None) == L.EINVAL
     pkt, _ = ers_packet(rng, (0, 0, 0), 1)
     assert L.lib().asarf_parse_ers_im_packet(pkt, 11497, ctypes.byref(m), None) == L.ESHAPE
+
+
+# ------------------------------------------------------------------------------- product header and MDSR ----
+def product_file(rng, n_packets, dsr_size=b"-0000000001"):
+    mph = (b'PRODUCT="ASA_IM__0PNPDK20040109_194924_000000182023_00157_09730_1479.N1"\n'
+           b'SENSING_START="09-JAN-2004 19:49:24.000000"\nREL_ORBIT=+00157\nPHASE=2\n')
+    mph = mph.ljust(1246, b" ") + b"\n"
+    sph = b'SPH_DESCRIPTOR="ASAR Image Mode Level 0"\nSWATH="IS2"\nTX_RX_POLAR="V/V"\n'.ljust(1956 - 4 * 280, b" ")
+    packets, datas = [], []
+    for i in range(n_packets):
+        pkt, data = asar_packet(rng, 3 + i % 3, 5 * i, swst=700 + 11 * i)
+        packets.append(pkt)
+        datas.append(data)
+    body = b"".join(packets)
+    offset = 1247 + 1956
+    dsd = (b'DS_NAME="ASAR SOURCE PACKETS         "\nDS_TYPE=M\nFILENAME="' + b" " * 62 + b'"\n'
+           + b"DS_OFFSET=+%020d<bytes>\nDS_SIZE=+%020d<bytes>\nNUM_DSR=+%010d\nDSR_SIZE=" % (offset, len(body), n_packets)
+           + dsr_size + b"<bytes>\n")
+    dsd = dsd.ljust(279, b" ") + b"\n"
+    spare = (b"DS_NAME=\n" * 7).ljust(279, b" ") + b"\n"
+    sph += dsd + spare * 3
+    assert len(mph) == 1247 and len(sph) == 1956
+    return mph + sph + body, datas
+
+
+def test_header_lookup_and_mdsr_descriptor():
+    rng = np.random.default_rng(12)
+    f, _ = product_file(rng, 4)
+    v = ctypes.create_string_buffer(128)
+    L.check(L.lib().asarf_header_get(f, 1247, b"PRODUCT", v, 128))
+    assert v.value == b"ASA_IM__0PNPDK20040109_194924_000000182023_00157_09730_1479.N1"
+    assert L.lib().asarf_product_type_from_name(v.value) == L.PRODUCT_ASA_IM0
+    L.check(L.lib().asarf_header_get(f, 1247, b"REL_ORBIT", v, 128))
+    assert v.value == b"+00157"
+    L.check(L.lib().asarf_header_get(f[1247:], 1956, b"SWATH", v, 128))
+    assert v.value == b"IS2"
+    with pytest.raises(L.AsarfError):
+        L.check(L.lib().asarf_header_get(f, 1247, b"ABS_ORBIT", v, 128))
+    d = L.Dsd()
+    L.check(L.lib().asarf_level0_mdsr(f, len(f), ctypes.byref(d)))
+    assert d.ds_name == b'DS_NAME="ASAR SOURCE PACKETS         "' and d.ds_type == b"DS_TYPE=M"
+    assert (d.ds_offset, d.num_dsr) == (1247 + 1956, 4) and d.ds_size == len(f) - 3203
+    assert d.dsr_size == 1  # "-0000000001" (variable-size records) reads as 1, as the reference reads it
+    f2, _ = product_file(rng, 1, dsr_size=b"+0000011498")
+    L.check(L.lib().asarf_level0_mdsr(f2, len(f2), ctypes.byref(d)))
+    assert d.dsr_size == 11498
+    assert L.lib().asarf_level0_mdsr(f, 3000, ctypes.byref(d)) == L.ESHAPE
+
+
+def test_level0_file_walk_to_decoder_records():
+    # MPH -> MDSR -> packets stepped by 39 + ISP length -> decoder records
+    rng = np.random.default_rng(13)
+    f, datas = product_file(rng, 6)
+    d = L.Dsd()
+    L.check(L.lib().asarf_level0_mdsr(f, len(f), ctypes.byref(d)))
+    off, swsts = d.ds_offset, []
+    recs = np.zeros((d.num_dsr, 2 + 64 * 6), np.uint8)
+    for i in range(d.num_dsr):
+        m = L.AsarEcho()
+        L.check(L.lib().asarf_parse_asar_im_packet(f[off:], len(f) - off, ctypes.byref(m), recs[i].ctypes.data,
+                                                   recs.shape[1]))
+        assert bytes(recs[i, 2:2 + m.sample_bytes]) == datas[i]
+        swsts.append(m.swst_code)
+        off += m.dsr_bytes
+    assert off == len(f) and swsts == [700 + 11 * i for i in range(6)]
