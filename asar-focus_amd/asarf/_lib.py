@@ -198,6 +198,9 @@ SIGNATURES = {
     "asarf_parse_ers_im_packet": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(ErsEcho), ctypes.c_void_p]),
     "asarf_header_get": (_ci, [ctypes.c_char_p, ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, _ci]),
     "asarf_level0_mdsr": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(Dsd)]),
+    "asarf_asar_forecast": (_ci, [ctypes.c_char_p, ctypes.c_long, ctypes.c_long, _P(Mjd), _P(Mjd),
+                                  _P(ctypes.c_long), _P(ctypes.c_long), _P(ctypes.c_long), _P(Mjd), ctypes.c_long,
+                                  _P(ctypes.c_long)]),
     "asarf_results_window": (_ci, [_cd, ctypes.c_long, ctypes.c_long, ctypes.c_long, _ci, _P(ResultsWindow)]),
     "asarf_condition_slc": (_ci, [_vp, _P(Image), ctypes.c_float, _ci, _vp, ctypes.c_size_t]),
     "asarf_condition_detected": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_float, _ci, _vp, ctypes.c_size_t]),

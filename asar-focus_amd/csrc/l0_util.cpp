@@ -3,7 +3,9 @@
 // identification. Host-only integer work; the packets they describe are decoded on the device by k_ingest.
 #include <cctype>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "asarf.h"
 #include "internal.h"
@@ -397,4 +399,64 @@ extern "C" int asarf_level0_mdsr(const unsigned char* file, long n_bytes, asarf_
         return clear_error();
     }
     return set_error(ASARF_EINVAL, "asarf_level0_mdsr: no data set descriptor");
+}
+
+extern "C" int asarf_asar_forecast(const unsigned char* mds, long mds_bytes, long num_dsr, const asarf_mjd* start,
+                                   const asarf_mjd* stop, long* before_start, long* after_stop, long* offsets,
+                                   asarf_mjd* times, long capacity, long* n_out) {
+    // FetchEnvisatL0ImForecastMeta (envisat_im_parse.cc:297-384): walk the num_dsr records of the measurement data
+    // set; skip packets before the first echo and packets that are neither echo nor calibration; count those
+    // sensed before start and after stop; stop once more than *after_stop (the lines the caller may keep after the
+    // stop) lie after it; then drop the surplus before the start beyond *before_start, and leading calibration
+    // packets, so the list starts with an echo. On return *before_start / *after_stop hold the counts kept.
+    if (!mds || !start || !stop || !before_start || !after_stop || !offsets || !times || !n_out)
+        return set_error(ASARF_EINVAL, "asarf_asar_forecast: null");
+    std::vector<long> off;
+    std::vector<asarf_mjd> tim;
+    std::vector<char> is_echo;
+    long before = 0, after = 0, pos = 0;
+    for (long i = 0; i < num_dsr; ++i) {
+        if (pos + 68 > mds_bytes)
+            return set_error(ASARF_ESHAPE, "asarf_asar_forecast: record %ld beyond the data set", i);
+        const unsigned char* p = mds + pos;
+        const asarf_mjd t = be_mjd(p);
+        const uint16_t isp_length = be16(p + 24);
+        if (be16(p + 38) != 29)
+            return set_error(ASARF_EINVAL, "asarf_asar_forecast: record %ld data field header length is not 29", i);
+        const bool echo = (p[52] & 0x80) != 0, cal = (p[52] & 0x20) != 0;
+        const long here = pos;
+        pos += 39 + (long)isp_length;
+        if (off.empty() && !echo) continue;
+        if (!echo && !cal) continue;
+        if (asarf_mjd_compare(&t, start) < 0) ++before;
+        if (asarf_mjd_compare(&t, stop) > 0) ++after;
+        if (after > *after_stop) {
+            --after;
+            break;
+        }
+        off.push_back(here);
+        tim.push_back(t);
+        is_echo.push_back(echo ? 1 : 0);
+    }
+    size_t first = 0;
+    if (before > *before_start) {
+        first = (size_t)(before - *before_start);
+        before = *before_start;
+        // the list must start with an echo: drop leading calibration packets (the reference tests the calibration
+        // flag of the last record it read instead of the erased entry's; every kept non-echo entry is one)
+        while (first < off.size() && !is_echo[first]) {
+            ++first;
+            --before;
+        }
+    }
+    const long n = (long)(off.size() - std::min(first, off.size()));
+    if (n > capacity) return set_error(ASARF_EWORKSPACE, "asarf_asar_forecast: %ld entries > capacity %ld", n, capacity);
+    for (long k = 0; k < n; ++k) {
+        offsets[k] = off[first + k];
+        times[k] = tim[first + k];
+    }
+    *n_out = n;
+    *before_start = before;
+    *after_stop = after;
+    return clear_error();
 }

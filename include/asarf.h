@@ -532,6 +532,14 @@ typedef struct {
  * digit 1-9 after '=', so a sign is dropped ("-0000000001" reads as 1). */
 int asarf_level0_mdsr(const unsigned char* file, long n_bytes, asarf_dsd* out);
 
+/* FetchEnvisatL0ImForecastMeta (envisat_im_parse.cc:297-384): the ASAR IM records to focus between the sensing
+ * times start and stop. *before_start / *after_stop: in, the lines allowed before the start and after the stop;
+ * out, the lines kept there. offsets (relative to mds) and ISP times of the n_out kept records, the first an echo.
+ * The leading calibration records are dropped by their own flags (the reference tests the last record read). */
+int asarf_asar_forecast(const unsigned char* mds, long mds_bytes, long num_dsr, const asarf_mjd* start,
+                        const asarf_mjd* stop, long* before_start, long* after_stop, long* offsets, asarf_mjd* times,
+                        long capacity, long* n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -154,13 +154,13 @@ def test_product_types_reference_cases():
 
 
 # ------------------------------------------------------------------------------------- ASAR IM packets ----
-def asar_packet(rng, blocks, tail, swst=700, mode=0x54, dfh=29, flags=0x80, extra=0):
+def asar_packet(rng, blocks, tail, swst=700, mode=0x54, dfh=29, flags=0x80, extra=0, sensing=(-3026, 77935, 870958)):
     """A data set record laid out as envisat_im_parse.cc:433-600 reads it (big-endian fields)."""
     data_len = 64 * blocks + tail
     data = bytearray(rng.integers(0, 256, data_len, dtype=np.uint8).tobytes())
     for b in range(blocks + (1 if tail else 0)):
         data[64 * b] = int(rng.integers(0, 256))  # block id
-    hdr = struct.pack(">iII", -3026, 77935, 870958)                          # ISP sensing time
+    hdr = struct.pack(">iII", *sensing)                                       # ISP sensing time
     hdr += struct.pack(">iIIHHHH", -3026, 77935, 870000, data_len + 29, 0, 0, 0)  # FEP (ISP length)
     hdr += struct.pack(">HHHH", 0x0A11, (1 << 14) | 513, data_len + 29, dfh)  # packet id, sequence, lengths
     hdr += bytes([0, mode]) + (0x0102030405).to_bytes(5, "big") + b"\x00"      # mode id, on-board time, spare
@@ -367,3 +367,59 @@ def test_level0_file_walk_to_decoder_records():
         swsts.append(m.swst_code)
         off += m.dsr_bytes
     assert off == len(f) and swsts == [700 + 11 * i for i in range(6)]
+
+
+# ------------------------------------------------------------------------------------ ASAR packet forecast ----
+def forecast(mds, n, start, stop, before, after, cap=64):
+    b, a, cnt = ctypes.c_long(before), ctypes.c_long(after), ctypes.c_long()
+    offs = (ctypes.c_long * cap)()
+    times = (L.Mjd * cap)()
+    L.check(L.lib().asarf_asar_forecast(mds, len(mds), n, ctypes.byref(L.Mjd(*start)), ctypes.byref(L.Mjd(*stop)),
+                                        ctypes.byref(b), ctypes.byref(a), offs, times, cap, ctypes.byref(cnt)))
+    return list(offs[:cnt.value]), [t.astuple()[2] for t in times[:cnt.value]], b.value, a.value
+
+
+NOISE, CAL, ECHO = 0x40, 0x20, 0x80
+
+
+def build_mds(rng, flags):
+    pkts, offs, pos = [], [], 0
+    for i, f in enumerate(flags):
+        pkt, _ = asar_packet(rng, 1, 0, flags=f, sensing=(100, 5, 1000 * i))
+        pkts.append(pkt)
+        offs.append(pos)
+        pos += len(pkt)
+    return b"".join(pkts), offs
+
+
+def test_asar_forecast_window_and_trimming():
+    # envisat_im_parse.cc:297-384 on ten records sensed 1 ms apart: leading noise and calibration records are
+    # skipped until the first echo, a noise record later on is skipped, a calibration record is kept; the window
+    # is [4 ms, 7 ms]; one line may be kept before the start and one after the stop
+    rng = np.random.default_rng(14)
+    flags = [NOISE, CAL, ECHO, ECHO, ECHO, CAL, NOISE, ECHO, ECHO, ECHO]
+    mds, offs = build_mds(rng, flags)
+    o, t, before, after = forecast(mds, 10, (100, 5, 4000), (100, 5, 7000), 1, 1)
+    # kept before trimming: 2, 3 (before the start), 4, 5 (cal), 7, 8 (after the stop); 9 ends the walk
+    assert t == [3000, 4000, 5000, 7000, 8000] and o == [offs[i] for i in (3, 4, 5, 7, 8)]
+    assert (before, after) == (1, 1)
+    # no lines allowed outside the window: both edges trimmed
+    o, t, before, after = forecast(mds, 10, (100, 5, 4000), (100, 5, 7000), 0, 0)
+    assert t == [4000, 5000, 7000] and (before, after) == (0, 0)
+    # generous allowances keep every echo and calibration record from the first echo on
+    o, t, before, after = forecast(mds, 10, (100, 5, 4000), (100, 5, 7000), 10, 10)
+    assert t == [2000, 3000, 4000, 5000, 7000, 8000, 9000] and (before, after) == (2, 2)
+
+
+def test_asar_forecast_drops_leading_calibration_after_trim():
+    rng = np.random.default_rng(15)
+    flags = [ECHO, CAL, ECHO, ECHO, ECHO]
+    mds, offs = build_mds(rng, flags)
+    # before the start: records 0 (echo) and 1 (cal); one allowed -> record 0 goes, then the leading calibration
+    # record 1 goes too and the kept count before the start drops to 0
+    o, t, before, after = forecast(mds, 5, (100, 5, 2000), (100, 5, 9000), 1, 0)
+    assert t == [2000, 3000, 4000] and o == offs[2:] and (before, after) == (0, 0)
+    with pytest.raises(L.AsarfError):
+        forecast(mds, 5, (100, 5, 2000), (100, 5, 9000), 9, 9, cap=2)
+    with pytest.raises(L.AsarfError):
+        forecast(mds[:-10], 6, (100, 5, 2000), (100, 5, 9000), 9, 9)
