@@ -460,3 +460,81 @@ extern "C" int asarf_asar_forecast(const unsigned char* mds, long mds_bytes, lon
     *after_stop = after;
     return clear_error();
 }
+
+extern "C" int asarf_ers_forecast(const unsigned char* mds, long mds_bytes, long num_dsr, const asarf_mjd* start,
+                                  const asarf_mjd* stop, const asarf_patc_reference* patc, long* before_start,
+                                  long* after_stop, long* offsets, asarf_mjd* times, long capacity, long* n_out) {
+    // FetchErsL0ImForecastMeta (ers_im_parse.cc:228-331): walk the num_dsr 11498-byte records; each ISP time is
+    // SBT-corrected (asarf_adjust_isp_sensing_time; overflow = SBT below the first record's, as the reference
+    // compares with the SBT it read before the loop); a data-record-number gap of g > 1 repeats the previous entry
+    // g - 1 times (at most 900 missing records), counted before / after the window by the previous entry's time;
+    // the walk stops once more than *after_stop records lie after the stop; the surplus before the start beyond
+    // *before_start is dropped, and with no allowance (*before_start == 0) every leading entry sensed before the
+    // start too. On return *before_start / *after_stop hold the counts kept.
+    constexpr long kDsr = 11498;
+    if (!mds || !start || !stop || !before_start || !after_stop || !offsets || !times || !n_out)
+        return set_error(ASARF_EINVAL, "asarf_ers_forecast: null");
+    if (num_dsr <= 0 || mds_bytes < num_dsr * kDsr)
+        return set_error(ASARF_ESHAPE, "asarf_ers_forecast: %ld records of 11498 bytes exceed %ld bytes", num_dsr,
+                         mds_bytes);
+    const long before_in = *before_start, after_in = *after_stop;
+    const uint32_t dr_first = be32(mds + 32);
+    uint32_t last_dr = dr_first != 0 ? dr_first - 1 : 0xFFFFFFFFu;
+    const uint32_t first_sbt = be32(mds + 48);
+    std::vector<long> off;
+    std::vector<asarf_mjd> tim;
+    long before = 0, after = 0;
+    for (long i = 0; i < num_dsr; ++i) {
+        const unsigned char* p = mds + i * kDsr;
+        if (be16(p + 24) != 11465)
+            return set_error(ASARF_EINVAL, "asarf_ers_forecast: record %ld ISP length is not 11465", i);
+        asarf_mjd t = be_mjd(p);
+        const uint32_t dr = be32(p + 32), sbt = be32(p + 48);
+        const int st = asarf_adjust_isp_sensing_time(patc, &t, sbt, sbt < first_sbt);
+        if (st != ASARF_OK) return st;
+        const unsigned long long gap = asarf_counter_gap(last_dr, dr, 0xFFFFFFFFull);
+        if (gap > 1) {
+            if (off.empty())
+                return set_error(ASARF_EINVAL, "asarf_ers_forecast: record number gap before any record was kept");
+            const unsigned long long fill = gap - 1;
+            if (fill > 900)
+                return set_error(ASARF_EINVAL, "asarf_ers_forecast: %llu records missing before record %ld (over 900)",
+                                 fill, i);
+            const long prev_off = off.back();
+            const asarf_mjd prev_t = tim.back();
+            for (unsigned long long k = 0; k < fill; ++k) {
+                off.push_back(prev_off);
+                tim.push_back(prev_t);
+            }
+            if (asarf_mjd_compare(&prev_t, start) < 0) before += (long)fill;
+            if (asarf_mjd_compare(&prev_t, stop) > 0) after += (long)fill;
+        } else {
+            if (asarf_mjd_compare(&t, start) < 0) ++before;
+            if (asarf_mjd_compare(&t, stop) > 0) ++after;
+        }
+        if (after > after_in) {
+            --after;
+            break;
+        }
+        last_dr = dr;
+        off.push_back(i * kDsr);
+        tim.push_back(t);
+    }
+    size_t first = 0;
+    if (before > before_in) {
+        first = (size_t)(before - before_in);
+        before = before_in;
+        if (before_in == 0)
+            while (first < off.size() && asarf_mjd_compare(&tim[first], start) < 0) ++first;
+    }
+    const long n = (long)(off.size() - std::min(first, off.size()));
+    if (n > capacity) return set_error(ASARF_EWORKSPACE, "asarf_ers_forecast: %ld entries > capacity %ld", n, capacity);
+    for (long k = 0; k < n; ++k) {
+        offsets[k] = off[first + k];
+        times[k] = tim[first + k];
+    }
+    *n_out = n;
+    *before_start = before;
+    *after_stop = after;
+    return clear_error();
+}

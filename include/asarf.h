@@ -540,6 +540,14 @@ int asarf_asar_forecast(const unsigned char* mds, long mds_bytes, long num_dsr, 
                         const asarf_mjd* stop, long* before_start, long* after_stop, long* offsets, asarf_mjd* times,
                         long capacity, long* n_out);
 
+/* FetchErsL0ImForecastMeta (ers_im_parse.cc:228-331): the ERS IM records (11498 bytes each) to focus between
+ * start and stop, ISP times SBT-corrected with patc (NULL: as recorded), a gap in the data record number filled by
+ * repeating the previous entry (at most 900 missing records; offsets repeat). In/out counts as in
+ * asarf_asar_forecast. */
+int asarf_ers_forecast(const unsigned char* mds, long mds_bytes, long num_dsr, const asarf_mjd* start,
+                       const asarf_mjd* stop, const asarf_patc_reference* patc, long* before_start, long* after_stop,
+                       long* offsets, asarf_mjd* times, long capacity, long* n_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -240,13 +240,13 @@ def test_asar_packet_errors_and_non_echo():
 
 
 # -------------------------------------------------------------------------------------- ERS IM packets ----
-def ers_packet(rng, sensing, sbt, swst=700, isp_length=11465, aux=0xAA):
+def ers_packet(rng, sensing, sbt, swst=700, isp_length=11465, aux=0xAA, dr_no=987654):
     """An 11498-byte data set record laid out as ers_im_parse.cc:366-548 reads it (big-endian fields)."""
     samples = rng.integers(0, 32, 11232, dtype=np.uint8).tobytes()
     p = bytearray(11498)
     p[0:12] = struct.pack(">iII", *sensing)
     p[24:26] = struct.pack(">H", isp_length)
-    p[32:36] = struct.pack(">I", 987654)
+    p[32:36] = struct.pack(">I", dr_no)
     p[36], p[37], p[46], p[47] = 17, 23, aux, 0x21
     p[48:52] = struct.pack(">I", sbt)
     p[52], p[53] = 0xAA, 0x11
@@ -423,3 +423,41 @@ def test_asar_forecast_drops_leading_calibration_after_trim():
         forecast(mds, 5, (100, 5, 2000), (100, 5, 9000), 9, 9, cap=2)
     with pytest.raises(L.AsarfError):
         forecast(mds[:-10], 6, (100, 5, 2000), (100, 5, 9000), 9, 9)
+
+
+# ------------------------------------------------------------------------------------- ERS packet forecast ----
+def ers_forecast(mds, n, start, stop, before, after, patc=None, cap=64):
+    b, a, cnt = ctypes.c_long(before), ctypes.c_long(after), ctypes.c_long()
+    offs = (ctypes.c_long * cap)()
+    times = (L.Mjd * cap)()
+    L.check(L.lib().asarf_ers_forecast(mds, len(mds), n, ctypes.byref(L.Mjd(*start)), ctypes.byref(L.Mjd(*stop)),
+                                       ctypes.byref(patc) if patc is not None else None, ctypes.byref(b),
+                                       ctypes.byref(a), offs, times, cap, ctypes.byref(cnt)))
+    return [o // 11498 for o in offs[:cnt.value]], [t.astuple() for t in times[:cnt.value]], b.value, a.value
+
+
+def test_ers_forecast_fills_missing_records_and_trims():
+    # ers_im_parse.cc:228-331 on six records 1 ms apart whose record numbers jump from 12 to 15: the two missing
+    # records repeat record 2 (counted by its time); record 3, which follows the gap, is not counted against the
+    # window (the reference counts only on the no-gap branch); window [2 ms, 4 ms]
+    rng = np.random.default_rng(16)
+    drs = [10, 11, 12, 15, 16, 17]
+    mds = b"".join(ers_packet(rng, (100, 5, 1000 * i), 7, dr_no=d)[0] for i, d in enumerate(drs))
+    idx, t, before, after = ers_forecast(mds, 6, (100, 5, 2000), (100, 5, 4000), 10, 10)
+    assert idx == [0, 1, 2, 2, 2, 3, 4, 5] and [x[2] for x in t] == [0, 1000, 2000, 2000, 2000, 3000, 4000, 5000]
+    assert (before, after) == (2, 1)
+    idx, t, before, after = ers_forecast(mds, 6, (100, 5, 2000), (100, 5, 4000), 0, 0)
+    assert idx == [2, 2, 2, 3, 4] and (before, after) == (0, 0)
+    # more than 900 missing records is an error
+    mds2 = b"".join(ers_packet(rng, (100, 5, 1000 * i), 7, dr_no=d)[0] for i, d in enumerate([1, 2, 905]))
+    with pytest.raises(L.AsarfError):
+        ers_forecast(mds2, 3, (100, 5, 0), (100, 5, 9000), 9, 9)
+
+
+def test_ers_forecast_sbt_correction():
+    # the TDS01 PATC of ers_sbt_test.cc: the recorded times are replaced by the SBT-derived ones
+    rng = np.random.default_rng(17)
+    ref = register(15236, 77966950, 1440813490, 3906249)
+    mds = b"".join(ers_packet(rng, (1, 2, 3), sbt, dr_no=d)[0] for d, sbt in ((5, 1440813490), (6, 1440813493)))
+    idx, t, _, _ = ers_forecast(mds, 2, (-4000, 0, 0), (0, 0, 0), 9, 9, patc=ref)
+    assert idx == [0, 1] and t == [(-3026, 77966, 950000), (-3026, 77966, 961718)]
