@@ -198,6 +198,7 @@ SIGNATURES = {
     "asarf_parse_ers_im_packet": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(ErsEcho), ctypes.c_void_p]),
     "asarf_header_get": (_ci, [ctypes.c_char_p, ctypes.c_long, ctypes.c_char_p, ctypes.c_char_p, _ci]),
     "asarf_level0_mdsr": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(Dsd)]),
+    "asarf_ins_fbaq4_tables": (_ci, [ctypes.c_char_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]),
     "asarf_ers_forecast": (_ci, [ctypes.c_char_p, ctypes.c_long, ctypes.c_long, _P(Mjd), _P(Mjd), _P(PatcReference),
                                  _P(ctypes.c_long), _P(ctypes.c_long), _P(ctypes.c_long), _P(Mjd), ctypes.c_long,
                                  _P(ctypes.c_long)]),

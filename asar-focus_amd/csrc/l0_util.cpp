@@ -538,3 +538,21 @@ extern "C" int asarf_ers_forecast(const unsigned char* mds, long mds_bytes, long
     *after_stop = after;
     return clear_error();
 }
+
+extern "C" int asarf_ins_fbaq4_tables(const unsigned char* ins, long n_bytes, float* i_lut, float* q_lut) {
+    // FindINSFile (envisat_aux_file.cc:67-95) and FixedBaselineParameters (envisat_aux_file.h:143-149): a 173273-byte
+    // ASAR instrument characterisation file = 1247-byte MPH + 378-byte SPH + InstrumentFile, whose 89408-byte
+    // FixedBaselineParameters follow the 72608-byte FixedLifetimeParameters and hold, big-endian, the 8-bit I and
+    // Q tables (256 floats each), then i_LUT_fbaq4 (4096), i_LUT_fbaq3 (2048), i_LUT_fbaq2 (1024), q_LUT_fbaq4 (4096)
+    constexpr long kInsSize = 173273, kFbp = 1247 + 378 + 72608;
+    constexpr long kILut = kFbp + 4 * (256 + 256), kQLut = kILut + 4 * (4096 + 2048 + 1024);
+    if (!ins || !i_lut || !q_lut) return set_error(ASARF_EINVAL, "asarf_ins_fbaq4_tables: null");
+    if (n_bytes != kInsSize)
+        return set_error(ASARF_ESHAPE, "asarf_ins_fbaq4_tables: %ld bytes, an instrument file has 173273", n_bytes);
+    for (int k = 0; k < 4096; ++k) {
+        const uint32_t wi = be32(ins + kILut + 4 * k), wq = be32(ins + kQLut + 4 * k);
+        std::memcpy(&i_lut[k], &wi, 4);
+        std::memcpy(&q_lut[k], &wq, 4);
+    }
+    return clear_error();
+}

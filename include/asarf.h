@@ -548,6 +548,10 @@ int asarf_ers_forecast(const unsigned char* mds, long mds_bytes, long num_dsr, c
                        const asarf_mjd* stop, const asarf_patc_reference* patc, long* before_start, long* after_stop,
                        long* offsets, asarf_mjd* times, long capacity, long* n_out);
 
+/* The FBAQ-4 reconstruction tables of an ASAR instrument characterisation file (ASA_INS_AX, 173273 bytes;
+ * envisat_aux_file.cc:67-95, envisat_aux_file.h:143-149): 4096 I and 4096 Q floats for asarf_ingest_asar_fbaq4. */
+int asarf_ins_fbaq4_tables(const unsigned char* ins, long n_bytes, float* i_lut, float* q_lut);
+
 #ifdef __cplusplus
 }
 #endif

@@ -461,3 +461,25 @@ def test_ers_forecast_sbt_correction():
     mds = b"".join(ers_packet(rng, (1, 2, 3), sbt, dr_no=d)[0] for d, sbt in ((5, 1440813490), (6, 1440813493)))
     idx, t, _, _ = ers_forecast(mds, 2, (-4000, 0, 0), (0, 0, 0), 9, 9, patc=ref)
     assert idx == [0, 1] and t == [(-3026, 77966, 950000), (-3026, 77966, 961718)]
+
+
+# ------------------------------------------------------------------------------------- instrument file ----
+def test_ins_fbaq4_tables():
+    # offsets from the reference's layout (envisat_aux_file.h): MPH 1247 + SPH 378, FixedLifetimeParameters 72608,
+    # then in FixedBaselineParameters i_LUT_8bit[256], q_LUT_8bit[256], i_LUT_fbaq4[4096], i_LUT_fbaq3[2048],
+    # i_LUT_fbaq2[1024], q_LUT_fbaq4[4096]; big-endian floats
+    rng = np.random.default_rng(18)
+    buf = bytearray(rng.integers(0, 256, 173273, dtype=np.uint8).tobytes())
+    i_want = rng.standard_normal(4096).astype(np.float32)
+    q_want = rng.standard_normal(4096).astype(np.float32)
+    fbp = 1247 + 378 + 72608
+    i_at = fbp + 4 * 512
+    q_at = i_at + 4 * (4096 + 2048 + 1024)
+    buf[i_at:i_at + 16384] = i_want.astype(">f4").tobytes()
+    buf[q_at:q_at + 16384] = q_want.astype(">f4").tobytes()
+    assert q_at + 16384 <= 1247 + 378 + 72608 + 89408  # inside FixedBaselineParameters
+    i_lut, q_lut = np.zeros(4096, np.float32), np.zeros(4096, np.float32)
+    L.check(L.lib().asarf_ins_fbaq4_tables(bytes(buf), len(buf), i_lut.ctypes.data, q_lut.ctypes.data))
+    assert np.array_equal(i_lut, i_want) and np.array_equal(q_lut, q_want)
+    assert L.lib().asarf_ins_fbaq4_tables(bytes(buf[:-1]), len(buf) - 1, i_lut.ctypes.data,
+                                          q_lut.ctypes.data) == L.ESHAPE
