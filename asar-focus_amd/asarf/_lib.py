@@ -131,6 +131,11 @@ class Dsd(ctypes.Structure):
                 ("dsr_size", ctypes.c_ulonglong)]
 
 
+class TiePoints(ctypes.Structure):
+    _fields_ = [("samp_numbers", ctypes.c_uint * 11), ("slant_range_times", ctypes.c_float * 11),
+                ("angles", ctypes.c_float * 11), ("lats", ctypes.c_int * 11), ("longs", ctypes.c_int * 11)]
+
+
 class ResultsWindow(ctypes.Structure):
     _fields_ = [("lines_before_sensing_start", ctypes.c_long), ("lines_after_sensing_start", ctypes.c_long),
                 ("lines_before_sensing_stop", ctypes.c_long), ("lines_after_sensing_stop", ctypes.c_long),
@@ -185,6 +190,7 @@ SIGNATURES = {
     "asarf_xyz_to_geo": (_ci, [_P(_cd), _P(_cd)]),
     "asarf_scene_center": (_ci, [_P(Geometry), _P(_cd), _P(_cd), _P(_cd)]),
     "asarf_aperture_pixels": (_ci, [_P(Geometry), _P(_ci), _ci, _P(_ci)]),
+    "asarf_geolocation_tie_points": (_ci, [_P(Geometry), _ci, ctypes.c_void_p, _P(TiePoints)]),
     "asarf_mjd_compare": (_ci, [_P(Mjd), _P(Mjd)]),
     "asarf_mjd_add_micros": (_ci, [_P(Mjd), ctypes.c_longlong, _P(Mjd)]),
     "asarf_counter_gap": (ctypes.c_ulonglong, [ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong]),

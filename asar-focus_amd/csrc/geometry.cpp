@@ -331,3 +331,33 @@ extern "C" int asarf_results_window(double dc_constant, long lines_before_start,
     *out = w;
     return clear_error();
 }
+
+extern "C" int asarf_geolocation_tie_points(const asarf_geometry* g, int az_idx, const double* grsr5,
+                                            asarf_tie_points* out) {
+    // FillGeoLocationAds (envisat_lvl1_writer.cc:277-324), one line: 11 tie points at range samples
+    // i * range_size / 10; slant range R_first + sample * dr, or with the slant-to-ground polynomial applied
+    // (grsr5 non-NULL; dr is then the ground spacing) R_first + grsr(sample * dr); the zero-Doppler point at the
+    // line's time from the geometry's centre point; sample number + 1, two-way slant time [ns], incidence angle
+    // [deg] and latitude / longitude in microdegrees (truncated toward zero into int32)
+    ST_CHECK(check_geometry(g, "asarf_geolocation_tie_points"));
+    if (!out) return set_error(ASARF_EINVAL, "asarf_geolocation_tie_points: null");
+    asarf_osv o{};
+    ST_CHECK(interpolate(g->osv, g->n_osv, line_time(g, az_idx), &o));
+    asarf_tie_points r{};
+    for (int i = 0; i < 11; ++i) {
+        const int samp = i * g->range_size / 10;
+        double R = g->slant_range_first + (double)samp * g->range_spacing;
+        if (grsr5) R = g->slant_range_first + polyval(grsr5, 5, (double)samp * g->range_spacing);
+        double xyz[3];
+        ST_CHECK(geolocate(o.pos, o.vel, g->center_point, R, xyz));
+        double ll[2];
+        ST_CHECK(asarf_xyz_to_geo(xyz, ll));
+        r.samp_numbers[i] = (unsigned)(samp + 1);
+        r.slant_range_times[i] = (float)(2.0 * (R / kSpeedOfLight) * 1e9);
+        r.angles[i] = (float)asarf_incidence_angle(xyz, o.pos);
+        r.lats[i] = (int)(ll[0] * 1e6);
+        r.longs[i] = (int)(ll[1] * 1e6);
+    }
+    *out = r;
+    return clear_error();
+}

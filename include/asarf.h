@@ -249,6 +249,18 @@ int asarf_xyz_to_geo(const double* xyz, double* out_lat_lon);
  * field of geometry except vr_poly, dc_poly and center_point. */
 int asarf_scene_center(const asarf_geometry* geometry, const double* guess_xyz, double* out_center_xyz,
                        double* azimuth_spacing);
+/* Tie points of one line of the geolocation grid ADS (envisat_lvl1_ads.h:640-646). */
+typedef struct {
+    unsigned samp_numbers[11];
+    float slant_range_times[11]; /* two-way, ns */
+    float angles[11];            /* incidence, deg */
+    int lats[11], longs[11];     /* microdegrees */
+} asarf_tie_points;
+/* FillGeoLocationAds (envisat_lvl1_writer.cc:277-324) for the line az_idx: 11 points at range samples
+ * i * range_size / 10, slant range from range_spacing, or (grsr5 != NULL) from the slant-to-ground polynomial of
+ * the ground range sample * range_spacing. */
+int asarf_geolocation_tie_points(const asarf_geometry* geometry, int az_idx, const double* grsr5,
+                                 asarf_tie_points* out);
 /* CalcAperturePixels (sar/sar_metadata.h:117-141): the synthetic aperture in lines at the range indices 0, step, ...
  * below range_size - chirp_samples (step = that / 10) from vr_poly and dc_poly; *n_out entries (10 or 11) are
  * written, ASARF_EINVAL when capacity is smaller. */

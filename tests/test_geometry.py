@@ -348,3 +348,32 @@ def test_results_window_cases(dc, before, after, aperture, want_after_start, wan
     assert w["near_range_pixels"] == 8 and w["far_range_pixels"] == 8
     with pytest.raises(L.AsarfError):
         focus.results_window(dc, -1, after, aperture, 16)
+
+
+# ------------------------------------------------------------------------------- geolocation grid tie points ----
+def test_geolocation_tie_points_equatorial_closed_form():
+    # line 1000 of the straight-line scene is sensed at t = 0 with the sensor at (rs, 0, 0) moving along +z: every
+    # tie point lies on the equator (latitude 0) at x = (rs^2 + a^2 - R^2) / (2 rs), y > 0
+    import ctypes
+    rs, vz = A + 790e3, 7450.0
+    ts = np.array([-10.0, 10.0])
+    g = scene_geometry(ts, [np.array([rs, 0.0, vz * t]) for t in ts], [np.array([0.0, 0.0, vz])] * 2,
+                       first_line_time=-1.0, line_time_interval=0.001, azimuth_size=2000,
+                       center_point=[A * 0.99, 3e5, 0.0])
+    s, keep = focus.geometry(g)
+    for grsr, scale in ((None, 1.0), ([0.0, 0.0, 0.0, 1.0, 0.0], 1.0), ([0.0, 0.0, 0.0, 2.0, 0.0], 2.0)):
+        tp = L.TiePoints()
+        gp = (ctypes.c_double * 5)(*grsr) if grsr else None
+        L.check(L.lib().asarf_geolocation_tie_points(ctypes.byref(s), 1000, gp, ctypes.byref(tp)))
+        for i in range(11):
+            samp = i * g["range_size"] // 10
+            r = g["slant_range_first"] + scale * samp * g["range_spacing"]
+            x = (rs * rs + A * A - r * r) / (2 * rs)
+            y = math.sqrt(A * A - x * x)
+            assert tp.samp_numbers[i] == samp + 1
+            assert tp.lats[i] == 0
+            assert abs(tp.longs[i] - int(math.degrees(math.atan2(y, x)) * 1e6)) <= 1
+            assert tp.slant_range_times[i] == np.float32(2 * r / po.SPEED_OF_LIGHT * 1e9)
+            inc = math.degrees(math.acos((rs * rs - A * A - r * r) / (2 * A * r)))
+            assert abs(tp.angles[i] - inc) < 1e-4
+    del keep
