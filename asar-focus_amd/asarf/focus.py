@@ -308,8 +308,9 @@ def subset_copy(ctx: Context, img: L.Image, x0: int, y0: int, w: int, h: int, ds
 
 
 def focus_params(cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
-                 orbit_height: float | None = None):
-    """asarf_focus_params of a scene configuration (and the SRGR fit when imp)."""
+                 orbit_height: float | None = None, lines_before: int = 0, lines_after: int = 0):
+    """asarf_focus_params of a scene configuration (and the SRGR fit when imp). lines_before / lines_after are the
+    lines cached before the sensing start and after the sensing stop (main.cc:296-298)."""
     p = L.FocusParams()
     p.na, p.nr, p.nc = cfg.na, cfg.nr, cfg.nc
     p.nrp = cfg.nrp or padding_size(cfg.nr + cfg.nc)
@@ -322,6 +323,7 @@ def focus_params(cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool =
     p.n_total_samples = 0
     p.quirks = quirks
     p.imp = int(imp)
+    p.lines_before, p.lines_after = int(lines_before), int(lines_after)
     grsr, ng = None, None
     if imp:
         near = 8
@@ -342,10 +344,11 @@ class Plan:
     """Fused focusing plan (the GPU stage of main.cc:233-327) for one scene configuration."""
 
     def __init__(self, ctx: Context, cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
-                 chirp: np.ndarray | None = None, orbit_height: float | None = None):
+                 chirp: np.ndarray | None = None, orbit_height: float | None = None, lines_before: int = 0,
+                 lines_after: int = 0):
         self.ctx = ctx
         self.cfg = cfg
-        p, self.grsr, self.ng = focus_params(cfg, quirks, imp, orbit_height)
+        p, self.grsr, self.ng = focus_params(cfg, quirks, imp, orbit_height, lines_before, lines_after)
         self.nrv = cfg.nr - cfg.nc
         self.params = p
         self.chirp = generate_chirp(cfg, p.nrp) if chirp is None else np.ascontiguousarray(chirp, np.complex64)

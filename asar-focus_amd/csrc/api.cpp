@@ -71,13 +71,13 @@ int ctx_init(asarf_ctx* c) {
     int lds = 0;
     HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     c->max_lds = (size_t)lds;
-    set_max_lds_bytes(c->max_lds);
     // ASARF_COL_STATIC=0 keeps the runtime-radix column kernels for every length (comparison runs)
     const char* cs_env = std::getenv("ASARF_COL_STATIC");
-    set_col_static_enabled(!(cs_env && cs_env[0] == '0'));
+    c->col_static = !(cs_env && cs_env[0] == '0');
     // ASARF_COL_WIDTH=16|32 forces the column tile width of the compile-time column kernels (default: by pass)
     const char* cw_env = std::getenv("ASARF_COL_WIDTH");
-    set_col_static_width(cw_env ? std::atoi(cw_env) : 0);
+    const int cw = cw_env ? std::atoi(cw_env) : 0;
+    c->col_width = (cw == 16 || cw == 32) ? cw : 0;
     c->initialized = true;
     return ASARF_OK;
 }
@@ -597,10 +597,12 @@ extern "C" int asarf_range_compression(asarf_ctx* c, asarf_image* img, const flo
 }
 
 namespace asarf {
-ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stride, int ncols, int groups, long in_g,
-                    long in_m, int in_rows, long out_g, long out_m, int out_rows, const float2* twf, float scale,
-                    const FftPlanHost* sub) {
+ColFftArgs col_args(const asarf_ctx* c, const float2* in, long in_stride, float2* out, long out_stride, int ncols,
+                    int groups, long in_g, long in_m, int in_rows, long out_g, long out_m, int out_rows,
+                    const float2* twf, float scale, const FftPlanHost* sub) {
     ColFftArgs a{};
+    a.no_static = c->col_static ? 0 : 1;
+    a.col_width = c->col_width;
     a.in = in;
     a.in_stride = in_stride;
     a.out = out;
@@ -636,18 +638,18 @@ int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* i
     const int n = cs.n;
     if (cs.n2 == 1) {
         if (pass == 0) {
-            const ColFftArgs a = col_args(in, in_stride, img, stride, ncols, 1, 0, 1, in_rows, 0, 1, n, nullptr, 1.f,
+            const ColFftArgs a = col_args(c, in, in_stride, img, stride, ncols, 1, 0, 1, in_rows, 0, 1, n, nullptr, 1.f,
                                           cs.p1);
             HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
         }
         return ASARF_OK;
     }
     if (pass == 0) {
-        const ColFftArgs a = col_args(in, in_stride, img, stride, ncols, cs.n2, 1, cs.n2, in_rows, 1, cs.n2, n,
+        const ColFftArgs a = col_args(c, in, in_stride, img, stride, ncols, cs.n2, 1, cs.n2, in_rows, 1, cs.n2, n,
                                       cs.full->d_tw, 1.f, cs.p1);
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
     } else {
-        const ColFftArgs a = col_args(in, in_stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr, 1.f,
+        const ColFftArgs a = col_args(c, in, in_stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr, 1.f,
                                       cs.p2);
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
     }
@@ -661,9 +663,9 @@ int col_inverse_pass_to(asarf_ctx* c, const ColSplit& cs, int pass, const float2
     const int n = cs.n;
     if (cs.n2 == 1) return set_error(ASARF_EINVAL, "col_inverse_pass_to: no four-step split");
     const ColFftArgs a =
-        pass == 0 ? col_args(in, in_stride, out, out_stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, cs.full->d_tw,
+        pass == 0 ? col_args(c, in, in_stride, out, out_stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, cs.full->d_tw,
                              1.f, cs.p2)
-                  : col_args(in, in_stride, out, out_stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows, nullptr,
+                  : col_args(c, in, in_stride, out, out_stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows, nullptr,
                              scale, cs.p1);
     HIP_CHECK(launch_col_fft(a, true, c->stream));
     return ASARF_OK;
@@ -676,18 +678,18 @@ int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, lo
     const int n = cs.n;
     if (cs.n2 == 1) {
         if (pass == 0) {
-            const ColFftArgs a = col_args(img, stride, img, stride, ncols, 1, 0, 1, n, 0, 1, out_rows, nullptr, scale,
+            const ColFftArgs a = col_args(c, img, stride, img, stride, ncols, 1, 0, 1, n, 0, 1, out_rows, nullptr, scale,
                                           cs.p1);
             HIP_CHECK(launch_col_fft(with_mfma(mfma, a), true, c->stream));
         }
         return ASARF_OK;
     }
     if (pass == 0) {
-        const ColFftArgs a = col_args(img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, cs.full->d_tw,
+        const ColFftArgs a = col_args(c, img, stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, cs.full->d_tw,
                                       1.f, cs.p2);
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), true, c->stream));
     } else {
-        const ColFftArgs a = col_args(img, stride, img, stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows, nullptr,
+        const ColFftArgs a = col_args(c, img, stride, img, stride, ncols, cs.n2, 1, cs.n2, n, 1, cs.n2, out_rows, nullptr,
                                       scale, cs.p1);
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), true, c->stream));
     }
@@ -1055,7 +1057,7 @@ extern "C" int asarf_fft_c2c(asarf_ctx* c, void* data, int n, int batch, long di
     ColSplit cs;
     ST_CHECK(get_col_split(c, n, &cs));
     if (cs.n2 == 1) {
-        HIP_CHECK(launch_col_fft(col_args(d, dist, d, dist, batch, 1, 0, 1, n, 0, 1, n, nullptr, scale, cs.p1), inv,
+        HIP_CHECK(launch_col_fft(col_args(c, d, dist, d, dist, batch, 1, 0, 1, n, 0, 1, n, nullptr, scale, cs.p1), inv,
                                  c->stream));
         return clear_error();
     }
@@ -1065,17 +1067,17 @@ extern "C" int asarf_fft_c2c(asarf_ctx* c, void* data, int n, int batch, long di
     auto* t = static_cast<float2*>(scratch);
     const long ts = batch;
     if (!inv) {
-        HIP_CHECK(launch_col_fft(col_args(d, dist, t, ts, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, cs.full->d_tw, 1.f,
+        HIP_CHECK(launch_col_fft(col_args(c, d, dist, t, ts, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, cs.full->d_tw, 1.f,
                                           cs.p1),
                                  false, c->stream));
-        HIP_CHECK(launch_col_fft(col_args(t, ts, d, dist, batch, cs.n1, cs.n2, 1, n, 1, cs.n1, n, nullptr, scale,
+        HIP_CHECK(launch_col_fft(col_args(c, t, ts, d, dist, batch, cs.n1, cs.n2, 1, n, 1, cs.n1, n, nullptr, scale,
                                           cs.p2),
                                  false, c->stream));
     } else {
-        HIP_CHECK(launch_col_fft(col_args(d, dist, t, ts, batch, cs.n1, 1, cs.n1, n, cs.n2, 1, n, cs.full->d_tw, 1.f,
+        HIP_CHECK(launch_col_fft(col_args(c, d, dist, t, ts, batch, cs.n1, 1, cs.n1, n, cs.n2, 1, n, cs.full->d_tw, 1.f,
                                           cs.p2),
                                  true, c->stream));
-        HIP_CHECK(launch_col_fft(col_args(t, ts, d, dist, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, nullptr, scale,
+        HIP_CHECK(launch_col_fft(col_args(c, t, ts, d, dist, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, nullptr, scale,
                                           cs.p1),
                                  true, c->stream));
     }
@@ -1118,7 +1120,7 @@ extern "C" int asarf_fft_c2c_split_bf16(asarf_ctx* c, void* data, int n, int bat
         int st = ASARF_OK;
         const FftPlanHost* p = get_plan(c, n, kThreads / kColTile, &st);
         if (!p) return st;
-        ColFftArgs a = col_args(d, dist, d, dist, batch, 1, 0, 1, n, 0, 1, n, nullptr, scale, p);
+        ColFftArgs a = col_args(c, d, dist, d, dist, batch, 1, 0, 1, n, 0, 1, n, nullptr, scale, p);
         a.mfma = 1;
         HIP_CHECK(launch_col_fft(a, inv, c->stream));
         return clear_error();
@@ -1135,11 +1137,11 @@ extern "C" int asarf_fft_c2c_split_bf16(asarf_ctx* c, void* data, int n, int bat
     const long ts = batch;
     ColFftArgs a1, a2;
     if (!inv) {
-        a1 = col_args(d, dist, t, ts, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, cs.full->d_tw, 1.f, cs.p1);
-        a2 = col_args(t, ts, d, dist, batch, cs.n1, cs.n2, 1, n, 1, cs.n1, n, nullptr, scale, cs.p2);
+        a1 = col_args(c, d, dist, t, ts, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, cs.full->d_tw, 1.f, cs.p1);
+        a2 = col_args(c, t, ts, d, dist, batch, cs.n1, cs.n2, 1, n, 1, cs.n1, n, nullptr, scale, cs.p2);
     } else {
-        a1 = col_args(d, dist, t, ts, batch, cs.n1, 1, cs.n1, n, cs.n2, 1, n, cs.full->d_tw, 1.f, cs.p2);
-        a2 = col_args(t, ts, d, dist, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, nullptr, scale, cs.p1);
+        a1 = col_args(c, d, dist, t, ts, batch, cs.n1, 1, cs.n1, n, cs.n2, 1, n, cs.full->d_tw, 1.f, cs.p2);
+        a2 = col_args(c, t, ts, d, dist, batch, cs.n2, 1, cs.n2, n, 1, cs.n2, n, nullptr, scale, cs.p1);
     }
     a1.mfma = a2.mfma = 1;
     HIP_CHECK(launch_col_fft(a1, inv, c->stream));

@@ -52,6 +52,9 @@ struct asarf_ctx {
     bool own_stream = false;
     bool initialized = false;
     size_t max_lds = 65536;
+    // column-transform knobs, read from the environment at initialisation (ASARF_COL_STATIC, ASARF_COL_WIDTH)
+    bool col_static = true;
+    int col_width = 0;
     // plan caches keyed by (length, lanes per transform)
     std::map<std::pair<int, int>, std::unique_ptr<asarf::FftPlanHost>> plans;
     std::map<int, float2*> tw_tables;
@@ -81,9 +84,9 @@ float2* get_twiddles(asarf_ctx* ctx, int n, int* status);
 int get_col_split(asarf_ctx* ctx, int n, ColSplit* out, int n2_max = 0);
 // Column-transform launch arguments: element m of group g is read from row g*in_g + m*in_m, element k written
 // to row g*out_g + k*out_m; twf (nullptr = none) is the inter-pass twiddle table.
-ColFftArgs col_args(const float2* in, long in_stride, float2* out, long out_stride, int ncols, int groups, long in_g,
-                    long in_m, int in_rows, long out_g, long out_m, int out_rows, const float2* twf, float scale,
-                    const FftPlanHost* sub);
+ColFftArgs col_args(const asarf_ctx* c, const float2* in, long in_stride, float2* out, long out_stride, int ncols,
+                    int groups, long in_g, long in_m, int in_rows, long out_g, long out_m, int out_rows,
+                    const float2* twf, float scale, const FftPlanHost* sub);
 bool radix_plan(int n, int lanes, std::vector<int>& out);
 bool radix_plan_relaxed(int n, int lanes, std::vector<int>& out);
 bool radix_plan_fits(int n, int lanes, const std::vector<int>& radix);

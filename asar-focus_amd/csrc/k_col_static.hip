@@ -76,11 +76,6 @@ __global__ __launch_bounds__(16 * COLS) void k_col_static(ColFftArgs a) {
     pass_visit<RL>(v, lane, L, N, out_write);
 }
 
-// Columns per workgroup: 0 = by access pattern (32 where the transform's elements are strided rows, as in passes
-// A and A' of the four-step split, so each row access is 256 contiguous bytes; 16 for contiguous sub-transforms,
-// whose 16-column tiles measured faster), else forced to 16 or 32.
-static int g_col_width = 0;
-void set_col_static_width(int cols) { g_col_width = (cols == 16 || cols == 32) ? cols : 0; }
 
 template <int N, class Seq, int COLS>
 static hipError_t launch_cs_w(const ColFftArgs& a, bool inverse, hipStream_t s) {
@@ -96,9 +91,12 @@ static hipError_t launch_cs_w(const ColFftArgs& a, bool inverse, hipStream_t s) 
     return hipGetLastError();
 }
 
+// Columns per workgroup: ColFftArgs::col_width, or by access pattern when 0 (32 where the transform's elements are
+// strided rows, as in passes A and A' of the four-step split, so each row access is 256 contiguous bytes; 16 for
+// contiguous sub-transforms, whose 16-column tiles measured faster).
 template <int N, class Seq>
 static hipError_t launch_cs(const ColFftArgs& a, bool inverse, hipStream_t s) {
-    const int w = g_col_width ? g_col_width : (a.in_m > 1 ? 32 : 16);
+    const int w = (a.col_width == 16 || a.col_width == 32) ? a.col_width : (a.in_m > 1 ? 32 : 16);
     return w == 32 ? launch_cs_w<N, Seq, 32>(a, inverse, s) : launch_cs_w<N, Seq, 16>(a, inverse, s);
 }
 

@@ -8,13 +8,29 @@
 // sub-transform of length n <= 512 along the column; 16 lanes cooperate per column. Long azimuth transforms run
 // as two such passes (four-step decomposition N = N1*N2 with the inter-pass twiddle fused into the first pass's
 // store), so there is no transpose.
+#include <atomic>
+
 #include "kernels.h"
 
 namespace asarf {
 
-static size_t g_max_lds = 65536;
-void set_max_lds_bytes(size_t b) { g_max_lds = b; }
-size_t max_lds_bytes() { return g_max_lds; }
+constexpr int kMaxDevices = 64;
+
+// LDS bytes a workgroup may allocate on the current device (queried once per device; 64 KiB if the query fails)
+size_t max_lds_bytes() {
+    static std::atomic<size_t> cache[kMaxDevices];
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kMaxDevices) return 65536;
+    size_t v = cache[d].load(std::memory_order_relaxed);
+    if (v == 0) {
+        int lds = 0;
+        v = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, d) == hipSuccess && lds > 0
+                ? (size_t)lds
+                : 65536;
+        cache[d].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
 
 // LDS index with one pad slot every 32 elements (spreads power-of-two strides over banks).
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
@@ -70,7 +86,7 @@ static int plan_mask(const FftPlanDev& p) {
 hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s) {
     if (a.mfma) return launch_row_mfma(a, inverse, s);
     const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
-    if (lds > g_max_lds) return hipErrorInvalidValue;
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     const int m = plan_mask(a.plan);
 #define X_(SET)                                                                                          \
     if ((m & ~(SET)) == 0) {                                                                             \
@@ -177,7 +193,7 @@ __global__ __launch_bounds__(kRowThreads) void k_range_compress(RcArgs a) {
 hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
     if (a.mfma) return launch_rc_mfma(a, fmt, s);
     const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
-    if (lds > g_max_lds) return hipErrorInvalidValue;
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8) return hipErrorInvalidValue;
     hipError_t serr = hipSuccess;
     if (!a.no_static && launch_range_compress_static(a, fmt, s, &serr)) return serr;
@@ -257,15 +273,12 @@ __global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
 #undef VISIT_WR_
 }
 
-static bool g_col_static = true;
-void set_col_static_enabled(bool on) { g_col_static = on; }
-
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s) {
     if (a.mfma) return launch_col_mfma(a, inverse, s);
     hipError_t serr = hipSuccess;
-    if (g_col_static && launch_col_static(a, inverse, s, &serr)) return serr;
+    if (!a.no_static && launch_col_static(a, inverse, s, &serr)) return serr;
     const size_t lds = sizeof(float2) * (size_t)(a.sub.n * (kColTile + 1) + tw_lds_elems(a.sub.n));
-    if (lds > g_max_lds) return hipErrorInvalidValue;
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((long)a.ntiles * a.groups));
     const int m = plan_mask(a.sub);
 #define X_(SET)                                                                               \

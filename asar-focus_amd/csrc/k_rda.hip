@@ -14,9 +14,7 @@
 namespace asarf {
 
 // numpy.blackman(16) rounded to float (the table at sar/range_doppler_algorithm.cu:49-52 holds the same values).
-__constant__ float c_window[16];
-static bool g_window_ready = false;
-
+// The kernels take it by argument (RcmcArgs::window), so no device symbol has to be initialised per device.
 void rcmc_window(float w[16]) {
     for (int i = 0; i < 16; ++i) {
         const double a = 2.0 * kPi * (double)i / 15.0;
@@ -24,12 +22,10 @@ void rcmc_window(float w[16]) {
     }
 }
 
-static void ensure_window(hipStream_t s) {
-    if (g_window_ready) return;
-    float w[16];
-    rcmc_window(w);
-    (void)hipMemcpyToSymbol(HIP_SYMBOL(c_window), w, sizeof(w), 0, hipMemcpyHostToDevice);
-    g_window_ready = true;
+static RcmcArgs with_window(const RcmcArgs& in) {
+    RcmcArgs a = in;
+    rcmc_window(a.window);
+    return a;
 }
 
 __global__ void k_col_const(ColConstArgs a) {
@@ -134,7 +130,7 @@ __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(7)
     const double half = 0.5 * a.prf;
     float w[16];  // signed window (-1)^i w[i]
 #pragma unroll
-    for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
+    for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -a.window[i] : a.window[i];
     // rows strided over a bounded grid, so that the launch costs little when the gate sends the work elsewhere
     for (int q = blockIdx.x; q < a.nbins; q += gridDim.x) {
         const float2* img = a.img + (long)q * a.stride;
@@ -219,7 +215,7 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
     const double half = 0.5 * a.prf;
     float w[16];  // signed window (-1)^i w[i]
 #pragma unroll
-    for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -c_window[i] : c_window[i];
+    for (int i = 0; i < 16; ++i) w[i] = (i & 1) ? -a.window[i] : a.window[i];
     float2 pre0, pre1;
     auto fetch = [&](int q) {
         const float2* r = a.img + (long)q * a.stride;
@@ -291,8 +287,8 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
     }
 }
 
-hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s) {
-    ensure_window(s);
+hipError_t launch_rcmc_tile(const RcmcArgs& args, hipStream_t s) {
+    const RcmcArgs a = with_window(args);
     if (!a.out || a.out == a.img) return hipErrorInvalidValue;  // out of place only
     const dim3 grid((a.nrv + kRtCols - 1) / kRtCols, (a.nbins + kRtRows - 1) / kRtRows);
     switch (a.interp) {
@@ -304,8 +300,8 @@ hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s) {
-    ensure_window(s);
+hipError_t launch_rcmc_mf(const RcmcArgs& args, hipStream_t s) {
+    const RcmcArgs a = with_window(args);
     const size_t lds = sizeof(float2) * (size_t)a.nrv;
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_rcmc_mf, dim3(std::min(a.nbins, 2048)), dim3(kRcmcThreads), lds, s, a);

@@ -102,6 +102,8 @@ struct ColFftArgs {
     FftPlanDev sub;
     const int* fused_gate; // non-null: do nothing when *fused_gate <= kAzFusedReach (the fused kernel ran)
     int mfma;              // split-bfloat16 MFMA sub-transform (lengths 128 and 256 only)
+    int no_static;         // 1: runtime-radix kernel even where a compile-time kernel exists (comparison runs)
+    int col_width;         // columns per workgroup of the compile-time kernels: 16, 32, or 0 = by access pattern
 };
 
 struct RcmcArgs {
@@ -124,6 +126,7 @@ struct RcmcArgs {
     const unsigned char* lut_ok;  // [ns][judge_nd]
     int lut_nd, lut_ns, judge_nd;
     float window8[8];
+    float window[16];  // Blackman window of the 16-tap interpolator (filled by the launchers)
 };
 
 // Fused RCMC + matched filter + inverse pass B' (out of place; four-step split n1 x n2).
@@ -256,8 +259,6 @@ hipError_t launch_rc_mfma(const RcArgs& a, int raw_format, hipStream_t s);
 hipError_t launch_row_mfma(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_col_mfma(const ColFftArgs& a, bool inverse, hipStream_t s);
 bool launch_col_static(const ColFftArgs& a, bool inverse, hipStream_t s, hipError_t* err);
-void set_col_static_enabled(bool on);
-void set_col_static_width(int cols);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_az_fused(const AzFusedArgs& a, hipStream_t s);
@@ -283,7 +284,6 @@ hipError_t launch_condition(const OutputArgs& a, int detected, hipStream_t s);
 int fbaq4_index_host(int block, int code);
 
 // Max dynamic LDS bytes per workgroup the launchers may request (set by the context at initialisation).
-void set_max_lds_bytes(size_t bytes);
 size_t max_lds_bytes();
 
 }  // namespace asarf

@@ -315,15 +315,21 @@ namespace {
 constexpr long kMphSize = 1247, kSphSize = 1956, kDsdSize = 280;
 
 // extract_bytes / extract_num (envisat_mph_sph_parser.h:86-107): from the first digit 1-9 after the '=' (signs and
-// leading zeros are skipped, so "-0000000001" reads as 1 and an all-zero field as 0), the decimal number
-unsigned long long dsd_number(const char* p, long n) {
+// leading zeros are skipped, so "-0000000001" reads as 1 and an all-zero field as 0), the decimal number. False
+// when it does not fit 64 bits (the reference's std::stoul throws std::out_of_range there).
+bool dsd_number(const char* p, long n, unsigned long long* out) {
     long i = 0;
     while (i < n && p[i] != '=') ++i;
     for (; i < n; ++i)
         if (p[i] >= '1' && p[i] <= '9') break;
     unsigned long long v = 0;
-    for (; i < n && p[i] >= '0' && p[i] <= '9'; ++i) v = v * 10 + (unsigned long long)(p[i] - '0');
-    return v;
+    for (; i < n && p[i] >= '0' && p[i] <= '9'; ++i) {
+        const unsigned d = (unsigned)(p[i] - '0');
+        if (v > (~0ull - d) / 10) return false;
+        v = v * 10 + d;
+    }
+    *out = v;
+    return true;
 }
 
 // the idx-th '\n'-separated token of [p, p + n): its start and length (false when there are fewer tokens)
@@ -391,10 +397,15 @@ extern "C" int asarf_level0_mdsr(const unsigned char* file, long n_bytes, asarf_
         copy_text(r.ds_name, sizeof(r.ds_name), ln[0], len[0]);
         copy_text(r.ds_type, sizeof(r.ds_type), ln[1], len[1]);
         copy_text(r.filename, sizeof(r.filename), ln[2], len[2]);
-        r.ds_offset = dsd_number(ln[3], len[3]);
-        r.ds_size = dsd_number(ln[4], len[4]);
-        r.num_dsr = dsd_number(ln[5], len[5]);
-        r.dsr_size = dsd_number(ln[6], len[6]);
+        unsigned long long v[4];
+        for (int i = 0; i < 4; ++i)
+            if (!dsd_number(ln[3 + i], len[3 + i], &v[i]))
+                return set_error(ASARF_EINVAL, "asarf_level0_mdsr: descriptor %d field %d does not fit 64 bits", d,
+                                 3 + i);
+        r.ds_offset = v[0];
+        r.ds_size = v[1];
+        r.num_dsr = v[2];
+        r.dsr_size = v[3];
         *out = r;
         return clear_error();
     }
@@ -443,10 +454,12 @@ extern "C" int asarf_asar_forecast(const unsigned char* mds, long mds_bytes, lon
         first = (size_t)(before - *before_start);
         before = *before_start;
         // the list must start with an echo: drop leading calibration packets (the reference tests the calibration
-        // flag of the last record it read instead of the erased entry's; every kept non-echo entry is one)
+        // flag of the last record it read instead of the erased entry's; every kept non-echo entry is one). A
+        // dropped packet leaves the count before the start only if it was sensed before the start: the reference
+        // decrements its unsigned count for every dropped packet, which wraps below zero when the allowance is 0.
         while (first < off.size() && !is_echo[first]) {
+            if (asarf_mjd_compare(&tim[first], start) < 0 && before > 0) --before;
             ++first;
-            --before;
         }
     }
     const long n = (long)(off.size() - std::min(first, off.size()));
@@ -474,7 +487,7 @@ extern "C" int asarf_ers_forecast(const unsigned char* mds, long mds_bytes, long
     constexpr long kDsr = 11498;
     if (!mds || !start || !stop || !before_start || !after_stop || !offsets || !times || !n_out)
         return set_error(ASARF_EINVAL, "asarf_ers_forecast: null");
-    if (num_dsr <= 0 || mds_bytes < num_dsr * kDsr)
+    if (num_dsr <= 0 || mds_bytes < 0 || num_dsr > mds_bytes / kDsr)
         return set_error(ASARF_ESHAPE, "asarf_ers_forecast: %ld records of 11498 bytes exceed %ld bytes", num_dsr,
                          mds_bytes);
     const long before_in = *before_start, after_in = *after_stop;

@@ -433,13 +433,13 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
         ST_CHECK(mark(pl, "azimuth_core_fused"));
         rc.fused_gate = gate;
         HIP_CHECK(launch_rcmc_mf(rc, s));
-        ColFftArgs ib = col_args(pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n1, cs.n2, 1, n, cs.n2, 1, n,
+        ColFftArgs ib = col_args(pl->ctx, pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n1, cs.n2, 1, n, cs.n2, 1, n,
                                  cs.full->d_tw, 1.f, cs.p2);
         ib.fused_gate = gate;
         HIP_CHECK(launch_col_fft(ib, true, s));
         ST_CHECK(mark(pl, "azimuth_core_wide"));
         // 5. inverse pass A' (img2 -> img), 1/Nap, rows < Na
-        ColFftArgs ia = col_args(pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1, cs.n2,
+        ColFftArgs ia = col_args(pl->ctx, pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1, cs.n2,
                                  p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
         HIP_CHECK(launch_col_fft(ia, true, s));
         ST_CHECK(mark(pl, "azimuth_ifft_a"));
@@ -458,19 +458,19 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
         ST_CHECK(mark(pl, "rcmc_matched_filter"));
         // 5. azimuth IFFT (img2 -> img), 1/Nap, rows < Na
         if (cs.n2 == 1) {
-            ColFftArgs ia = col_args(pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, 1, 0, 1, n, 0, 1, p.na,
+            ColFftArgs ia = col_args(pl->ctx, pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, 1, 0, 1, n, 0, 1, p.na,
                                      nullptr, 1.0f / (float)pl->nap, cs.p1);
             ia.mfma = pl->fft_mfma;
             HIP_CHECK(launch_col_fft(ia, true, s));
             ST_CHECK(mark(pl, "azimuth_ifft_b"));
         } else {
-            ColFftArgs ib = col_args(pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, cs.n1, cs.n2, 1, n, cs.n2,
+            ColFftArgs ib = col_args(pl->ctx, pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, cs.n1, cs.n2, 1, n, cs.n2,
                                      1, n, cs.full->d_tw, 1.f, cs.p2);
             ib.mfma = pl->fft_mfma;
             HIP_CHECK(launch_col_fft(ib, true, s));
             ST_CHECK(mark(pl, "azimuth_ifft_b"));
             // inverse pass A' (img -> img2, out of place), 1/Nap, rows < Na: the SLC ends in img2
-            ColFftArgs ia = col_args(pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1, cs.n2,
+            ColFftArgs ia = col_args(pl->ctx, pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1, cs.n2,
                                      p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
             ia.mfma = pl->fft_mfma;
             HIP_CHECK(launch_col_fft(ia, true, s));

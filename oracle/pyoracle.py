@@ -226,7 +226,8 @@ def srgr(img: np.ndarray, grsr5, dr, gs, out_nr):
     return out
 
 
-def aperture_pixels(wavelength, prf, beta, vr_poly, dc_poly, r_first, dr, x):
+def aperture_pixels_at(wavelength, prf, beta, vr_poly, dc_poly, r_first, dr, x):
+    """CalcAperturePixels at one range index (sar/sar_metadata.h:117-125), C oracle."""
     v = np.ascontiguousarray(vr_poly, np.float64)
     d = np.ascontiguousarray(dc_poly if len(dc_poly) else [0.0], np.float64)
     return lib().or_aperture_pixels(wavelength, prf, beta, v.ctypes.data, d.ctypes.data, len(dc_poly), r_first, dr,

@@ -6,6 +6,7 @@ import torch
 
 import pyoracle as po
 from asarf import analysis, configs, focus, scene
+from parity import oracle_crop, oracle_imp, rel_rms_real
 
 pytestmark = pytest.mark.gpu
 
@@ -74,17 +75,12 @@ def test_plan_imp_matches_oracle_chain(ctx, nr):
     cfg = configs.asar_scene().scaled(na=1200, nr=nr, n_targets_random=8)
     raw, slc, res, imp = run_plan(ctx, cfg, imp=True)
     ref, rres = po.focus_slc(scene.raw_to_complex(raw), cfg)
-    front, back, near, far = res["crop"]
-    sub = ref[front:cfg.na - back, near:ref.shape[1] - far]
-    ml = po.az_multilook(sub, cfg.az_spacing)
-    po.rsl(ml, cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, 800e3)
-    nr_s = ref.shape[1] - near - far
-    grsr, ng = focus.srgr_fit_spherical(cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, nr_s,
-                                        cfg.earth_radius + cfg.orbit_height, cfg.earth_radius, 12.5)
-    gr = po.srgr(ml, grsr, cfg.range_spacing, 12.5, ng)
+    crop = oracle_crop(cfg, rres["dc_poly"])  # from the oracle alone, not from the plan's report
+    assert tuple(res["crop"]) == crop, (res["crop"], crop)
+    gr = oracle_imp(cfg, ref, crop)
     assert res["imp_rows"] == gr.shape[0]
-    got = imp[: res["imp_rows"], :ng]
-    err = float(np.sqrt(np.sum((got - gr) ** 2) / np.sum(gr ** 2)))
+    got = imp[: res["imp_rows"], :gr.shape[1]]
+    err = rel_rms_real(got, gr)
     print(f"imp rel rms {err:.3e}")
     assert err < 2e-4
 

@@ -305,7 +305,7 @@ def test_ers_packet_lines_feed_the_decoder_and_errors():
 
 
 # ------------------------------------------------------------------------------- product header and MDSR ----
-def product_file(rng, n_packets, dsr_size=b"-0000000001"):
+def product_file(rng, n_packets, dsr_size=b"-0000000001", num_dsr_field=None):
     mph = (b'PRODUCT="ASA_IM__0PNPDK20040109_194924_000000182023_00157_09730_1479.N1"\n'
            b'SENSING_START="09-JAN-2004 19:49:24.000000"\nREL_ORBIT=+00157\nPHASE=2\n')
     mph = mph.ljust(1246, b" ") + b"\n"
@@ -318,8 +318,9 @@ def product_file(rng, n_packets, dsr_size=b"-0000000001"):
     body = b"".join(packets)
     offset = 1247 + 1956
     dsd = (b'DS_NAME="ASAR SOURCE PACKETS         "\nDS_TYPE=M\nFILENAME="' + b" " * 62 + b'"\n'
-           + b"DS_OFFSET=+%020d<bytes>\nDS_SIZE=+%020d<bytes>\nNUM_DSR=+%010d\nDSR_SIZE=" % (offset, len(body), n_packets)
-           + dsr_size + b"<bytes>\n")
+           + b"DS_OFFSET=+%020d<bytes>\nDS_SIZE=+%020d<bytes>\n" % (offset, len(body))
+           + (b"NUM_DSR=+%010d\n" % n_packets if num_dsr_field is None else b"NUM_DSR=" + num_dsr_field + b"\n")
+           + b"DSR_SIZE=" + dsr_size + b"<bytes>\n")
     dsd = dsd.ljust(279, b" ") + b"\n"
     spare = (b"DS_NAME=\n" * 7).ljust(279, b" ") + b"\n"
     sph += dsd + spare * 3
@@ -349,6 +350,22 @@ def test_header_lookup_and_mdsr_descriptor():
     L.check(L.lib().asarf_level0_mdsr(f2, len(f2), ctypes.byref(d)))
     assert d.dsr_size == 11498
     assert L.lib().asarf_level0_mdsr(f, 3000, ctypes.byref(d)) == L.ESHAPE
+
+
+def test_mdsr_numbers_that_do_not_fit_64_bits_are_errors():
+    # the reference reads the counts with std::stoul, which throws past 64 bits; a wrapped count would let the
+    # record walk run past the data set
+    rng = np.random.default_rng(18)
+    d = L.Dsd()
+    f, _ = product_file(rng, 1, num_dsr_field=b"+18446744073709551615")  # 2^64 - 1 still fits
+    L.check(L.lib().asarf_level0_mdsr(f, len(f), ctypes.byref(d)))
+    assert d.num_dsr == 2 ** 64 - 1
+    for field in (b"+18446744073709551616", b"+99999999999999999999999"):
+        f, _ = product_file(rng, 1, num_dsr_field=field)
+        assert L.lib().asarf_level0_mdsr(f, len(f), ctypes.byref(d)) == L.EINVAL
+    f, _ = product_file(rng, 1, dsr_size=b"+000000000000000000000000011498")  # leading zeros do not count
+    L.check(L.lib().asarf_level0_mdsr(f, len(f), ctypes.byref(d)))
+    assert d.dsr_size == 11498
 
 
 def test_level0_file_walk_to_decoder_records():
@@ -425,6 +442,19 @@ def test_asar_forecast_drops_leading_calibration_after_trim():
         forecast(mds[:-10], 6, (100, 5, 2000), (100, 5, 9000), 9, 9)
 
 
+def test_asar_forecast_calibration_after_the_start_keeps_the_count():
+    # no allowance before the start: both echoes before it go; the calibration records that now lead the list were
+    # sensed at and after the start, so dropping them leaves the count before the start at 0 (the reference's
+    # unsigned count would wrap below zero here)
+    rng = np.random.default_rng(19)
+    mds, offs = build_mds(rng, [ECHO, ECHO, CAL, CAL, ECHO, ECHO])
+    o, t, before, after = forecast(mds, 6, (100, 5, 2000), (100, 5, 9000), 0, 0)
+    assert t == [4000, 5000] and o == offs[4:] and (before, after) == (0, 0)
+    # with one line allowed before the start, the echo at 1 ms is kept and nothing is dropped after it
+    o, t, before, after = forecast(mds, 6, (100, 5, 2000), (100, 5, 9000), 1, 0)
+    assert t == [1000, 2000, 3000, 4000, 5000] and (before, after) == (1, 0)
+
+
 # ------------------------------------------------------------------------------------- ERS packet forecast ----
 def ers_forecast(mds, n, start, stop, before, after, patc=None, cap=64):
     b, a, cnt = ctypes.c_long(before), ctypes.c_long(after), ctypes.c_long()
@@ -452,6 +482,10 @@ def test_ers_forecast_fills_missing_records_and_trims():
     mds2 = b"".join(ers_packet(rng, (100, 5, 1000 * i), 7, dr_no=d)[0] for i, d in enumerate([1, 2, 905]))
     with pytest.raises(L.AsarfError):
         ers_forecast(mds2, 3, (100, 5, 0), (100, 5, 9000), 9, 9)
+    # a record count whose byte size overflows 64-bit arithmetic is rejected before the walk
+    for n in (2 ** 63 // 11498 + 1, 2 ** 62, 7):
+        with pytest.raises(L.AsarfError):
+            ers_forecast(mds, n, (100, 5, 0), (100, 5, 9000), 9, 9)
 
 
 def test_ers_forecast_sbt_correction():
