@@ -31,7 +31,9 @@ def _make(cfg, env):
 
 
 def test_two_contexts_two_threads_keep_their_own_settings():
-    cfg = configs.asar_scene().scaled(na=900, nr=1400, n_targets_random=4)
+    # 2400 lines pad to 2401 = 49 x 49: the column passes have a compile-time kernel that ASARF_COL_STATIC=0 turns
+    # off, so the two contexts run different kernels
+    cfg = configs.asar_scene().scaled(na=2400, nr=1400, n_targets_random=4)
     raw, _ = scene.generate(cfg)
     traw = torch.from_numpy(raw).cuda()
     ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
