@@ -174,6 +174,7 @@ SIGNATURES = {
     "asarf_polyval": (_cd, [_vp, _ci, _cd]),
     "asarf_iq_correction": (_ci, [_vp, _P(Image), _sz, _ci, _P(IqResults)]),
     "asarf_doppler_centroid": (_ci, [_vp, _P(Image), _cd, _ci, _vp, _vp, _vp, _ci, _P(_ci)]),
+    "asarf_doppler_centroid_noalloc": (_ci, [_vp, _P(Image), _cd, _ci, _vp, _vp, _vp, _ci, _P(_ci)]),
     "asarf_range_compression": (_ci, [_vp, _P(Image), _vp, _ci, _ci, _P(Workspace)]),
     "asarf_range_doppler": (_ci, [_vp, _P(RdaParams), _P(Image), _P(Image), _P(Workspace)]),
     "asarf_rcmc_window_size": (_ci, []),

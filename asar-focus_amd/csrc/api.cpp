@@ -423,7 +423,7 @@ extern "C" int asarf_image_subset_copy(asarf_ctx* c, const asarf_image* img, int
 }
 
 // ---------------------------------------------------------------------------------------------- stages ----
-static int check_cf32(const asarf_image* img, const char* who) {
+int asarf::check_cf32(const asarf_image* img, const char* who) {
     if (!img || !img->data) return set_error(ASARF_EINVAL, "%s: null image", who);
     if (img->dtype != ASARF_CF32) return set_error(ASARF_EINVAL, "%s: image must be cf32", who);
     ST_CHECK(check_geometry(img->x_size, img->y_size, img->x_stride, img->y_stride, img->dtype, who));
@@ -494,49 +494,65 @@ extern "C" int asarf_iq_correction(asarf_ctx* c, asarf_image* img, size_t n_tota
     return clear_error();
 }
 
-extern "C" int asarf_doppler_centroid(asarf_ctx* c, const asarf_image* img, double prf, int quirks, double* poly3,
-                                      double* block_freq, double* block_x, int max_blocks, int* n_blocks) {
-    if (!c || !poly3) return set_error(ASARF_EINVAL, "asarf_doppler_centroid: null argument");
-    ST_CHECK(check_cf32(img, "asarf_doppler_centroid"));
-    if (!(prf > 0.0)) return set_error(ASARF_EINVAL, "asarf_doppler_centroid: prf must be > 0");
+namespace asarf {
+int dc_layout(const asarf_image* img, double prf, int quirks, const char* who, DcLayout* L) {
+    ST_CHECK(check_cf32(img, who));
+    if (!(prf > 0.0)) return set_error(ASARF_EINVAL, "%s: prf must be > 0", who);
     const int xs = img->x_size, ys = img->y_size;
     const int step = xs / 32;
-    if (step <= 0) return set_error(ASARF_ESHAPE, "asarf_doppler_centroid: range size %d < 32", xs);
+    if (step <= 0) return set_error(ASARF_ESHAPE, "%s: range size %d < 32", who, xs);
     int nb = 0;
     for (int xb = 0; xb < xs; xb += step) {
         if (!(quirks & ASARF_Q4_DC_EXTRA_BLOCK) && xb + step > xs) break;
         ++nb;
     }
-    if (nb > kMaxDcBlocks) return set_error(ASARF_EUNSUPPORTED, "asarf_doppler_centroid: %d blocks", nb);
-    if (ys < 2) return set_error(ASARF_ESHAPE, "asarf_doppler_centroid: need at least 2 lines");
-    ST_CHECK(ctx_init(c));
-    const int nstrips = std::max(1, std::min(ys - 1, 2048 / nb));
-    const size_t nvals = (size_t)nstrips * nb * 2;
-    void* scratch = nullptr;
-    ST_CHECK(ctx_scratch(c, sizeof(double) * nvals, &scratch));
-    HIP_CHECK(launch_dc_cf32(static_cast<const float2*>(img->data), img->x_stride, ys, 0, step, nb, img->x_stride,
-                             nstrips, static_cast<double*>(scratch), c->stream));
-    void* hp = nullptr;
-    ST_CHECK(ctx_pinned(c, sizeof(double) * nvals, &hp));
-    HIP_CHECK(hipMemcpyAsync(hp, scratch, sizeof(double) * nvals, hipMemcpyDeviceToHost, c->stream));
+    if (nb > kMaxDcBlocks) return set_error(ASARF_EUNSUPPORTED, "%s: %d blocks", who, nb);
+    if (ys < 2) return set_error(ASARF_ESHAPE, "%s: need at least 2 lines", who);
+    L->step = step;
+    L->nb = nb;
+    L->nstrips = std::max(1, std::min(ys - 1, kDcMaxPartials / nb));
+    L->nvals = (size_t)L->nstrips * nb * 2;
+    return ASARF_OK;
+}
+
+int dc_run(asarf_ctx* c, const asarf_image* img, double prf, const DcLayout& L, double* d_partials,
+           double* h_partials, double* poly3, double* block_freq, double* block_x, int max_blocks, int* n_blocks) {
+    const int nb = L.nb, nstrips = L.nstrips;
+    HIP_CHECK(launch_dc_cf32(static_cast<const float2*>(img->data), img->x_stride, img->y_size, 0, L.step, nb,
+                             img->x_stride, nstrips, d_partials, c->stream));
+    HIP_CHECK(hipMemcpyAsync(h_partials, d_partials, sizeof(double) * L.nvals, hipMemcpyDeviceToHost, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
-    const double* p = static_cast<const double*>(hp);
-    std::vector<double> fx(nb), fy(nb);
+    double fx[kMaxDcBlocks], fy[kMaxDcBlocks];
     for (int b = 0; b < nb; ++b) {
         double re = 0.0, im = 0.0;
         for (int s = 0; s < nstrips; ++s) {
-            re += p[((size_t)s * nb + b) * 2];
-            im += p[((size_t)s * nb + b) * 2 + 1];
+            re += h_partials[((size_t)s * nb + b) * 2];
+            im += h_partials[((size_t)s * nb + b) * 2 + 1];
         }
-        const int xb = b * step;
-        fx[b] = (double)((xb + xb + step) / 2);
+        const int xb = b * L.step;
+        fx[b] = (double)((xb + xb + L.step) / 2);
         fy[b] = prf * std::atan2(im, re) / (2.0 * M_PI);
         if (block_freq && b < max_blocks) block_freq[b] = fy[b];
         if (block_x && b < max_blocks) block_x[b] = fx[b];
     }
     if (n_blocks) *n_blocks = nb;
-    if (polyfit_qr(fx.data(), fy.data(), nb, 2, poly3) != 0)
-        return set_error(ASARF_EINVAL, "asarf_doppler_centroid: polynomial fit failed");
+    if (polyfit_qr(fx, fy, nb, 2, poly3) != 0) return set_error(ASARF_EINVAL, "Doppler centroid: polynomial fit failed");
+    return ASARF_OK;
+}
+}  // namespace asarf
+
+extern "C" int asarf_doppler_centroid(asarf_ctx* c, const asarf_image* img, double prf, int quirks, double* poly3,
+                                      double* block_freq, double* block_x, int max_blocks, int* n_blocks) {
+    if (!c || !poly3) return set_error(ASARF_EINVAL, "asarf_doppler_centroid: null argument");
+    DcLayout L;
+    ST_CHECK(dc_layout(img, prf, quirks, "asarf_doppler_centroid", &L));
+    ST_CHECK(ctx_init(c));
+    void* scratch = nullptr;
+    ST_CHECK(ctx_scratch(c, sizeof(double) * L.nvals, &scratch));
+    void* hp = nullptr;
+    ST_CHECK(ctx_pinned(c, sizeof(double) * L.nvals, &hp));
+    ST_CHECK(dc_run(c, img, prf, L, static_cast<double*>(scratch), static_cast<double*>(hp), poly3, block_freq,
+                    block_x, max_blocks, n_blocks));
     return clear_error();
 }
 

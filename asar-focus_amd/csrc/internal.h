@@ -87,6 +87,18 @@ int get_col_split(asarf_ctx* ctx, int n, ColSplit* out, int n2_max = 0);
 ColFftArgs col_args(const asarf_ctx* c, const float2* in, long in_stride, float2* out, long out_stride, int ncols,
                     int groups, long in_g, long in_m, int in_rows, long out_g, long out_m, int out_rows,
                     const float2* twf, float scale, const FftPlanHost* sub);
+// Doppler-centroid lag sums (CalculateDopplerCentroid, fractional_doppler_centroid.cu:98-162): the block layout
+// of an image (validated; `who` names the entry point in errors) and the run that fills d_partials (device) with
+// nvals doubles, copies them into h_partials (host) and fits the polynomial. dc_run allocates nothing.
+constexpr int kDcMaxPartials = 2048;  // strips x blocks, so nvals <= 2 * kDcMaxPartials
+struct DcLayout {
+    int step = 0, nb = 0, nstrips = 0;
+    size_t nvals = 0;
+};
+int dc_layout(const asarf_image* img, double prf, int quirks, const char* who, DcLayout* out);
+int dc_run(asarf_ctx* c, const asarf_image* img, double prf, const DcLayout& layout, double* d_partials,
+           double* h_partials, double* poly3, double* block_freq, double* block_x, int max_blocks, int* n_blocks);
+int check_cf32(const asarf_image* img, const char* who);
 bool radix_plan(int n, int lanes, std::vector<int>& out);
 bool radix_plan_relaxed(int n, int lanes, std::vector<int>& out);
 bool radix_plan_fits(int n, int lanes, const std::vector<int>& radix);

@@ -132,6 +132,14 @@ int asarf_iq_correction(asarf_ctx* ctx, asarf_image* img, size_t n_total_samples
 int asarf_doppler_centroid(asarf_ctx* ctx, const asarf_image* img, double prf, int quirks, double* poly3,
                            double* block_freq, double* block_x, int max_blocks, int* n_blocks);
 
+/* No counterpart in the reference: asarf_doppler_centroid for callers that cannot allocate (tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and errors; it calls no hipMalloc, hipFree or other
+ * allocating routine. Its lag sums go to one fixed device array per GPU (2 x 2048 doubles, 32 KiB, held until the
+ * process ends) and return through the stack; layouts needing more are rejected with ASARF_ESHAPE (the block
+ * layout of asarf_doppler_centroid never does). Calls that overlap on one GPU share that array and race. */
+int asarf_doppler_centroid_noalloc(asarf_ctx* ctx, const asarf_image* img, double prf, int quirks, double* poly3,
+                                   double* block_freq, double* block_x, int max_blocks, int* n_blocks);
+
 /* RangeCompression (sar/range_compression.cu:57-111), in place on a cf32 image whose x_stride equals the chirp
  * length. chirp_iq: host, n_chirp complex floats (the padded, windowed chirp). On return x_size = x_size - nc.
  * ws may be NULL (the fused row kernel needs no FFT work area). */
