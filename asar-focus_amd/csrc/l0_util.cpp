@@ -20,8 +20,14 @@ constexpr int64_t kDays1950To2000 = 18262;
 // days from 1950-01-01 to 1991-01-01; a PATC epoch before that is rejected (ers_sbt.cc:48-54)
 constexpr int32_t kMinPatcDays1950 = 14975;
 
+// Day counts beyond +-1e8 (about 274,000 years) are not times of any product: entry points that take an mjd from
+// the caller reject them, and days decoded from packet bytes are clamped here, so the microsecond arithmetic
+// below cannot overflow.
+constexpr int64_t kMaxMjdDays = 100000000;
+
 int64_t mjd_to_micros(const asarf_mjd& m) {
-    return (int64_t)m.days * kMicrosPerDay + (int64_t)m.seconds * 1000000LL + (int64_t)m.micros;
+    const int64_t days = std::min<int64_t>(std::max<int64_t>(m.days, -kMaxMjdDays), kMaxMjdDays);
+    return days * kMicrosPerDay + (int64_t)m.seconds * 1000000LL + (int64_t)m.micros;
 }
 
 // PtimeToMjd (envisat_utils.h:50-59): days = floor(t / day) from 2000-01-01, then whole seconds of the day and the
@@ -74,7 +80,13 @@ extern "C" int asarf_mjd_compare(const asarf_mjd* a, const asarf_mjd* b) {
 extern "C" int asarf_mjd_add_micros(const asarf_mjd* in, long long micros, asarf_mjd* out) {
     // MjdAddMicros (envisat_utils.h:74-78)
     if (!in || !out) return set_error(ASARF_EINVAL, "asarf_mjd_add_micros: null");
-    *out = micros_to_mjd(mjd_to_micros(*in) + (int64_t)micros);
+    if (in->days > kMaxMjdDays || in->days < -kMaxMjdDays)
+        return set_error(ASARF_EINVAL, "asarf_mjd_add_micros: %d days is out of range", in->days);
+    int64_t us = 0;
+    if (__builtin_add_overflow(mjd_to_micros(*in), (int64_t)micros, &us) || us / kMicrosPerDay > kMaxMjdDays ||
+        us / kMicrosPerDay < -kMaxMjdDays)
+        return set_error(ASARF_EINVAL, "asarf_mjd_add_micros: result out of range");
+    *out = micros_to_mjd(us);
     return clear_error();
 }
 
@@ -123,6 +135,9 @@ extern "C" int asarf_patc_register(const asarf_patc* patc, asarf_patc_reference*
     if (patc->epoch_1950 < kMinPatcDays1950)
         return set_error(ASARF_EINVAL, "asarf_patc_register: epoch %d days after 1950-01-01 is before 1991",
                          patc->epoch_1950);
+    if (patc->epoch_1950 > kMaxMjdDays)
+        return set_error(ASARF_EINVAL, "asarf_patc_register: epoch %d days after 1950-01-01 is out of range",
+                         patc->epoch_1950);
     out->time_us = ((int64_t)patc->epoch_1950 - kDays1950To2000) * kMicrosPerDay + (int64_t)patc->milliseconds * 1000;
     out->sbt_counter = patc->sbt_counter;
     out->sbt_period_ns = patc->sbt_period;
@@ -142,8 +157,12 @@ extern "C" int asarf_adjust_isp_sensing_time(const asarf_patc_reference* ref, as
         delta = (int64_t)sbt - (int64_t)ref->sbt_counter;
     else
         delta = (int64_t)asarf_counter_gap(ref->sbt_counter, sbt, 0xFFFFFFFFull);
-    const int64_t ns = (int64_t)ref->sbt_period_ns * delta;
-    *isp_time = micros_to_mjd(ref->time_us + ns / 1000);
+    int64_t ns = 0, us = 0;
+    if (__builtin_mul_overflow((int64_t)ref->sbt_period_ns, delta, &ns) ||
+        __builtin_add_overflow((int64_t)ref->time_us, ns / 1000, &us) || us / kMicrosPerDay > kMaxMjdDays ||
+        us / kMicrosPerDay < -kMaxMjdDays)
+        return set_error(ASARF_EINVAL, "asarf_adjust_isp_sensing_time: time out of range");
+    *isp_time = micros_to_mjd(us);
     return clear_error();
 }
 

@@ -56,6 +56,15 @@ def test_mjd_add_micros_matches_datetime():
         assert add_micros(mjd(d, s, u), add) == (delta.days, delta.seconds, delta.microseconds)
 
 
+def test_mjd_arithmetic_out_of_range_is_rejected():
+    # day counts beyond +-1e8 or sums past the 64-bit microsecond range are errors, not wrapped times
+    out = L.Mjd()
+    for m, us in ((mjd(200_000_000, 0, 0), 0), (mjd(-200_000_000, 0, 0), 1), (mjd(99_000_000, 0, 0), 2 ** 62),
+                  (mjd(0, 0, 0), -(2 ** 63) + 5)):
+        assert L.lib().asarf_mjd_add_micros(ctypes.byref(m), us, ctypes.byref(out)) != 0
+    assert add_micros(mjd(99_000_000, 5, 6), 1) == (99_000_000, 5, 7)
+
+
 # ----------------------------------------------------------------------------------------- counter gap ----
 def test_counter_gap_reference_cases():
     # parse_util_test.cc:27-49 (MaxValueForBits<uint16_t, 14> = 16383)
