@@ -16,7 +16,7 @@ OK, EINVAL, ESHAPE, EWORKSPACE, EHIP, ENOMEM, EUNSUPPORTED = range(7)
 Q1_IQ_PADDED_WIDTH, Q2_COS_NO_SQRT, Q4_DC_EXTRA_BLOCK = 1, 2, 4
 QUIRKS_REFERENCE = 7
 CF32, F32 = 0, 1
-RAW_CF32, RAW_U8, RAW_I8 = 0, 1, 2
+RAW_CF32, RAW_U8, RAW_I8, RAW_ERS_LINES = 0, 1, 2, 3
 FORWARD, INVERSE = 0, 1
 FFT_ROWS, FFT_COLUMNS = 0, 1
 
@@ -65,7 +65,8 @@ class FocusParams(ctypes.Structure):
                 ("vr_poly", ctypes.c_double * 3), ("n_total_samples", ctypes.c_size_t), ("quirks", ctypes.c_int),
                 ("imp", ctypes.c_int), ("azimuth_spacing", ctypes.c_double), ("reference_range", ctypes.c_double),
                 ("grsr_poly", ctypes.c_double * 5), ("gr_range_size", ctypes.c_int),
-                ("ground_spacing", ctypes.c_double), ("lines_before", ctypes.c_int), ("lines_after", ctypes.c_int)]
+                ("ground_spacing", ctypes.c_double), ("lines_before", ctypes.c_int), ("lines_after", ctypes.c_int),
+                ("samples_per_line", ctypes.c_int), ("line_bytes", ctypes.c_long), ("swst_mult", ctypes.c_int)]
 
 
 class FocusResults(ctypes.Structure):
@@ -224,6 +225,7 @@ SIGNATURES = {
     "asarf_plan_destroy": (_ci, [_vp]),
     "asarf_plan_device_bytes": (_sz, [_vp]),
     "asarf_plan_execute": (_ci, [_vp, _vp, _P(Image), _vp]),
+    "asarf_plan_execute_lines": (_ci, [_vp, _vp, _vp, _ci, _P(Image), _vp]),
     "asarf_plan_imp_capacity": (_ci, [_vp, _P(_ci), _P(_ci)]),
     "asarf_plan_results": (_ci, [_vp, _P(FocusResults)]),
     "asarf_plan_set_timing": (_ci, [_vp, _ci]),
@@ -241,6 +243,8 @@ SIGNATURES = {
     "asarf_fft_c2c_split_bf16": (_ci, [_vp, _vp, _ci, _ci, ctypes.c_long, _ci, _ci, _cf]),
     "asarf_plan_set_fft": (_ci, [_vp, _ci]),
     "asarf_autotune_fft": (_ci, [_vp, _ci, _ci, _ci, _ci, _P(_ci), _P(_cf), _P(_ci)]),
+    "asarf_tune_save": (_ci, [_vp, ctypes.c_char_p]),
+    "asarf_tune_load": (_ci, [_vp, ctypes.c_char_p, _P(_ci)]),
     "asarf_batch_create": (_ci, [_vp, _ci, _P(_vp)]),
     "asarf_batch_add": (_ci, [_vp, _P(FocusParams), _vp, _vp, _vp, _P(_ci)]),
     "asarf_batch_execute": (_ci, [_vp, _ci]),

@@ -46,6 +46,9 @@ class SceneConfig:
     seed: int = SEED_BASE
     orbit_height: float = 790e3  # spherical-earth SRGR geometry
     earth_radius: float = 6371e3
+    # processing velocity over the range index (highest power first), e.g. estimated from an orbit; empty -> the
+    # constant vr the scene was simulated with
+    vr_coeffs: tuple = ()
 
     @property
     def nc(self) -> int:
@@ -67,7 +70,7 @@ class SceneConfig:
 
     @property
     def vr_poly(self):
-        return [0.0, 0.0, self.vr]
+        return [float(v) for v in self.vr_coeffs] if self.vr_coeffs else [0.0, 0.0, self.vr]
 
     def scaled(self, na: int | None = None, nr: int | None = None, name: str | None = None, **kw):
         """A smaller scene with the same radar (for oracle-sized parity tests)."""

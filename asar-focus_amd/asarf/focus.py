@@ -6,6 +6,7 @@ on the torch stream the Context was created with, so torch events and synchronis
 from __future__ import annotations
 
 import ctypes
+import os
 import math
 
 import numpy as np
@@ -308,14 +309,21 @@ def subset_copy(ctx: Context, img: L.Image, x0: int, y0: int, w: int, h: int, ds
 
 
 def focus_params(cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
-                 orbit_height: float | None = None, lines_before: int = 0, lines_after: int = 0):
+                 orbit_height: float | None = None, lines_before: int = 0, lines_after: int = 0,
+                 srgr: tuple | None = None, azimuth_spacing: float | None = None, lines: tuple | None = None):
     """asarf_focus_params of a scene configuration (and the SRGR fit when imp). lines_before / lines_after are the
-    lines cached before the sensing start and after the sensing stop (main.cc:296-298)."""
+    lines cached before the sensing start and after the sensing stop (main.cc:296-298). srgr = (grsr5, ground
+    range size) replaces the spherical-earth fit (e.g. asarf_srgr_fit from an orbit); azimuth_spacing replaces
+    vr / prf. lines = (samples_per_line, line_bytes, swst_mult) makes the input ERS level-0 lines
+    (ASARF_RAW_ERS_LINES; cfg.nr is then the range size after the SWST shifts)."""
     p = L.FocusParams()
     p.na, p.nr, p.nc = cfg.na, cfg.nr, cfg.nc
     p.nrp = cfg.nrp or padding_size(cfg.nr + cfg.nc)
     p.nap = cfg.nap or padding_size(cfg.na)
     p.raw_format = L.RAW_U8 if cfg.raw_format == "u8_ers" else L.RAW_I8
+    if lines is not None:
+        p.raw_format = L.RAW_ERS_LINES
+        p.samples_per_line, p.line_bytes, p.swst_mult = int(lines[0]), int(lines[1]), int(lines[2])
     p.prf, p.wavelength, p.range_spacing = cfg.prf, cfg.wavelength, cfg.range_spacing
     p.slant_range_first, p.az_bandwidth_fraction = cfg.r_first, cfg.az_bandwidth_fraction
     for i, v in enumerate(cfg.vr_poly):
@@ -329,9 +337,12 @@ def focus_params(cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool =
         near = 8
         nr_s = cfg.nr - cfg.nc - 16
         h = cfg.orbit_height if orbit_height is None else orbit_height
-        grsr, ng = srgr_fit_spherical(cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, nr_s,
-                                      cfg.earth_radius + h, cfg.earth_radius, 12.5)
-        p.azimuth_spacing = cfg.az_spacing
+        if srgr is not None:
+            grsr, ng = [float(v) for v in srgr[0]], int(srgr[1])
+        else:
+            grsr, ng = srgr_fit_spherical(cfg.r_first + near * cfg.range_spacing, cfg.range_spacing, nr_s,
+                                          cfg.earth_radius + h, cfg.earth_radius, 12.5)
+        p.azimuth_spacing = cfg.az_spacing if azimuth_spacing is None else azimuth_spacing
         p.reference_range = 800000.0
         for i in range(5):
             p.grsr_poly[i] = grsr[i]
@@ -345,10 +356,12 @@ class Plan:
 
     def __init__(self, ctx: Context, cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
                  chirp: np.ndarray | None = None, orbit_height: float | None = None, lines_before: int = 0,
-                 lines_after: int = 0):
+                 lines_after: int = 0, srgr: tuple | None = None, azimuth_spacing: float | None = None,
+                 lines: tuple | None = None):
         self.ctx = ctx
         self.cfg = cfg
-        p, self.grsr, self.ng = focus_params(cfg, quirks, imp, orbit_height, lines_before, lines_after)
+        p, self.grsr, self.ng = focus_params(cfg, quirks, imp, orbit_height, lines_before, lines_after, srgr,
+                                             azimuth_spacing, lines)
         self.nrv = cfg.nr - cfg.nc
         self.params = p
         self.chirp = generate_chirp(cfg, p.nrp) if chirp is None else np.ascontiguousarray(chirp, np.complex64)
@@ -371,6 +384,17 @@ class Plan:
         assert raw.is_cuda and raw.is_contiguous() and raw.numel() == self.cfg.na * self.cfg.nr * 2
         L.check(L.lib().asarf_plan_execute(self.h, raw.data_ptr(), ctypes.byref(self.slc),
                                            self.imp_out.data_ptr() if self.imp_out is not None else None))
+        return self.slc
+
+    def execute_lines(self, lines: torch.Tensor, swst: torch.Tensor, min_swst: int):
+        """asarf_plan_execute_lines: ERS level-0 lines [na, line_bytes] uint8 and their SWST codes [na] int16, on
+        the device."""
+        assert lines.is_cuda and lines.dtype == torch.uint8 and lines.is_contiguous()
+        assert lines.shape[0] == self.cfg.na and lines.shape[1] == self.params.line_bytes
+        assert swst.is_cuda and swst.dtype == torch.int16 and swst.numel() == self.cfg.na
+        L.check(L.lib().asarf_plan_execute_lines(self.h, lines.data_ptr(), swst.data_ptr(), int(min_swst),
+                                                 ctypes.byref(self.slc),
+                                                 self.imp_out.data_ptr() if self.imp_out is not None else None))
         return self.slc
 
     def rcmc_pairs(self, raw: torch.Tensor, n: int, seed: int) -> torch.Tensor:
@@ -494,6 +518,18 @@ def autotune(ctx: Context, n: int, columns: bool, batch: int, reps: int = 3) -> 
     L.check(L.lib().asarf_autotune_fft(ctx.h, n, L.FFT_COLUMNS if columns else L.FFT_ROWS, batch, reps,
                                        ctypes.byref(choice), ctypes.byref(ms), ctypes.byref(n_cand)))
     return {"choice": choice.value, "ms": ms.value, "candidates": n_cand.value}
+
+
+def tune_save(ctx: Context, path: str) -> None:
+    """Writes ctx's autotuner choices to a cache file."""
+    L.check(L.lib().asarf_tune_save(ctx.h, os.fsencode(path)))
+
+
+def tune_load(ctx: Context, path: str) -> int:
+    """Reads an autotuner cache file into ctx (same device architecture); returns the number of choices."""
+    n = ctypes.c_int()
+    L.check(L.lib().asarf_tune_load(ctx.h, os.fsencode(path), ctypes.byref(n)))
+    return n.value
 
 
 BATCH_SEQUENTIAL, BATCH_GRAPH = 0, 1

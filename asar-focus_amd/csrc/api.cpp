@@ -217,8 +217,13 @@ const FftPlanHost* get_plan(asarf_ctx* c, int n, int lanes, int* status) {
     auto p = std::make_unique<FftPlanHost>();
     p->n = n;
     if (!radix_plan_relaxed(n, lanes, p->radix)) {
-        *status = set_error(ASARF_EUNSUPPORTED, "no Stockham plan for length %d with %d lanes per transform", n, lanes);
-        return nullptr;
+        // a column length with a compile-time kernel needs only the twiddle table (no runtime radix plan)
+        if (!(lanes == kThreads / kColTile && c->col_static && col_static_length(n))) {
+            *status = set_error(ASARF_EUNSUPPORTED, "no Stockham plan for length %d with %d lanes per transform", n,
+                                lanes);
+            return nullptr;
+        }
+        p->radix.clear();
     }
     p->d_tw = get_twiddles(c, n, status);
     if (!p->d_tw) return nullptr;
@@ -251,12 +256,14 @@ int get_col_split(asarf_ctx* c, int n, ColSplit* cs, int n2_max) {
         best2 = n / tuned;
     }
     const bool use_tuned = best1 != 0;
+    // a length is usable with a runtime radix plan, or with a compile-time column kernel when those are enabled
+    auto usable = [&](int m) { return radix_plan_relaxed(m, lanes, tmp) || (c->col_static && col_static_length(m)); };
     for (int n1 = 2; n1 <= n / 2 && !use_tuned; ++n1) {
         if (n % n1) continue;
         const int n2 = n / n1;
         if ((size_t)n1 > lds_cap || (size_t)n2 > lds_cap) continue;
         if (n2_max > 0 && n2 > n2_max) continue;
-        if (!radix_plan_relaxed(n1, lanes, tmp) || !radix_plan_relaxed(n2, lanes, tmp)) continue;
+        if (!usable(n1) || !usable(n2)) continue;
         bool better;
         if (!best1)
             better = true;
@@ -650,7 +657,7 @@ static ColFftArgs with_mfma(int mfma, ColFftArgs a) {
 // `in` (row pitch in_stride; pass 0 reads rows >= in_rows as zero, pass 1 reads all n rows) and writes img; in
 // and img may be the same buffer (in place) or two buffers (out of place, which streams slightly faster).
 int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
-                     float2* img, long stride, int ncols, int mfma) {
+                     float2* img, long stride, int ncols, int mfma, const int* fused_gate) {
     const int n = cs.n;
     if (cs.n2 == 1) {
         if (pass == 0) {
@@ -665,8 +672,9 @@ int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* i
                                       cs.full->d_tw, 1.f, cs.p1);
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
     } else {
-        const ColFftArgs a = col_args(c, in, in_stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr, 1.f,
-                                      cs.p2);
+        ColFftArgs a = col_args(c, in, in_stride, img, stride, ncols, cs.n1, cs.n2, 1, n, cs.n2, 1, n, nullptr, 1.f,
+                                cs.p2);
+        a.fused_gate = fused_gate;  // pass B gated off where the fused azimuth core runs it
         HIP_CHECK(launch_col_fft(with_mfma(mfma, a), false, c->stream));
     }
     return ASARF_OK;

@@ -104,7 +104,7 @@ bool radix_plan_relaxed(int n, int lanes, std::vector<int>& out);
 bool radix_plan_fits(int n, int lanes, const std::vector<int>& radix);
 // Column transform passes of a four-step split (api.cpp); mfma selects the split-bfloat16 kernels.
 int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, int in_rows,
-                     float2* img, long stride, int ncols, int mfma = 0);
+                     float2* img, long stride, int ncols, int mfma = 0, const int* fused_gate = nullptr);
 int col_inverse_pass_to(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, float2* out,
                         long out_stride, int ncols, int out_rows, float scale);
 int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,

@@ -28,6 +28,18 @@ __device__ __forceinline__ float2 sinc16_interp(const float2* tap, const float (
     return make_float2(acc.x * inv, acc.y * inv);
 }
 
+// Bin frequency f aliased into the prf-wide band centred on the Doppler centroid dc
+// (sar/range_doppler_algorithm.cu:122-140).
+__device__ __forceinline__ double alias_about_dc(double f, double dc, double prf) {
+    const double half = 0.5 * prf;
+    if (dc < 0.0) {
+        if (f > half + dc) f -= prf;
+    } else if (dc > 0.0) {
+        if (f < -half + dc) f += prf;
+    }
+    return f;
+}
+
 // Azimuth band test of the matched filter (sar/range_doppler_algorithm.cu:238-246): a bin is dropped when its
 // aliased frequency fn lies more than beta * prf / 2 from the bin-quantised centroid dc_off. Both are whole
 // multiples of the bin spacing, so the test runs on bin offsets, |rint(fn / step) - rint(dc_off / step)| against

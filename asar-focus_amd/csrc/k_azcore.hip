@@ -1,0 +1,225 @@
+// Fused azimuth core: forward pass B of the azimuth FFT, 16-tap sinc RCMC, azimuth matched filter and the
+// inverse pass B', one workgroup per (k1 group, 64-column tile), out of place (pass-A output -> pass-B' output).
+//
+// With the four-step split N = N1*N2, pass A leaves group k1's length-N2 transform inputs in the contiguous rows
+// k1*N2 + m; pass B turns them into the bins k1 + N1*k2 (k2 < N2) of those rows. RCMC interpolates along range
+// within one bin and the matched filter is pointwise, so a workgroup that transforms its k1 group for its 64
+// columns plus a range halo can correct the bins in LDS and run the inverse length-N2 transform before anything
+// returns to HBM: the spectrum makes one round trip to HBM instead of three (B out, RCMC in and out, B' in).
+//   staged columns:    [x0 - 8, x0 + 80)   (forward transforms of the 8 + 64 + 16 staged columns)
+//   produced columns:  [x0, x0 + 64)       (RCMC reads x + p - 8 .. x + p + 7, p = round(shift) <= kAzCoreReach)
+// The per-pixel arithmetic is k_rcmc_tile's (k_rda.hip) and the transforms are the Stockham passes of the column
+// kernels, so the core reproduces the separate kernels up to the rounding of a different N1 x N2 split.
+//
+// With N2 = 84 the tile (84 x 89 complex samples) and the twiddle tables fit the 64 KB a workgroup may allocate.
+// Phases (512 lanes, 16 per column transform, 32 columns per batch):
+//   1. three batches (the last 24 columns wide): load 32 staged columns of the group's N2 rows, forward
+//      transform (LDS exchange through the batch's own tile columns), bins left in the tile;
+//   2. RCMC + matched filter: lane (wave w, column c) produces bins w, w + 8, ... of column c (a wave reads 64
+//      consecutive tile columns of one bin row); results held in registers until every tap is read, then
+//      written over the produced columns;
+//   3. two batches: inverse transform of 32 produced columns, inverse inter-pass twiddle, store.
+#include "interp.h"
+#include "kernels.h"
+
+namespace asarf {
+
+constexpr int kAcThreads = 512;
+constexpr int kAcOut = 64;                                    // produced columns per tile
+constexpr int kAcLeft = 8;                                    // first produced column inside the tile
+constexpr int kAcCols = kAcLeft + kAcOut + 8 + kAzCoreReach;  // staged columns per tile
+constexpr int kAcPitch = kAcCols + 1;
+constexpr int kAcBatch = 32;                                  // columns per transform batch
+constexpr int kAcLanes = kAcThreads / kAcBatch;               // lanes per column transform
+static_assert(kAcOut % kAcBatch == 0, "whole batches of produced columns");
+static_assert(kAcOut == 64, "phase 2 maps one wave onto the produced columns");
+
+template <int N2, class Seq>
+__global__ __launch_bounds__(kAcThreads) void k_az_core(AzCoreArgs a) {
+    static_assert(Seq::product() == N2, "radix sequence must factor N2");
+    extern __shared__ float2 lds[];
+    if (a.gate && *a.gate > kAzCoreReach) return;
+    // XCD-aware work order: workgroups are dealt to the 8 XCDs round robin, so consecutive work items
+    // (neighbouring tiles of one k1 group, whose staged halos overlap) are given to the same XCD and its L2
+    const int per_xcd = gridDim.x / 8;
+    const int w = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (w >= a.items) return;
+    const int k1 = w / a.ntiles;
+    const int x0 = (w % a.ntiles) * kAcOut;
+    const int tid = threadIdx.x;
+    float2* tl = lds;  // [N2][kAcPitch]
+    const TwLds tw2 = tw_fill(lds + N2 * kAcPitch, a.tw2, N2, tid, kAcThreads);
+    const TwLds twf = tw_fill(lds + N2 * kAcPitch + tw_lds_elems(N2), a.tw_full, a.nbins, tid, kAcThreads);
+    constexpr int E = seq_max_elems<N2, Seq, kAcLanes>();
+    const int b = tid & (kAcBatch - 1);
+    const int lane = tid / kAcBatch;
+    const long row0 = (long)k1 * N2;
+    float2 v[E];
+
+    // ---- 1. forward pass B of the staged columns
+    for (int bt = 0; bt < (kAcCols + kAcBatch - 1) / kAcBatch; ++bt) {
+        const int c = bt * kAcBatch + b;
+        const bool cin = c < kAcCols;  // lanes past the tile (last batch) transform zeros and touch no LDS
+        const int x = x0 - kAcLeft + c;
+        const bool xin = cin && x >= 0 && x < a.nrv;  // columns outside the image are zero (never read as taps)
+        const float2* src = a.in + (xin ? x : 0);
+        auto gl_load = [&](int m) -> float2 {
+            return xin ? src[(row0 + m) * a.stride] : make_float2(0.f, 0.f);
+        };
+        auto lds_store = [&](int i, float2 val) {
+            if (cin) tl[i * kAcPitch + c] = val;
+        };
+        auto lds_load = [&](int i) { return cin ? tl[i * kAcPitch + c] : make_float2(0.f, 0.f); };
+        constexpr int R0 = Seq::at(0);
+        pass_load<R0>(v, lane, kAcLanes, N2, gl_load);
+        pass_compute<R0, E, false>(v, lane, kAcLanes, N2, 1, tw2);
+        static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int Rp = Seq::at(p - 1), R = Seq::at(p);
+            pass_store<Rp>(v, lane, kAcLanes, N2, Seq::prefix(p - 1), lds_store);
+            __syncthreads();
+            pass_load<R>(v, lane, kAcLanes, N2, lds_load);
+            __syncthreads();
+            pass_compute<R, E, false>(v, lane, kAcLanes, N2, Seq::prefix(p), tw2);
+        });
+        constexpr int RL = Seq::at(Seq::count - 1);
+        pass_visit<RL>(v, lane, kAcLanes, N2, [&](int k, float2& val, bool ok) {
+            if (ok && cin) tl[k * kAcPitch + c] = val;
+        });
+    }
+    __syncthreads();
+
+    // ---- 2. RCMC + matched filter of the produced columns (the arithmetic of k_rcmc_tile)
+    {
+        const int cw = tid & 63;
+        const int wv = tid >> 6;
+        const int xc = x0 + cw;
+        const bool xok = xc < a.nrv;
+        const ColConst cc = a.col[xok ? xc : a.nrv - 1];
+        const double step = a.prf / (double)a.nbins;
+        const double inv_step = (double)a.nbins / a.prf;
+        const double half_bins = 0.5 * a.beta * (double)a.nbins;
+        const double m_dc = rint(cc.dc_off * inv_step);
+        const double half = 0.5 * a.prf;
+        float ws[16];  // signed window (-1)^i w[i]
+#pragma unroll
+        for (int i = 0; i < 16; ++i) ws[i] = (i & 1) ? -a.window[i] : a.window[i];
+        constexpr int NR = (N2 + 7) / 8;
+        float2 res[NR];
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int m = wv + 8 * i;
+            if (m >= N2) break;
+            const int k = k1 + a.n1 * m;
+            const double fb = (k < a.nbins / 2) ? (double)k * step : (double)(k - a.nbins) * step;
+            const double fn = alias_about_dc(fb, cc.dc, a.prf);
+            float2 val = make_float2(0.f, 0.f);
+            // outside the azimuth band the matched filter zeroes the pixel: skip its interpolation
+            if (!outside_band(fn, m_dc, inv_step, half_bins)) {
+                const double shift = cc.shift_k * fn * fn;
+                const double p = floor(shift + 0.5);  // round half away from zero: shift >= 0
+                const int pi = (int)fmin(p, 4096.0);
+                const int cx = xc + pi;
+                if (!(cx - 8 < 0 || cx + 8 >= a.nrv)) {
+                    const float d = (float)(shift - p);
+                    // the gate guarantees pi <= kAzCoreReach; the clamp only keeps the taps inside the tile
+                    const float2* tap = tl + m * kAcPitch + min(kAcLeft + cw + pi - 8, kAcCols - 16);
+                    val = (d == 0.0f) ? tap[8] : sinc16_interp(tap, ws, d);
+                }
+                double fc = fb - cc.dc_off;
+                if (fc > half)
+                    fc -= a.prf;
+                else if (fc < -half)
+                    fc += a.prf;
+                const double turns = cc.inv_ka * (0.5 * fc * fc + cc.dc_off * fb);
+                const float fr = (float)(turns - rint(turns));
+                float sn, cs;
+                sincos_turns(fr, &sn, &cs);
+                val = make_float2(val.x * cs + val.y * sn, val.y * cs - val.x * sn);
+            }
+            res[i] = val;
+        }
+        __syncthreads();  // every tap is read before the produced columns are overwritten
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int m = wv + 8 * i;
+            if (m >= N2) break;
+            tl[m * kAcPitch + kAcLeft + cw] = res[i];
+        }
+    }
+    __syncthreads();
+
+    // ---- 3. inverse pass B' of the produced columns, inverse inter-pass twiddle, store
+    for (int bt = 0; bt < kAcOut / kAcBatch; ++bt) {
+        const int c = kAcLeft + bt * kAcBatch + b;
+        const int x = x0 + bt * kAcBatch + b;
+        const bool xw = x < a.nrv;
+        auto lds_store = [&](int i, float2 val) { tl[i * kAcPitch + c] = val; };
+        auto lds_load = [&](int i) { return tl[i * kAcPitch + c]; };
+        constexpr int RI0 = Seq::rat(0);
+        pass_load<RI0>(v, lane, kAcLanes, N2, lds_load);
+        __syncthreads();  // every element of the column is read before the first exchange overwrites it
+        pass_compute<RI0, E, true>(v, lane, kAcLanes, N2, 1, tw2);
+        static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int Rp = Seq::rat(p - 1), R = Seq::rat(p);
+            pass_store<Rp>(v, lane, kAcLanes, N2, Seq::rprefix(p - 1), lds_store);
+            __syncthreads();
+            pass_load<R>(v, lane, kAcLanes, N2, lds_load);
+            __syncthreads();
+            pass_compute<R, E, true>(v, lane, kAcLanes, N2, Seq::rprefix(p), tw2);
+        });
+        float2* dst = a.out + (xw ? x : 0);
+        constexpr int RIL = Seq::rat(Seq::count - 1);
+        pass_visit<RIL>(v, lane, kAcLanes, N2, [&](int n, float2& val, bool ok) {
+            if (!ok || !xw) return;
+            const float2 wn = twf(k1 * n);  // k1 * n < n1 * n2
+            dst[(row0 + n) * a.stride] = cmulc(val, wn);
+        });
+    }
+}
+
+size_t az_core_lds_bytes(int n2, int nbins) {
+    return sizeof(float2) * (size_t)(n2 * kAcPitch + tw_lds_elems(n2) + tw_lds_elems(nbins));
+}
+
+int az_core_tile_columns() { return kAcOut; }
+
+template <int N2, class Seq>
+static hipError_t core_launch(const AzCoreArgs& a, size_t lds, bool prepare, hipStream_t s) {
+    if (prepare)
+        return hipFuncSetAttribute((const void*)k_az_core<N2, Seq>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds);
+    const unsigned grid = (unsigned)((a.items + 7) / 8 * 8);
+    hipLaunchKernelGGL((k_az_core<N2, Seq>), dim3(grid), dim3(kAcThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+// Second-pass lengths with a compiled core (the split of the azimuth FFT picks one of them).
+bool az_core_supported(int n2) { return n2 == 84 || n2 == 90 || n2 == 96 || n2 == 98; }
+
+static hipError_t core_dispatch(const AzCoreArgs& a, int n2, bool prepare, hipStream_t s) {
+    const size_t lds = az_core_lds_bytes(n2, a.nbins);
+    if (lds > max_lds_bytes()) return hipErrorInvalidValue;
+    switch (n2) {
+        case 84: return core_launch<84, RadixSeq<12, 7>>(a, lds, prepare, s);
+        case 90: return core_launch<90, RadixSeq<9, 10>>(a, lds, prepare, s);
+        case 96: return core_launch<96, RadixSeq<8, 12>>(a, lds, prepare, s);
+        case 98: return core_launch<98, RadixSeq<7, 14>>(a, lds, prepare, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Raises the kernel's dynamic LDS limit (once per plan, outside any graph capture).
+hipError_t az_core_prepare(int n2, int nbins) {
+    AzCoreArgs a{};
+    a.nbins = nbins;
+    return core_dispatch(a, n2, true, nullptr);
+}
+
+hipError_t launch_az_core(const AzCoreArgs& a, int n2, hipStream_t s) {
+    if (!a.in || !a.out || a.in == a.out || a.items <= 0) return hipErrorInvalidValue;
+    return core_dispatch(a, n2, false, s);
+}
+
+}  // namespace asarf

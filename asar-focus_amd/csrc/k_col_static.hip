@@ -6,22 +6,30 @@
 //   84 = 12 x 7, 49 = 7 x 7        (ERS block, 4116 = 84 x 49)
 //   112 = 16 x 7, 147 = 21 x 7     (high-migration scene, 16464 = 112 x 147)
 //   128 = 16 x 8, 256 = 16 x 16    (32768 = 128 x 256)
+//   360 = 15 x 24, 196 = 14 x 14   (first passes of the splits the fused azimuth core uses: 30240 = 360 x 84,
+//   336 = 12 x 28                   16464 = 196 x 84, 30240 = 336 x 90)
 #include "kernels.h"
 
 namespace asarf {
 
+// Lanes per column. (32 lanes for the long first-pass lengths 336 and 360 halve the register block, but the
+// compiler then schedules those kernels into 188-239 VGPRs, or spills under a 4-wave bound: 16 stays.)
+template <int N>
+constexpr int col_lanes() {
+    return 16;
+}
+
+// One work item: the COLS-column tile `tile` of transform group `g` (w = g * ntiles + tile).
 template <bool INV, int N, class Seq, int COLS>
-__global__ __launch_bounds__(16 * COLS) void k_col_static(ColFftArgs a) {
+__device__ __forceinline__ void col_static_item(const ColFftArgs& a, float2* lds, int w) {
     static_assert(Seq::product() == N, "radix sequence must factor N");
-    extern __shared__ float2 lds[];
-    if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
-    constexpr int L = 16;         // lanes per column
+    constexpr int L = col_lanes<N>();  // lanes per column
     constexpr int T = L * COLS;   // threads per workgroup
     constexpr int E = seq_max_elems<N, Seq, L>();
     constexpr int P = COLS + 1;
     const int ntiles = (a.ncols + COLS - 1) / COLS;
-    const int tile = blockIdx.x % ntiles;
-    const int g = blockIdx.x / ntiles;
+    const int tile = w % ntiles;
+    const int g = w / ntiles;
     const int b = threadIdx.x & (COLS - 1);
     const int lane = threadIdx.x / COLS;
     const int col = tile * COLS + b;
@@ -76,6 +84,25 @@ __global__ __launch_bounds__(16 * COLS) void k_col_static(ColFftArgs a) {
     pass_visit<RL>(v, lane, L, N, out_write);
 }
 
+// LOOP = false: one work item per workgroup (the grid covers every item). LOOP = true: a gated fallback (it works
+// only when the fused azimuth core does not), launched on a bounded grid whose workgroups take the items in turn,
+// so that skipping it costs few workgroup launches. Kept as two kernels: the loop costs the ungated passes 20 % (the
+// 360-point pass A: 0.57 -> 0.71 ms).
+template <bool INV, int N, class Seq, int COLS, bool LOOP>
+__global__ __launch_bounds__(col_lanes<N>() * COLS) void k_col_static(ColFftArgs a) {
+    extern __shared__ float2 lds[];
+    if (a.fused_gate && *a.fused_gate <= kAzCoreReach) return;
+    if constexpr (LOOP) {
+        const int items = (a.ncols + COLS - 1) / COLS * a.groups;
+        for (int w = blockIdx.x; w < items; w += gridDim.x) {
+            col_static_item<INV, N, Seq, COLS>(a, lds, w);
+            __syncthreads();  // the next item refills LDS
+        }
+    } else {
+        col_static_item<INV, N, Seq, COLS>(a, lds, blockIdx.x);
+    }
+}
+
 
 template <int N, class Seq, int COLS>
 static hipError_t launch_cs_w(const ColFftArgs& a, bool inverse, hipStream_t s) {
@@ -83,21 +110,47 @@ static hipError_t launch_cs_w(const ColFftArgs& a, bool inverse, hipStream_t s) 
         sizeof(float2) * (size_t)(N * (COLS + 1) + tw_lds_elems(N) + (a.tw_full ? tw_lds_elems(a.groups * N) : 0));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     const long ntiles = (a.ncols + COLS - 1) / COLS;
-    const dim3 grid((unsigned)(ntiles * a.groups));
-    if (inverse)
-        hipLaunchKernelGGL((k_col_static<true, N, Seq, COLS>), grid, dim3(16 * COLS), lds, s, a);
-    else
-        hipLaunchKernelGGL((k_col_static<false, N, Seq, COLS>), grid, dim3(16 * COLS), lds, s, a);
+    const long items = ntiles * a.groups;
+    const dim3 block(col_lanes<N>() * COLS);
+    // gated fallbacks of the contiguous passes (16 columns wide) take the bounded-grid loop; any other launch
+    // covers every item (a gated one still exits at once, on a full grid)
+    if (a.fused_gate && COLS == 16) {
+        const dim3 grid((unsigned)std::min(items, 1024L));
+        if (inverse)
+            hipLaunchKernelGGL((k_col_static<true, N, Seq, COLS, true>), grid, block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_col_static<false, N, Seq, COLS, true>), grid, block, lds, s, a);
+    } else {
+        const dim3 grid((unsigned)items);
+        if (inverse)
+            hipLaunchKernelGGL((k_col_static<true, N, Seq, COLS, false>), grid, block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_col_static<false, N, Seq, COLS, false>), grid, block, lds, s, a);
+    }
     return hipGetLastError();
 }
 
 // Columns per workgroup: ColFftArgs::col_width, or by access pattern when 0 (32 where the transform's elements are
 // strided rows, as in passes A and A' of the four-step split, so each row access is 256 contiguous bytes; 16 for
-// contiguous sub-transforms, whose 16-column tiles measured faster).
+// contiguous sub-transforms, whose 16-column tiles measured faster). The strided passes stay 32 wide even where
+// the tile leaves room for one workgroup per CU (N = 336, 360: 0.57 ms against 1.2 ms for 16-column tiles on the
+// 30240-line azimuth transform).
 template <int N, class Seq>
 static hipError_t launch_cs(const ColFftArgs& a, bool inverse, hipStream_t s) {
-    const int w = (a.col_width == 16 || a.col_width == 32) ? a.col_width : (a.in_m > 1 ? 32 : 16);
+    const bool wide_fits = sizeof(float2) * (size_t)N * 33 <= max_lds_bytes();
+    const int w = (a.col_width == 16 || a.col_width == 32) ? a.col_width : (a.in_m > 1 && wide_fits ? 32 : 16);
     return w == 32 ? launch_cs_w<N, Seq, 32>(a, inverse, s) : launch_cs_w<N, Seq, 16>(a, inverse, s);
+}
+
+// Sub-transform lengths with a compile-time kernel (the planner accepts them even where no runtime radix plan
+// fits the 16 lanes of a column).
+bool col_static_length(int n) {
+    switch (n) {
+        case 168: case 180: case 84: case 49: case 112: case 147: case 128: case 256: case 336: case 360: case 196:
+            return true;
+        default:
+            return false;
+    }
 }
 
 // Returns true (and the launch status in *err) when a compile-time kernel exists for the sub-transform length.
@@ -111,6 +164,9 @@ bool launch_col_static(const ColFftArgs& a, bool inverse, hipStream_t s, hipErro
         case 147: *err = launch_cs<147, RadixSeq<21, 7>>(a, inverse, s); return true;
         case 128: *err = launch_cs<128, RadixSeq<16, 8>>(a, inverse, s); return true;
         case 256: *err = launch_cs<256, RadixSeq<16, 16>>(a, inverse, s); return true;
+        case 336: *err = launch_cs<336, RadixSeq<12, 28>>(a, inverse, s); return true;
+        case 360: *err = launch_cs<360, RadixSeq<15, 24>>(a, inverse, s); return true;
+        case 196: *err = launch_cs<196, RadixSeq<14, 14>>(a, inverse, s); return true;
         default: return false;
     }
 }

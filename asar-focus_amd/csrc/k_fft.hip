@@ -138,11 +138,19 @@ __global__ __launch_bounds__(kRowThreads) void k_range_compress(RcArgs a) {
         const float qc = (q - c.dc_q - ic * c.sin_corr) * c.inv_cos;
         return make_float2(ic, qc);
     };
+    const int shift = FMT == RAW_ERS_LINES ? a.lines.shift(row) : 0;
     auto gl_load = [&](int x) -> float2 {
         if constexpr (FMT == RAW_CF32) {
             const float2* in = static_cast<const float2*>(a.in) + (long)row * a.in_stride;
             const float2 r = in[x];
             return correct(r.x, r.y);
+        } else if constexpr (FMT == RAW_ERS_LINES) {
+            // NaN where no line sample lands becomes 0 (ZeroNaNs); padding is 0
+            const int sx = x - shift;
+            if (x >= nr_in || sx < 0 || sx >= a.lines.spl) return make_float2(0.f, 0.f);
+            const uchar2 r = reinterpret_cast<const uchar2*>(static_cast<const uint8_t*>(a.in) +
+                                                              (long)row * a.lines.line_bytes)[sx];
+            return correct((float)r.x, (float)r.y);
         } else {
             if (x >= nr_in) return make_float2(0.f, 0.f);
             using P = typename RawPair<typename std::conditional<FMT == RAW_U8, uint8_t, int8_t>::type>::type;
@@ -194,7 +202,7 @@ hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
     if (a.mfma) return launch_rc_mfma(a, fmt, s);
     const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
-    if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8) return hipErrorInvalidValue;
+    if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8 && fmt != RAW_ERS_LINES) return hipErrorInvalidValue;
     hipError_t serr = hipSuccess;
     if (!a.no_static && launch_range_compress_static(a, fmt, s, &serr)) return serr;
     const int m = plan_mask(a.plan);
@@ -204,6 +212,9 @@ hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
             hipLaunchKernelGGL((k_range_compress<RAW_CF32, SET>), dim3(a.rows), dim3(kRowThreads), lds, s, a);  \
         else if (fmt == RAW_U8)                                                                                 \
             hipLaunchKernelGGL((k_range_compress<RAW_U8, SET>), dim3(a.rows), dim3(kRowThreads), lds, s, a);    \
+        else if (fmt == RAW_ERS_LINES)                                                                          \
+            hipLaunchKernelGGL((k_range_compress<RAW_ERS_LINES, SET>), dim3(a.rows), dim3(kRowThreads), lds, s, \
+                               a);                                                                              \
         else                                                                                                    \
             hipLaunchKernelGGL((k_range_compress<RAW_I8, SET>), dim3(a.rows), dim3(kRowThreads), lds, s, a);    \
         return hipGetLastError();                                                                               \
@@ -217,11 +228,9 @@ hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
 // Workgroup (tile, g): columns [16*tile, 16*tile+16), sub-transform g. Lane t -> column b = t % 16,
 // sub-lane s = t / 16 (16 sub-lanes per column). LDS layout element (m, b) at m*17 + b.
 template <bool INV, int RSET>
-__global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
-    extern __shared__ float2 lds[];
-    if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
-    const int tile = blockIdx.x % a.ntiles;
-    const int g = blockIdx.x / a.ntiles;
+__device__ __forceinline__ void col_fft_item(const ColFftArgs& a, float2* lds, int w) {
+    const int tile = w % a.ntiles;
+    const int g = w / a.ntiles;
     const int b = threadIdx.x & (kColTile - 1);
     const int lane = threadIdx.x / kColTile;
     constexpr int nlanes = kThreads / kColTile;
@@ -273,21 +282,44 @@ __global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
 #undef VISIT_WR_
 }
 
+// One work item per workgroup (LOOP = false), or a gated fallback on a bounded grid whose workgroups take the
+// items in turn (LOOP = true; see k_col_static).
+template <bool INV, int RSET, bool LOOP>
+__global__ __launch_bounds__(kThreads) void k_col_fft(ColFftArgs a) {
+    extern __shared__ float2 lds[];
+    if (a.fused_gate && *a.fused_gate <= kAzCoreReach) return;
+    if constexpr (LOOP) {
+        for (int w = blockIdx.x; w < a.ntiles * a.groups; w += gridDim.x) {
+            col_fft_item<INV, RSET>(a, lds, w);
+            __syncthreads();  // the next item refills LDS
+        }
+    } else {
+        col_fft_item<INV, RSET>(a, lds, blockIdx.x);
+    }
+}
+
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s) {
     if (a.mfma) return launch_col_mfma(a, inverse, s);
     hipError_t serr = hipSuccess;
     if (!a.no_static && launch_col_static(a, inverse, s, &serr)) return serr;
+    if (a.sub.npass == 0) return hipErrorInvalidValue;  // a length planned for its compile-time kernel only
     const size_t lds = sizeof(float2) * (size_t)(a.sub.n * (kColTile + 1) + tw_lds_elems(a.sub.n));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((long)a.ntiles * a.groups));
+    const long items = (long)a.ntiles * a.groups;
+    const bool loop = a.fused_gate != nullptr;
+    const dim3 grid((unsigned)(loop ? std::min(items, 1024L) : items));
     const int m = plan_mask(a.sub);
-#define X_(SET)                                                                               \
-    if ((m & ~(SET)) == 0) {                                                                  \
-        if (inverse)                                                                          \
-            hipLaunchKernelGGL((k_col_fft<true, SET>), grid, dim3(kThreads), lds, s, a);      \
-        else                                                                                  \
-            hipLaunchKernelGGL((k_col_fft<false, SET>), grid, dim3(kThreads), lds, s, a);     \
-        return hipGetLastError();                                                             \
+#define X_(SET)                                                                                         \
+    if ((m & ~(SET)) == 0) {                                                                            \
+        if (inverse && loop)                                                                            \
+            hipLaunchKernelGGL((k_col_fft<true, SET, true>), grid, dim3(kThreads), lds, s, a);          \
+        else if (inverse)                                                                               \
+            hipLaunchKernelGGL((k_col_fft<true, SET, false>), grid, dim3(kThreads), lds, s, a);         \
+        else if (loop)                                                                                  \
+            hipLaunchKernelGGL((k_col_fft<false, SET, true>), grid, dim3(kThreads), lds, s, a);         \
+        else                                                                                            \
+            hipLaunchKernelGGL((k_col_fft<false, SET, false>), grid, dim3(kThreads), lds, s, a);        \
+        return hipGetLastError();                                                                       \
     }
     COL_SETS(X_)
 #undef X_

@@ -9,8 +9,11 @@ namespace asarf {
 // Each lane owns one range column of a tile of `rows` output rows. The |s|^2 of every input row the tile needs
 // (at most kMlSpan) is loaded first, all loads independent and in flight together, into the lane's own LDS
 // column (no barrier: a lane only reads what it wrote); the boxcars and the interpolation then run from LDS.
-// (Two columns per lane with 16-byte loads measured slower: the LDS per wave doubles and halves the waves per CU.)
-constexpr int kMlSpan = 32;     // input rows per tile
+// Neighbouring tiles share the rows of one output's boxcars (about 6 rows): with a 48-row span a tile produces 15
+// outputs of the ASAR scene (ratio 2.9) from 47 rows, 8 % more reads than the outputs consume (a 32-row span
+// made it 13 %). (A ring that walks each column once, 24 rows a batch, measured slower: each batch waits for
+// its own loads.)
+constexpr int kMlSpan = 48;     // input rows per tile
 constexpr int kMlLanes = 256;
 
 __global__ __launch_bounds__(kMlLanes) void k_multilook(MultilookArgs a, int rows) {
@@ -29,13 +32,12 @@ __global__ __launch_bounds__(kMlLanes) void k_multilook(MultilookArgs a, int row
     const int span = min(hi - lo, kMlSpan);  // the host sizes tiles so that hi - lo <= kMlSpan
     const bool xok = x < a.nr;
     const float2* col = a.slc + (long)(row0 + lo) * a.stride + a.col0 + (xok ? x : 0);
+    // rows past the span load a clamped copy that no output reads (unconditional loads stay in flight together)
     float2 v[kMlSpan];
 #pragma unroll
-    for (int i = 0; i < kMlSpan; ++i)
-        if (i < span) v[i] = col[(long)i * a.stride];
+    for (int i = 0; i < kMlSpan; ++i) v[i] = col[(long)min(i, max(span - 1, 0)) * a.stride];
 #pragma unroll
-    for (int i = 0; i < kMlSpan; ++i)
-        if (i < span) pw[i][threadIdx.x] = v[i].x * v[i].x + v[i].y * v[i].y;
+    for (int i = 0; i < kMlSpan; ++i) pw[i][threadIdx.x] = v[i].x * v[i].x + v[i].y * v[i].y;
     if (!xok) return;
     for (int y = y0; y < y1; ++y) {
         const float in_idx = (float)y * a.ratio;
@@ -63,7 +65,7 @@ hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s) {
     if (!(a.ratio > 0.f) || a.ratio > (float)(kMlSpan - 8)) return hipErrorInvalidValue;
     // rows per tile: the span (rows - 1) * ratio + 6 must fit kMlSpan (with one row of margin for rounding)
     int rows = (int)((float)(kMlSpan - 7) / a.ratio) + 1;
-    rows = max(1, min(rows, 16));
+    rows = max(1, min(rows, 24));
     hipLaunchKernelGGL(k_multilook, dim3((a.nr + kMlLanes - 1) / kMlLanes, (a.out_rows + rows - 1) / rows),
                        dim3(kMlLanes), 0, s, a, rows);
     return hipGetLastError();
@@ -83,10 +85,13 @@ hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_
 }
 
 // One lane per output column and a tile of kSrgrRows rows: the FP64 polynomial and the source index are
-// computed once per column and reused for every row of the tile.
+// computed once per column and reused for every row of the tile. 1024 columns per workgroup: the input span of
+// a workgroup row (about 1.6 input columns per output column) starts and ends in partly used cache lines, which
+// cost 16 % extra reads with 256-column workgroups and 4 % with 1024.
 constexpr int kSrgrRows = 16;
+constexpr int kSrgrLanes = 1024;
 
-__global__ __launch_bounds__(256) void k_srgr(SrgrArgs a) {
+__global__ __launch_bounds__(kSrgrLanes) void k_srgr(SrgrArgs a) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int rows = min(a.rows_dev ? *a.rows_dev : a.rows, a.rows);
     const int y0 = blockIdx.y * kSrgrRows;
@@ -115,8 +120,8 @@ __global__ __launch_bounds__(256) void k_srgr(SrgrArgs a) {
 
 hipError_t launch_srgr(const SrgrArgs& a, hipStream_t s) {
     if (a.rows <= 0 || a.out_nr <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_srgr, dim3((a.out_nr + 255) / 256, (a.rows + kSrgrRows - 1) / kSrgrRows), dim3(256), 0, s,
-                       a);
+    hipLaunchKernelGGL(k_srgr, dim3((a.out_nr + kSrgrLanes - 1) / kSrgrLanes, (a.rows + kSrgrRows - 1) / kSrgrRows),
+                       dim3(kSrgrLanes), 0, s, a);
     return hipGetLastError();
 }
 

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kMfRowT) void k_row_mfma(RowFftArgs a) {
 template <int N, bool INV>
 __global__ __launch_bounds__(kThreads) void k_col_mfma(ColFftArgs a) {
     extern __shared__ float2 lds[];
-    if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
+    if (a.fused_gate && *a.fused_gate <= kAzCoreReach) return;
     constexpr int P = kColTile + 1;
     float2* buf = lds;
     const int tile = blockIdx.x % a.ntiles;

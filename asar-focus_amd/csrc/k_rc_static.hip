@@ -175,6 +175,9 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
     else if (fmt == RAW_U8)
         hipLaunchKernelGGL((k_rc_static<RAW_U8, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T), lds,
                            s, a);
+    else if (fmt == RAW_ERS_LINES)
+        hipLaunchKernelGGL((k_rc_static<RAW_ERS_LINES, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T),
+                           lds, s, a);
     else
         hipLaunchKernelGGL((k_rc_static<RAW_I8, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T), lds,
                            s, a);
@@ -183,7 +186,7 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
 
 // Compile-time variants per production length (radix sequence, lanes per line); the autotuner times them and
 // records the fastest (RcArgs::static_variant). Variant 0 is the default.
-int range_compress_static_variants(int n) {
+static int stockham_variants(int n) {
     switch (n) {
         case 6144: return 8;
         case 6400: return 6;
@@ -192,11 +195,19 @@ int range_compress_static_variants(int n) {
     }
 }
 
+// Variants past the Stockham ones are the padded-layout kernels of k_rc_pad.hip.
+int range_compress_static_variants(int n) { return stockham_variants(n) + range_compress_pad_variants(n); }
+
 // Returns true (and the launch status in *err) when a specialisation exists for the plan's length. Variant 0 is
 // the fastest measured on the ASAR/ERS scenes (512 lanes with a 4-wave occupancy target keeps 16 waves per CU;
 // for 6144 the raw bytes are also staged through LDS with 16-byte loads).
 bool launch_range_compress_static(const RcArgs& a, int fmt, hipStream_t s, hipError_t* err) {
-    const int v = a.static_variant;
+    int v = a.static_variant;
+    // 6144: variant 0 is the padded-layout kernel (measured fastest on the ASAR scene); the Stockham kernel that was
+    // the default before it takes index 8
+    if (a.plan.n == 6144) v = (v == 0) ? stockham_variants(6144) : (v == stockham_variants(6144)) ? 0 : v;
+    if (v >= stockham_variants(a.plan.n) && v > 0)
+        return launch_range_compress_pad(a, fmt, v - stockham_variants(a.plan.n), s, err);
     switch (a.plan.n) {
         case 6144:
             switch (v) {

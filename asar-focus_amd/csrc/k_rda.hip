@@ -68,16 +68,6 @@ hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-__device__ __forceinline__ double alias_about_dc(double f, double dc, double prf) {
-    const double half = 0.5 * prf;
-    if (dc < 0.0) {
-        if (f > half + dc) f -= prf;
-    } else if (dc > 0.0) {
-        if (f < -half + dc) f += prf;
-    }
-    return f;
-}
-
 // Plain 8-tap Blackman sinc at fractional offset d (d != 0), taps tap[i] at offsets 4 - i (same normalised form
 // as the 16-tap interpolator).
 __device__ __forceinline__ float2 sinc8_interp(const float2* tap, const float* window8, float d) {
@@ -121,7 +111,7 @@ __device__ __forceinline__ bool learned_interp(const RcmcArgs& a, const float2* 
 constexpr int kRcmcThreads = 512;
 __global__ __launch_bounds__(kRcmcThreads) __attribute__((amdgpu_waves_per_eu(7))) void k_rcmc_mf(RcmcArgs a) {
     extern __shared__ float2 row[];
-    if (a.fused_gate && *a.fused_gate <= kAzFusedReach) return;
+    if (a.fused_gate && *a.fused_gate <= kAzCoreReach) return;
     if (a.tile_gate && *a.tile_gate <= kRcmcTileReach) return;
     const int nrv = a.nrv;
     const double step = a.prf / (double)a.nbins;
@@ -196,12 +186,12 @@ constexpr int kRtCols = 256;
 constexpr int kRtRows = 32;
 constexpr int kRtSeg = kRtCols + 16 + kRcmcTileReach;
 
+// One tile: kRtCols columns from x0 by kRtRows bins from q0 (item = bin tile * column tiles + column tile).
 template <int INTERP>
-__global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
-    __shared__ float2 seg[2][kRtSeg];
-    if (a.tile_gate && *a.tile_gate > kRcmcTileReach) return;
-    const int x0 = blockIdx.x * kRtCols;
-    const int q0 = blockIdx.y * kRtRows;
+__device__ __forceinline__ void rcmc_tile_item(const RcmcArgs& a, float2 (*seg)[kRtSeg], int item) {
+    const int ntx = (a.nrv + kRtCols - 1) / kRtCols;
+    const int x0 = (item % ntx) * kRtCols;
+    const int q0 = (item / ntx) * kRtRows;
     const int q1 = min(q0 + kRtRows, a.nbins);
     const int t = threadIdx.x;
     const int nrv = a.nrv;
@@ -287,14 +277,40 @@ __global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
     }
 }
 
+// One tile per workgroup (LOOP = false), or a gated fallback on a bounded grid whose workgroups take the tiles in
+// turn (LOOP = true), as the column kernels do.
+template <int INTERP, bool LOOP>
+__global__ __launch_bounds__(kRtCols) void k_rcmc_tile(RcmcArgs a) {
+    __shared__ float2 seg[2][kRtSeg];
+    if (a.tile_gate && *a.tile_gate > kRcmcTileReach) return;
+    if (a.fused_gate && *a.fused_gate <= kAzCoreReach) return;
+    if constexpr (LOOP) {
+        const int nitems = (a.nrv + kRtCols - 1) / kRtCols * ((a.nbins + kRtRows - 1) / kRtRows);
+        for (int item = blockIdx.x; item < nitems; item += gridDim.x) rcmc_tile_item<INTERP>(a, seg, item);
+    } else {
+        rcmc_tile_item<INTERP>(a, seg, blockIdx.y * gridDim.x + blockIdx.x);
+    }
+}
+
 hipError_t launch_rcmc_tile(const RcmcArgs& args, hipStream_t s) {
     const RcmcArgs a = with_window(args);
     if (!a.out || a.out == a.img) return hipErrorInvalidValue;  // out of place only
+    // a gated fallback (it works only when the fused azimuth core does not) runs on a bounded grid
+    if (a.fused_gate) {
+        const dim3 grid(1024);
+        switch (a.interp) {
+            case 0: hipLaunchKernelGGL((k_rcmc_tile<0, true>), grid, dim3(kRtCols), 0, s, a); break;
+            case 1: hipLaunchKernelGGL((k_rcmc_tile<1, true>), grid, dim3(kRtCols), 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_rcmc_tile<2, true>), grid, dim3(kRtCols), 0, s, a); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const dim3 grid((a.nrv + kRtCols - 1) / kRtCols, (a.nbins + kRtRows - 1) / kRtRows);
     switch (a.interp) {
-        case 0: hipLaunchKernelGGL(k_rcmc_tile<0>, grid, dim3(kRtCols), 0, s, a); break;
-        case 1: hipLaunchKernelGGL(k_rcmc_tile<1>, grid, dim3(kRtCols), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_rcmc_tile<2>, grid, dim3(kRtCols), 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_rcmc_tile<0, false>), grid, dim3(kRtCols), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_rcmc_tile<1, false>), grid, dim3(kRtCols), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_rcmc_tile<2, false>), grid, dim3(kRtCols), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

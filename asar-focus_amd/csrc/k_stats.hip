@@ -190,7 +190,11 @@ __device__ __forceinline__ T wave_sum(T v) {
 // preceding row). With V = 4 a row of a lane is one 8-byte load (4 I/Q byte pairs). Statistics are reduced over
 // the workgroup; lag sums per Doppler block (slot = block - first block of the tile) through per-column LDS rows.
 // Integer ranges: a column's sums over 128 rows stay below 128 * 255^2 < 2^31.
-template <typename P, int V>
+// LINES: the rows are ERS level-0 lines (RawMomentsArgs::lines): row y of range column x holds line sample
+// x - shift(y) when that is one of the line's samples, else NaN, which the statistics skip and the lag-1 sums treat
+// as the zero ZeroNaNs makes it (a pair counts only when both of its samples are valid). V = 4 needs the shifts
+// and the samples per line to be multiples of 4, so a lane's four samples are valid or NaN together.
+template <typename P, int V, bool LINES = false>
 __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
     constexpr bool SIGNED = std::is_same<P, char2>::value;
     constexpr int TW = kThreads * V;
@@ -210,7 +214,24 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
     int rows = 0, lag_rows = 0;
     // one row of the lane's V samples as raw bytes (V = 4: one 8-byte word; V = 1: one I/Q pair), unpacked later
     using W = typename std::conditional<V == 4, uint2, unsigned short>::type;
-    auto load_row = [&](int y) -> W { return *reinterpret_cast<const W*>(raw + ((long)y * a.nr + x0) * 2); };
+    // LINES: whether row y has samples at the lane's columns (all V of them or none) and where they are
+    auto line_sample = [&](int y) { return x0 - a.lines.shift(y); };
+    auto row_valid = [&](int y) {
+        if constexpr (LINES) {
+            const int s0 = line_sample(y);
+            return s0 >= 0 && s0 + V <= a.lines.spl;
+        } else {
+            return true;
+        }
+    };
+    auto load_row = [&](int y) -> W {
+        if constexpr (LINES) {
+            if (!row_valid(y)) return W{};
+            return *reinterpret_cast<const W*>(raw + (long)y * a.lines.line_bytes + (long)line_sample(y) * 2);
+        } else {
+            return *reinterpret_cast<const W*>(raw + ((long)y * a.nr + x0) * 2);
+        }
+    };
     auto sample = [&](const W& w, int j, int& si, int& sq) {
         unsigned int word;
         if constexpr (V == 4)
@@ -229,7 +250,7 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
             const W w = load_row(y0 - 1);
 #pragma unroll
             for (int j = 0; j < V; ++j) sample(w, j, pi[j], pq[j]);
-            have_prev = true;
+            have_prev = row_valid(y0 - 1);
         }
         for (int y = y0; y < y1; y += U) {
             W w[U];
@@ -238,16 +259,19 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (y + u >= y1) break;
+                const bool ok = row_valid(y + u);  // always true without LINES
 #pragma unroll
                 for (int j = 0; j < V; ++j) {
                     int c_i, c_q;
                     sample(w[u], j, c_i, c_q);
-                    st[0] += c_i;
-                    st[1] += c_q;
-                    st[2] += c_i * c_i;
-                    st[3] += c_q * c_q;
-                    st[4] += c_i * c_q;
-                    if (have_prev) {
+                    if (ok) {
+                        st[0] += c_i;
+                        st[1] += c_q;
+                        st[2] += c_i * c_i;
+                        st[3] += c_q * c_q;
+                        st[4] += c_i * c_q;
+                    }
+                    if (have_prev && ok) {
                         lg[j][0] += pi[j] * c_i;
                         lg[j][1] += pq[j] * c_q;
                         lg[j][2] += pi[j] * c_q;
@@ -260,9 +284,9 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
                     pi[j] = c_i;
                     pq[j] = c_q;
                 }
-                ++rows;
-                if (have_prev) ++lag_rows;
-                have_prev = true;
+                if (ok) ++rows;
+                if (have_prev && ok) ++lag_rows;
+                have_prev = ok;
             }
         }
     }
@@ -308,9 +332,211 @@ __global__ __launch_bounds__(kThreads) void k_raw_moments(RawMomentsArgs a) {
     }
 }
 
+// Eight columns per lane (one 16-byte load per row: four 32-bit words of two I/Q byte pairs), with the sums of
+// squares, cross products and lag-1 products of both pairs of a word in one packed 8-bit dot product each:
+// with mi = w & 0x00FF00FF (the two I bytes) and mq = (w >> 8) & 0x00FF00FF (the two Q bytes),
+// dot4(mi, mi) = I0^2 + I1^2, dot4(mi, mq) = I0 Q0 + I1 Q1, dot4(mi_prev, mq) = I0' Q0 + I1' Q1, ...
+// Lag sums are kept per word and filed under the Doppler block of the word's first column; where a block starts
+// at an odd column the word straddles two blocks, and the lane holding that column also sums its lag products
+// alone (corr[strip][block] of the arguments, plain integer products in a branch only such lanes take), which the reduction
+// moves to its own block. The lag-row sums of I and Q follow from the per-word column sums and the rows at the
+// strip's edges, so the loop carries nine dot products per word and row.
+template <bool SIGNED>
+__device__ __forceinline__ int dot4(unsigned a, unsigned b, int c) {
+    if constexpr (SIGNED)
+        return __builtin_amdgcn_sdot4((int)a, (int)b, c, false);
+    else
+        return (int)__builtin_amdgcn_udot4(a, b, (unsigned)c, false);
+}
+
+template <bool SIGNED>
+__global__ __launch_bounds__(kThreads) void k_raw_moments8(RawMomentsArgs a) {
+    constexpr int V = 8, NW = V / 2, TW = kThreads * V;
+    constexpr unsigned MI = 0x00FF00FFu, ONES = 0x01010101u;
+    const int tile = blockIdx.x;
+    const int strip = blockIdx.y;
+    const int x0 = (tile * kThreads + threadIdx.x) * V;
+    const int y0 = strip * kStripRows;
+    const int y1 = min(y0 + kStripRows, a.na);
+    const bool valid = x0 < a.nr;  // nr % 8 == 0: all eight columns of the lane, or none
+    const unsigned char* raw = static_cast<const unsigned char*>(a.raw);
+    auto load_row = [&](int y) -> uint4 { return *reinterpret_cast<const uint4*>(raw + ((long)y * a.nr + x0) * 2); };
+    int sii = 0, sqq = 0, siq = 0;
+    int wi[NW], wq[NW], lii[NW], lqq[NW], liq[NW], lqi[NW];
+    unsigned pmi[NW], pmq[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) wi[k] = wq[k] = lii[k] = lqq[k] = liq[k] = lqi[k] = 0, pmi[k] = pmq[k] = 0;
+    int edge_i[NW], edge_q[NW];  // row y0 - 1 (strip > 0) or row 0 (strip 0), per word
+    bool have_prev = false;
+    // the lane's block start at an odd column c = sb * step (step >= 8: at most one in the lane's eight columns),
+    // the second column of word sk; its lag sums over the strip's lag rows
+    int sb = -1, sk = 0;
+    if (valid) {
+        const int bb = (x0 + a.step - 1) / a.step;
+        const int cs = bb * a.step;
+        if (bb >= 1 && bb < a.nblocks && cs < x0 + V && (cs & 1)) {
+            sb = bb;
+            sk = (cs - x0) >> 1;
+        }
+    }
+    int s_ii = 0, s_qq = 0, s_iq = 0, s_qi = 0, s_pi = 0, s_pq = 0, s_ci = 0, s_cq = 0, s_n = 0;
+    int s_prev_i = 0, s_prev_q = 0;
+    auto byte_at = [](unsigned word, int shift) -> int {
+        return SIGNED ? (int)(signed char)(word >> shift) : (int)((word >> shift) & 0xFFu);
+    };
+    auto pick = [&](const unsigned (&ws)[NW]) -> unsigned {
+        return sk == 0 ? ws[0] : sk == 1 ? ws[1] : sk == 2 ? ws[2] : ws[3];
+    };
+    if (valid) {
+        const uint4 w = load_row(y0 > 0 ? y0 - 1 : 0);
+        const unsigned ws[NW] = {w.x, w.y, w.z, w.w};
+        if (sb >= 0) {
+            const unsigned e = pick(ws);
+            s_prev_i = byte_at(e, 16);
+            s_prev_q = byte_at(e, 24);
+        }
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const unsigned mi = ws[k] & MI, mq = (ws[k] >> 8) & MI;
+            edge_i[k] = dot4<SIGNED>(mi, ONES, 0);
+            edge_q[k] = dot4<SIGNED>(mq, ONES, 0);
+            pmi[k] = mi;
+            pmq[k] = mq;
+        }
+        have_prev = y0 > 0;
+        constexpr int U = 8;  // rows loaded per iteration (independent loads in flight)
+        for (int y = y0; y < y1; y += U) {
+            uint4 w8[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) w8[u] = (y + u < y1) ? load_row(y + u) : uint4{};
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (y + u >= y1) break;
+                const unsigned ws2[NW] = {w8[u].x, w8[u].y, w8[u].z, w8[u].w};
+#pragma unroll
+                for (int k = 0; k < NW; ++k) {
+                    const unsigned mi = ws2[k] & MI, mq = (ws2[k] >> 8) & MI;
+                    sii = dot4<SIGNED>(mi, mi, sii);
+                    sqq = dot4<SIGNED>(mq, mq, sqq);
+                    siq = dot4<SIGNED>(mi, mq, siq);
+                    wi[k] = dot4<SIGNED>(mi, ONES, wi[k]);
+                    wq[k] = dot4<SIGNED>(mq, ONES, wq[k]);
+                    if (have_prev) {
+                        lii[k] = dot4<SIGNED>(pmi[k], mi, lii[k]);
+                        lqq[k] = dot4<SIGNED>(pmq[k], mq, lqq[k]);
+                        liq[k] = dot4<SIGNED>(pmi[k], mq, liq[k]);
+                        lqi[k] = dot4<SIGNED>(pmq[k], mi, lqi[k]);
+                    }
+                    pmi[k] = mi;
+                    pmq[k] = mq;
+                }
+                if (sb >= 0) {
+                    const unsigned e = pick(ws2);
+                    const int ci = byte_at(e, 16), cq = byte_at(e, 24);
+                    if (have_prev) {
+                        s_ii += s_prev_i * ci;
+                        s_qq += s_prev_q * cq;
+                        s_iq += s_prev_i * cq;
+                        s_qi += s_prev_q * ci;
+                        s_pi += s_prev_i;
+                        s_pq += s_prev_q;
+                        s_ci += ci;
+                        s_cq += cq;
+                        ++s_n;
+                    }
+                    s_prev_i = ci;
+                    s_prev_q = cq;
+                }
+                have_prev = true;
+            }
+        }
+    }
+    if (sb >= 0) {
+        const int sv[kRawLag] = {s_ii, s_qq, s_iq, s_qi, s_pi, s_pq, s_ci, s_cq, s_n};
+        long long* cp = a.corr + ((long)strip * a.nblocks + sb) * kRawLag;
+#pragma unroll
+        for (int i = 0; i < kRawLag; ++i) cp[i] = sv[i];
+    }
+    const int rows = valid ? y1 - y0 : 0;
+    const int lag_rows = valid ? (y1 - y0) - (y0 == 0 ? 1 : 0) : 0;
+    const int nval = kRawStats + a.nslots * kRawLag;
+    long long* out = a.partials + ((long)strip * a.ntiles + tile) * nval;
+    __shared__ long long shs[kThreads / 64][kRawStats];
+    __shared__ int shl[kRawLag][TW / 2];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int si = 0, sq = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) si += wi[k], sq += wq[k];
+    const long long stv[kRawStats] = {si, sq, sii, sqq, siq, (long long)rows * V};
+#pragma unroll
+    for (int i = 0; i < kRawStats; ++i) {
+        const long long v = wave_sum<long long>(stv[i]);
+        if (lane == 0) shs[wave][i] = v;
+    }
+    // lag sums of each word: cur over the lag rows = strip sum less row 0 (strip 0 only); prev = strip sum less
+    // the last row plus the row above the strip (strips after the first)
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        const int last_i = dot4<SIGNED>(pmi[k], ONES, 0), last_q = dot4<SIGNED>(pmq[k], ONES, 0);
+        const int li2 = y0 == 0 ? wi[k] - edge_i[k] : wi[k];
+        const int lq2 = y0 == 0 ? wq[k] - edge_q[k] : wq[k];
+        const int li1 = wi[k] - last_i + (y0 > 0 ? edge_i[k] : 0);
+        const int lq1 = wq[k] - last_q + (y0 > 0 ? edge_q[k] : 0);
+        const int c = threadIdx.x * NW + k;
+        const int vals[kRawLag] = {lii[k], lqq[k], liq[k], lqi[k], li1, lq1, li2, lq2, 2 * lag_rows};
+#pragma unroll
+        for (int i = 0; i < kRawLag; ++i) shl[i][c] = valid ? vals[i] : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < kRawStats) {
+        long long t = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) t += shs[w][threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+    // lag sums per block slot: the words whose first column lies in the block
+    const int c0 = tile * TW;
+    const int b0 = c0 / a.step;
+    for (int k = threadIdx.x; k < a.nslots * kRawLag; k += kThreads) {
+        const int slot = k / kRawLag, v = k % kRawLag;
+        const int b = b0 + slot;
+        long long t = 0;
+        if (b < a.nblocks) {
+            // words [l0, l1): first columns c0 + 2l in [b step, (b + 1) step)
+            const int l0 = max(0, (b * a.step - c0 + 1) / 2);
+            const int l1 = min(TW / 2, ((b + 1) * a.step - c0 + 1) / 2);
+            for (int l = l0; l < l1; ++l) t += shl[v][l];
+        }
+        out[kRawStats + k] = t;
+    }
+}
+
 hipError_t launch_raw_moments(const RawMomentsArgs& a, int fmt, hipStream_t s) {
     const dim3 grid(a.ntiles, a.nstrips);
+    if (a.vec == 8) {
+        // 16-byte loads of whole rows of eight pairs, at most one block boundary per lane, the straddle columns
+        // into a.corr
+        if (a.nr % 8 != 0 || (reinterpret_cast<uintptr_t>(a.raw) & 15) != 0 || a.step < 8 || !a.corr ||
+            (fmt != RAW_U8 && fmt != RAW_I8))
+            return hipErrorInvalidValue;
+        if (fmt == RAW_U8)
+            hipLaunchKernelGGL(k_raw_moments8<false>, grid, dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_raw_moments8<true>, grid, dim3(kThreads), 0, s, a);
+        return hipGetLastError();
+    }
     if (a.vec != 1 && a.vec != 4) return hipErrorInvalidValue;
+    if (fmt == RAW_ERS_LINES) {
+        // V = 4: 8-byte aligned rows of whole lane groups, shifted by multiples of 4 samples
+        if (a.vec == 4 && (a.nr % 4 != 0 || a.lines.spl % 4 != 0 || a.lines.mult % 4 != 0 ||
+                           a.lines.line_bytes % 8 != 0 || (reinterpret_cast<uintptr_t>(a.raw) & 7) != 0))
+            return hipErrorInvalidValue;
+        if (a.vec == 4)
+            hipLaunchKernelGGL((k_raw_moments<uchar2, 4, true>), grid, dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_raw_moments<uchar2, 1, true>), grid, dim3(kThreads), 0, s, a);
+        return hipGetLastError();
+    }
     // 8-byte loads: rows and the base must be 8-byte aligned
     if (a.vec == 4 && (a.nr % 4 != 0 || (reinterpret_cast<uintptr_t>(a.raw) & 7) != 0)) return hipErrorInvalidValue;
     if (fmt == RAW_U8) {
@@ -349,6 +575,16 @@ __global__ __launch_bounds__(kThreads) void k_reduce_raw(RawReduceArgs a) {
             const int strip = (int)(p / nt), t = t_lo + (int)(p % nt);
             const int slot = b - (t * a.tile_cols) / a.step;
             if (slot >= 0 && slot < a.nslots) acc += a.partials[((long)strip * a.ntiles + t) * nval + kRawStats + slot * kRawLag + k];
+        }
+        // eight-column tiling: a column that starts block b at an odd index was filed under block b - 1
+        if (a.corr) {
+            const bool own = b > 0 && ((b * a.step) & 1);
+            const bool next = b + 1 < a.nblocks && (((b + 1) * a.step) & 1);
+            for (int p = threadIdx.x; p < a.nstrips && (own || next); p += kThreads) {
+                const long base = (long)p * a.nblocks;
+                if (own) acc += a.corr[(base + b) * kRawLag + k];
+                if (next) acc -= a.corr[(base + b + 1) * kRawLag + k];
+            }
         }
     }
     __shared__ long long sh[kThreads];

@@ -18,16 +18,23 @@ constexpr int kStripRows = 128; // rows per workgroup in the raw statistics pass
 constexpr int kRawStats = 6;    // SI, SQ, SII, SQQ, SIQ, N
 constexpr int kRawLag = 9;      // LII, LQQ, LIQ, LQI, LI1, LQ1, LI2, LQ2, M per Doppler block
 constexpr int kMaxDcBlocks = 64;
-// Largest integer RCMC shift (pixels) the fused azimuth kernel's range halo covers; above it the plan runs the
-// separate RCMC/matched-filter and inverse-pass kernels instead.
-constexpr int kAzFusedReach = 8;
+// Largest integer RCMC shift (pixels) the fused azimuth core's range halo covers; above it the plan runs the
+// separate pass-B, RCMC/matched-filter and pass-B' kernels instead.
+constexpr int kAzCoreReach = 8;
 // Largest integer RCMC shift (pixels) covered by the column-tiled RCMC kernel's right halo.
 constexpr int kRcmcTileReach = 24;
-// Default upper bound on the second-pass length of the column split when the fused azimuth kernel runs (its
-// LDS tile is n2 x 41 complex samples, so a shorter second pass fits more workgroups per CU); 0 = balanced.
-constexpr int kAzFusedN2Max = 0;
 
-enum RawFormat { RAW_CF32 = 0, RAW_U8 = 1, RAW_I8 = 2 };
+enum RawFormat { RAW_CF32 = 0, RAW_U8 = 1, RAW_I8 = 2, RAW_ERS_LINES = 3 };
+
+// ERS IM level-0 lines (RAW_ERS_LINES): line y holds spl u8 (I, Q) pairs at bytes + y * line_bytes; its sample s
+// lands on range sample s + mult * (swst[y] - min_swst); range samples below the row's logical width that no
+// sample lands on are NaN (zero after the I/Q correction, ZeroNaNs).
+struct LinesDesc {
+    const short* swst;
+    int min_swst, mult, spl;
+    long line_bytes;
+    __device__ __forceinline__ int shift(int y) const { return mult * ((int)swst[y] - min_swst); }
+};
 
 // Affine I/Q correction: I' = (I - dc_i) * inv_gain, Q' = (Q - dc_q - I' * sin_corr) * inv_cos.
 struct IqCoef {
@@ -83,6 +90,7 @@ struct RcArgs {
     int mfma;           // split-bfloat16 MFMA transforms (length 8192 only)
     int no_static;      // use the runtime-radix kernel even where a compile-time kernel exists (autotuner)
     int static_variant; // which compile-time variant of the length (autotuner; 0 = default)
+    LinesDesc lines;    // RAW_ERS_LINES
 };
 
 struct ColFftArgs {
@@ -100,7 +108,7 @@ struct ColFftArgs {
     const float2* tw_full; // post-twiddle W_full^{g*k} (conjugated for the inverse); nullptr = none
     float scale;
     FftPlanDev sub;
-    const int* fused_gate; // non-null: do nothing when *fused_gate <= kAzFusedReach (the fused kernel ran)
+    const int* fused_gate; // non-null: do nothing when *fused_gate <= kAzCoreReach (the fused kernel ran)
     int mfma;              // split-bfloat16 MFMA sub-transform (lengths 128 and 256 only)
     int no_static;         // 1: runtime-radix kernel even where a compile-time kernel exists (comparison runs)
     int col_width;         // columns per workgroup of the compile-time kernels: 16, 32, or 0 = by access pattern
@@ -116,7 +124,7 @@ struct RcmcArgs {
     double prf;
     double beta;      // azimuth bandwidth fraction
     int mode;         // 0 = RCMC + matched filter, 1 = RCMC only, 2 = matched filter only
-    const int* fused_gate;  // non-null: do nothing when *fused_gate <= kAzFusedReach (the fused kernel ran)
+    const int* fused_gate;  // non-null: do nothing when *fused_gate <= kAzCoreReach (the fused kernel ran)
     // interpolator: 0 = 16-tap Blackman sinc (the reference's), 1 = learned 8-tap table where its judge cell
     // passed (16-tap sinc elsewhere), 2 = plain 8-tap Blackman sinc
     int interp;
@@ -129,22 +137,22 @@ struct RcmcArgs {
     float window[16];  // Blackman window of the 16-tap interpolator (filled by the launchers)
 };
 
-// Fused RCMC + matched filter + inverse pass B' (out of place; four-step split n1 x n2).
-struct AzFusedArgs {
-    const float2* in;     // pass-B output: row k1*n2 + k2 holds bin k1 + n1*k2
-    float2* out;          // inverse pass-B' output with the inverse inter-pass twiddle applied
+// Fused azimuth core: forward pass B + RCMC + matched filter + inverse pass B' (out of place; split n1 x n2).
+struct AzCoreArgs {
+    const float2* in;      // pass-A output: row k1*n2 + m holds element m of group k1's length-n2 transform
+    float2* out;           // inverse pass-B' output with the inverse inter-pass twiddle applied
     long stride;
     int nrv;
-    int nbins;            // n1 * n2
+    int nbins;             // n1 * n2
     int n1;
-    int ntiles;           // ceil(nrv / 16)
-    int items;            // n1 * ntiles
+    int ntiles;            // ceil(nrv / az_core_tile_columns())
+    int items;             // n1 * ntiles
     const ColConst* col;
     double prf, beta;
+    const float2* tw2;     // W_n2^m
     const float2* tw_full; // W_nbins^m
-    FftPlanDev sub;       // length-n2 plan
-    const int* gate;      // non-null: do nothing when *gate > kAzFusedReach
-    float window[16];     // Blackman window of the RCMC interpolator
+    const int* gate;       // non-null: do nothing when *gate > kAzCoreReach
+    float window[16];      // Blackman window of the RCMC interpolator
 };
 
 struct RawMomentsArgs {
@@ -152,17 +160,23 @@ struct RawMomentsArgs {
     int nr, na;
     int step;         // DC block width
     int nblocks;      // Doppler blocks (columns at or beyond nblocks*step only enter the statistics)
-    int vec;          // adjacent columns per lane (4 when nr % 4 == 0: one 8-byte load per row, else 1)
+    int vec;          // adjacent columns per lane: 8 (one 16-byte load per row, packed dot products), 4 (one
+                      // 8-byte load) or 1
     int ntiles;       // tiles of 256*vec columns
     int nslots;       // Doppler blocks a tile can touch: (256*vec - 1)/step + 2
     int nstrips;
     long long* partials; // [strip][tile][kRawStats + nslots*kRawLag]
+    long long* corr;     // vec 8: [nstrips][nblocks][kRawLag] lag sums of the odd block-start columns (written for
+                         // odd starts only)
+    LinesDesc lines;     // RAW_ERS_LINES: rows are shifted lines, NaN where no sample lands
 };
 
 struct RawReduceArgs {
     const long long* partials;
     int nstrips, ntiles, nslots, step, nblocks;
     int tile_cols;      // 256 * vec
+    const long long* corr;  // vec 8: straddle-column lag sums ([nstrips][nblocks][kRawLag]) to move between
+                            // neighbouring blocks (else null)
     long long* totals;  // [kRawStats + nblocks*kRawLag]
 };
 
@@ -254,15 +268,24 @@ hipError_t launch_row_fft(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_range_compress(const RcArgs& a, int raw_format, hipStream_t s);
 bool launch_range_compress_static(const RcArgs& a, int raw_format, hipStream_t s, hipError_t* err);
 int range_compress_static_variants(int n);
+// Padded-layout range compression (k_rc_pad.hip): variants per length and their launcher.
+int range_compress_pad_variants(int n);
+bool launch_range_compress_pad(const RcArgs& a, int fmt, int variant, hipStream_t s, hipError_t* err);
 hipError_t launch_col_fft(const ColFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_rc_mfma(const RcArgs& a, int raw_format, hipStream_t s);
 hipError_t launch_row_mfma(const RowFftArgs& a, bool inverse, hipStream_t s);
 hipError_t launch_col_mfma(const ColFftArgs& a, bool inverse, hipStream_t s);
 bool launch_col_static(const ColFftArgs& a, bool inverse, hipStream_t s, hipError_t* err);
+bool col_static_length(int n);
 hipError_t launch_rcmc_mf(const RcmcArgs& a, hipStream_t s);
 hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s);
-hipError_t launch_az_fused(const AzFusedArgs& a, hipStream_t s);
-size_t az_fused_lds_bytes(int n2);
+// Fused azimuth core (k_azcore.hip): LDS bytes of a workgroup, produced columns per tile, the second-pass
+// lengths it is compiled for, the once-per-plan LDS limit, the launch.
+size_t az_core_lds_bytes(int n2, int nbins);
+int az_core_tile_columns();
+bool az_core_supported(int n2);
+hipError_t az_core_prepare(int n2, int nbins);
+hipError_t launch_az_core(const AzCoreArgs& a, int n2, hipStream_t s);
 void rcmc_window(float w[16]);
 hipError_t launch_col_const(const ColConstArgs& a, hipStream_t s);
 hipError_t launch_moments_cf32(const float2* img, long stride, int xs, int ys, int xstr, double* partials,

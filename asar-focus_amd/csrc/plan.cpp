@@ -38,17 +38,18 @@ struct asarf_plan {
     asarf_focus_params p{};
     int nrp = 0, nap = 0, nrv = 0;
     long stride = 0;
-    int nblocks = 0, step = 0, nstrips = 0, raw_vec = 1;
+    int nblocks = 0, step = 0, nstrips = 0, raw_vec = 1;  // raw_vec: widest column count per lane the rows allow
     const FftPlanHost* rplan = nullptr;
     ColSplit cs;
     // device buffers
     float2* img = nullptr;
     float2* img2 = nullptr;  // second image: out-of-place target of the column passes and of the RCMC stage
-    bool fused = false;      // fused RCMC + matched filter + inverse pass B' (opt-in)
+    bool core = false;       // fused azimuth core: pass B + RCMC + matched filter + pass B' in one kernel
     float2* hconj = nullptr;
     ColConst* col = nullptr;
     long long* partials = nullptr;
     long long* totals = nullptr;
+    long long* corr = nullptr;  // straddle-column lag sums of the eight-column statistics pass
     DevResults* res = nullptr;
     float* ml = nullptr;
     float* rsl = nullptr;
@@ -63,6 +64,7 @@ struct asarf_plan {
     bool rcmc_row_only = false;
     const asarf_rcmc_net* net = nullptr;
     int fft_mfma = 0;  // split-bfloat16 MFMA transforms
+    LinesDesc lines{};  // ERS level-0 lines of the current execution (asarf_plan_execute_lines)
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -85,7 +87,8 @@ static int plan_alloc(asarf_plan* pl, void** p, size_t bytes) {
 }
 
 static void plan_free(asarf_plan* pl) {
-    void* bufs[] = {pl->img, pl->img2, pl->hconj, pl->col, pl->partials, pl->totals, pl->res, pl->ml, pl->rsl};
+    void* bufs[] = {pl->img,    pl->img2,   pl->hconj, pl->col, pl->partials,
+                    pl->totals, pl->corr,   pl->res,   pl->ml,  pl->rsl};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : pl->ev) (void)hipEventDestroy(e);
@@ -95,8 +98,15 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     if (!c || !p || !chirp_iq || !out) return set_error(ASARF_EINVAL, "asarf_plan_create: null argument");
     if (p->na < 2 || p->nr < 32 || p->nc <= 0 || p->nc >= p->nr)
         return set_error(ASARF_ESHAPE, "asarf_plan_create: na=%d nr=%d nc=%d", p->na, p->nr, p->nc);
-    if (p->raw_format != ASARF_RAW_U8 && p->raw_format != ASARF_RAW_I8)
-        return set_error(ASARF_EINVAL, "asarf_plan_create: raw_format %d (u8 or i8)", p->raw_format);
+    if (p->raw_format != ASARF_RAW_U8 && p->raw_format != ASARF_RAW_I8 && p->raw_format != ASARF_RAW_ERS_LINES)
+        return set_error(ASARF_EINVAL, "asarf_plan_create: raw_format %d (u8, i8 or ERS lines)", p->raw_format);
+    if (p->raw_format == ASARF_RAW_ERS_LINES &&
+        (p->samples_per_line <= 0 || p->samples_per_line > p->nr || p->line_bytes < 2L * p->samples_per_line ||
+         p->swst_mult <= 0))
+        return set_error(ASARF_EINVAL,
+                         "asarf_plan_create: ERS lines need 0 < samples_per_line (%d) <= nr (%d), line_bytes (%ld) >= "
+                         "2 samples_per_line and swst_mult (%d) > 0",
+                         p->samples_per_line, p->nr, p->line_bytes, p->swst_mult);
     if (!(p->prf > 0) || !(p->wavelength > 0) || !(p->range_spacing > 0) || !(p->slant_range_first > 0) ||
         !(p->az_bandwidth_fraction > 0))
         return set_error(ASARF_EINVAL, "asarf_plan_create: non-positive radar parameter");
@@ -133,7 +143,14 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     pl->stride = (nrv + kColTile - 1) / kColTile * kColTile;
     pl->step = step;
     pl->nblocks = nblocks;
-    pl->raw_vec = (p->nr % 4 == 0) ? 4 : 1;  // 4 when the raw rows allow 8-byte loads
+    // 8 columns per lane when the rows allow 16-byte loads and a lane meets at most one Doppler block boundary,
+    // 4 when they allow 8-byte loads, else 1 (the width of a call also depends on the raw buffer's alignment)
+    pl->raw_vec = (p->nr % 8 == 0 && step >= 8) ? 8 : (p->nr % 4 == 0) ? 4 : 1;
+    if (p->raw_format == ASARF_RAW_ERS_LINES)  // lanes of 4 samples valid or NaN together, 8-byte loads
+        pl->raw_vec = (p->nr % 4 == 0 && p->samples_per_line % 4 == 0 && p->swst_mult % 4 == 0 &&
+                       p->line_bytes % 8 == 0)
+                          ? 4
+                          : 1;
     pl->nstrips = (p->na + kStripRows - 1) / kStripRows;
     auto fail = [&](int st) {
         plan_free(pl);
@@ -148,35 +165,40 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     // ASARF_RCMC_ROW=1: always the row RCMC kernel (the column-tiled kernel is not launched); for comparisons
     const char* row_env = std::getenv("ASARF_RCMC_ROW");
     pl->rcmc_row_only = row_env && row_env[0] == '1';
-    // Fused azimuth core (RCMC + matched filter + inverse pass B' in one tile kernel), selected with
-    // ASARF_AZ_FUSED=1 when the split has a second pass and the tile fits LDS. It is exact against the separate
-    // kernels but measured slower on the 30000 x 5616 scene (2.41 ms against 1.95 ms for RCMC + pass B'), so the
-    // separate kernels are the default. The fused kernel holds n2 x 41 samples in LDS, so its split may favour
-    // a shorter second pass (ASARF_AZ_N2 = upper bound on n2, 0 = balanced split).
+    // Fused azimuth core (pass B + RCMC + matched filter + pass B' in one kernel, k_azcore.hip): the default when
+    // the azimuth length splits into n1 x n2 with a compiled second-pass length n2 (a short n2 keeps the
+    // workgroup's n2 x 89 spectrum tile within a workgroup's LDS). ASARF_AZ_FUSED=0 keeps the separate kernels (and
+    // the balanced split). Whether the core or the separate kernels run is decided on the device by the RCMC
+    // shift bound (the core's range halo covers shifts up to kAzCoreReach pixels).
     const char* fz_env = std::getenv("ASARF_AZ_FUSED");
-    const bool fz_on = (fz_env && fz_env[0] == '1') && pl->cs.n2 > 1;
-    if (fz_on) {
-        const char* n2_env = std::getenv("ASARF_AZ_N2");
-        const int n2_max = n2_env ? std::atoi(n2_env) : kAzFusedN2Max;
-        ColSplit alt;
-        if (n2_max > 0 && get_col_split(c, nap, &alt, n2_max) == ASARF_OK && alt.n2 > 1) pl->cs = alt;
+    if (!(fz_env && fz_env[0] == '0') && pl->cs.n2 > 1) {
+        for (int n2 : {84, 90, 98, 96}) {
+            ColSplit alt;
+            if (nap % n2 != 0 || get_col_split(c, nap, &alt, n2) != ASARF_OK || alt.n2 != n2) continue;
+            if (!az_core_supported(n2) || az_core_lds_bytes(n2, nap) > c->max_lds) continue;
+            if (az_core_prepare(n2, nap) != hipSuccess) continue;
+            pl->cs = alt;
+            pl->core = true;
+            break;
+        }
         clear_error();
     }
     if ((st = plan_alloc(pl, (void**)&pl->img, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
     // second image: the passes ping-pong between the two (A img -> img2, B img2 -> img, RCMC img -> img2,
     // B' img2 -> img, A' img -> img2), since out-of-place passes stream faster than in-place ones
-    pl->fused = fz_on && az_fused_lds_bytes(pl->cs.n2) <= c->max_lds;
     if ((st = plan_alloc(pl, (void**)&pl->img2, sizeof(float2) * (size_t)nap * pl->stride))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->hconj, sizeof(float2) * (size_t)nrp))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->col, sizeof(ColConst) * (size_t)nrv))) return fail(st);
     size_t partial_vals = 0;  // the larger of the two tilings (the width is chosen per call from the raw alignment)
-    for (int v : {1, pl->raw_vec}) {
+    for (int v : {1, pl->raw_vec >= 4 ? 4 : 1, pl->raw_vec}) {
         int nt = 0, ns = 0;
         raw_tiles(pl, v, &nt, &ns);
         partial_vals = std::max(partial_vals, (size_t)pl->nstrips * nt * (kRawStats + ns * kRawLag));
     }
     if ((st = plan_alloc(pl, (void**)&pl->partials, sizeof(long long) * partial_vals))) return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->totals, sizeof(long long) * (size_t)(kRawStats + nblocks * kRawLag))))
+        return fail(st);
+    if ((st = plan_alloc(pl, (void**)&pl->corr, sizeof(long long) * (size_t)pl->nstrips * nblocks * kRawLag)))
         return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->res, sizeof(DevResults)))) return fail(st);
 
@@ -270,7 +292,9 @@ static int mark(asarf_plan* pl, const char* name) {
 
 // Statistics, Doppler, column constants, range compression and the forward azimuth transform (both passes):
 // on return pl->img holds the range-Doppler image in permuted bin order.
-static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
+// Statistics, range compression and the forward azimuth passes. With core_path, pass B is gated on the RCMC shift
+// bound: it runs only where the fused azimuth core (which includes pass B) will not.
+static int enqueue_to_spectrum(asarf_plan* pl, const void* raw, bool core_path) {
     asarf_ctx* c = pl->ctx;
     const asarf_focus_params& p = pl->p;
     hipStream_t s = c->stream;
@@ -281,7 +305,10 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     rm.na = p.na;
     rm.step = pl->step;
     rm.nblocks = pl->nblocks;
-    rm.vec = (pl->raw_vec == 4 && (reinterpret_cast<uintptr_t>(raw) & 7) == 0) ? 4 : 1;
+    const uintptr_t ra_bits = reinterpret_cast<uintptr_t>(raw);
+    rm.vec = (pl->raw_vec == 8 && (ra_bits & 15) == 0) ? 8 : (pl->raw_vec >= 4 && (ra_bits & 7) == 0) ? 4 : 1;
+    rm.corr = pl->corr;
+    rm.lines = pl->lines;
     raw_tiles(pl, rm.vec, &rm.ntiles, &rm.nslots);
     rm.nstrips = pl->nstrips;
     rm.partials = pl->partials;
@@ -292,6 +319,7 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     rr.nstrips = pl->nstrips;
     rr.ntiles = rm.ntiles;
     rr.tile_cols = kThreads * rm.vec;
+    rr.corr = rm.vec == 8 ? pl->corr : nullptr;
     rr.nslots = rm.nslots;
     rr.step = pl->step;
     rr.nblocks = pl->nblocks;
@@ -343,6 +371,7 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     ra.hconj = pl->hconj;
     ra.plan = pl->rplan->dev();
     ra.mfma = pl->fft_mfma;
+    ra.lines = pl->lines;
     {
         const auto t = c->tuned_rows.find(pl->nrp);
         ra.no_static = (t != c->tuned_rows.end() && !t->second.use_static) ? 1 : 0;
@@ -363,7 +392,7 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
                                   pl->fft_mfma));
         ST_CHECK(mark(pl, "azimuth_fft_a"));
         ST_CHECK(col_forward_pass(c, pl->cs, 1, pl->img2, pl->stride, pl->nap, pl->img, pl->stride, pl->nrv,
-                                  pl->fft_mfma));
+                                  pl->fft_mfma, core_path ? &pl->res->rcmc_reach : nullptr));
         ST_CHECK(mark(pl, "azimuth_fft_b"));
     } else {
         ST_CHECK(col_forward_pass(c, pl->cs, 0, pl->img, pl->stride, p.na, pl->img, pl->stride, pl->nrv,
@@ -373,15 +402,39 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw) {
     return ASARF_OK;
 }
 
+static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out);
+
 extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out) {
     if (!pl || !raw || !slc) return set_error(ASARF_EINVAL, "asarf_plan_execute: null argument");
     if (pl->p.imp && !imp_out) return set_error(ASARF_EINVAL, "asarf_plan_execute: IMP output required");
+    if (pl->p.raw_format == ASARF_RAW_ERS_LINES)
+        return set_error(ASARF_EINVAL, "asarf_plan_execute: a plan of ERS lines runs with asarf_plan_execute_lines");
+    return plan_execute(pl, raw, slc, imp_out);
+}
+
+extern "C" int asarf_plan_execute_lines(asarf_plan* pl, const void* lines, const short* swst, int min_swst,
+                                        asarf_image* slc, float* imp_out) {
+    if (!pl || !lines || !swst || !slc) return set_error(ASARF_EINVAL, "asarf_plan_execute_lines: null argument");
+    if (pl->p.imp && !imp_out) return set_error(ASARF_EINVAL, "asarf_plan_execute_lines: IMP output required");
+    if (pl->p.raw_format != ASARF_RAW_ERS_LINES)
+        return set_error(ASARF_EINVAL, "asarf_plan_execute_lines: the plan was created for raw format %d",
+                         pl->p.raw_format);
+    pl->lines.swst = swst;
+    pl->lines.min_swst = min_swst;
+    pl->lines.mult = pl->p.swst_mult;
+    pl->lines.spl = pl->p.samples_per_line;
+    pl->lines.line_bytes = pl->p.line_bytes;
+    return plan_execute(pl, lines, slc, imp_out);
+}
+
+static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out) {
     asarf_ctx* c = pl->ctx;
     const asarf_focus_params& p = pl->p;
     hipStream_t s = c->stream;
     pl->ev_names.clear();
     ST_CHECK(mark(pl, "start"));
-    ST_CHECK(enqueue_to_spectrum(pl, raw));
+    const bool core_path = pl->core && pl->rcmc_interp == ASARF_RCMC_SINC16 && !pl->fft_mfma && !pl->rcmc_row_only;
+    ST_CHECK(enqueue_to_spectrum(pl, raw, core_path));
     float2* slc_buf = pl->img;  // where the SLC ends (img2 after an out-of-place last pass)
     RcmcArgs rc{};
     rc.img = pl->img;
@@ -406,43 +459,49 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
         rc.judge_nd = pl->net->judge_nd;
     }
     for (int i = 0; i < 8; ++i) rc.window8[i] = blackman_value(i, 8);
-    if (pl->fused && pl->rcmc_interp == 0 && !pl->fft_mfma) {
+    if (core_path) {
         const ColSplit& cs = pl->cs;
         const int n = cs.n;
-        // 4. fused RCMC + matched filter + inverse pass B' (img -> img2) when the RCMC shift bound computed on
-        // the device fits the tile halo; otherwise the same steps as separate kernels. Both variants are
-        // enqueued and the device-side bound makes exactly one of them do the work.
+        // 4. fused azimuth core: pass B + RCMC + matched filter + pass B' (img2 -> img) when the RCMC shift bound
+        // computed on the device fits the core's halo; otherwise pass B (enqueued above), RCMC + matched filter
+        // and pass B' as separate kernels. Both variants are enqueued and the device-side bound makes exactly
+        // one of them do the work.
         const int* gate = &pl->res->rcmc_reach;
-        AzFusedArgs fz{};
-        fz.in = pl->img;
-        fz.out = pl->img2;
-        fz.stride = pl->stride;
-        fz.nrv = pl->nrv;
-        fz.nbins = pl->nap;
-        fz.n1 = cs.n1;
-        fz.ntiles = (pl->nrv + kColTile - 1) / kColTile;
-        fz.items = cs.n1 * fz.ntiles;
-        fz.col = pl->col;
-        fz.prf = p.prf;
-        fz.beta = p.az_bandwidth_fraction;
-        fz.tw_full = cs.full->d_tw;
-        fz.sub = cs.p2->dev();
-        fz.gate = gate;
-        rcmc_window(fz.window);
-        HIP_CHECK(launch_az_fused(fz, s));
+        AzCoreArgs ac{};
+        ac.in = pl->img2;
+        ac.out = pl->img;
+        ac.stride = pl->stride;
+        ac.nrv = pl->nrv;
+        ac.nbins = pl->nap;
+        ac.n1 = cs.n1;
+        ac.ntiles = (pl->nrv + az_core_tile_columns() - 1) / az_core_tile_columns();
+        ac.items = cs.n1 * ac.ntiles;
+        ac.col = pl->col;
+        ac.prf = p.prf;
+        ac.beta = p.az_bandwidth_fraction;
+        ac.tw2 = cs.p2->d_tw;
+        ac.tw_full = cs.full->d_tw;
+        ac.gate = gate;
+        rcmc_window(ac.window);
+        HIP_CHECK(launch_az_core(ac, cs.n2, s));
         ST_CHECK(mark(pl, "azimuth_core_fused"));
+        // the separate kernels for shift bounds beyond the core's halo (img -> img2, then B' img2 -> img)
+        rc.out = pl->img2;
         rc.fused_gate = gate;
+        rc.tile_gate = gate;
+        HIP_CHECK(launch_rcmc_tile(rc, s));
         HIP_CHECK(launch_rcmc_mf(rc, s));
-        ColFftArgs ib = col_args(pl->ctx, pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n1, cs.n2, 1, n, cs.n2, 1, n,
-                                 cs.full->d_tw, 1.f, cs.p2);
+        ColFftArgs ib = col_args(pl->ctx, pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, cs.n1, cs.n2, 1, n,
+                                 cs.n2, 1, n, cs.full->d_tw, 1.f, cs.p2);
         ib.fused_gate = gate;
         HIP_CHECK(launch_col_fft(ib, true, s));
         ST_CHECK(mark(pl, "azimuth_core_wide"));
-        // 5. inverse pass A' (img2 -> img), 1/Nap, rows < Na
-        ColFftArgs ia = col_args(pl->ctx, pl->img2, pl->stride, pl->img, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1, cs.n2,
-                                 p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
+        // 5. inverse pass A' (img -> img2), 1/Nap, rows < Na: the SLC ends in img2
+        ColFftArgs ia = col_args(pl->ctx, pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1,
+                                 cs.n2, p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
         HIP_CHECK(launch_col_fft(ia, true, s));
         ST_CHECK(mark(pl, "azimuth_ifft_a"));
+        slc_buf = pl->img2;
     } else {
         // 4. RCMC (any interpolator) + azimuth matched filter (img -> img2): the column-tiled kernel when the
         // device-side shift bound fits its halo, else the row kernel; both are enqueued and the bound makes
@@ -555,7 +614,8 @@ extern "C" int asarf_plan_results(asarf_plan* pl, asarf_focus_results* r) {
     r->crop_far = pl->far;
     r->imp_rows = pl->p.imp ? d.imp_rows : 0;
     r->rcmc_shift_bound = d.rcmc_reach;
-    const bool fused_ran = pl->fused && pl->rcmc_interp == 0 && !pl->fft_mfma && d.rcmc_reach <= kAzFusedReach;
+    const bool fused_ran = pl->core && pl->rcmc_interp == ASARF_RCMC_SINC16 && !pl->fft_mfma &&
+                           !pl->rcmc_row_only && d.rcmc_reach <= kAzCoreReach;
     r->azimuth_core_fused = fused_ran ? 1 : 0;
     return clear_error();
 }
@@ -575,9 +635,11 @@ extern "C" int asarf_plan_set_rcmc(asarf_plan* pl, int interp, const asarf_rcmc_
 extern "C" int asarf_plan_rcmc_pairs(asarf_plan* pl, const void* raw, long n_pairs, unsigned long long seed,
                                      float* pairs) {
     if (!pl || !raw || !pairs || n_pairs <= 0) return set_error(ASARF_EINVAL, "asarf_plan_rcmc_pairs: argument");
+    if (pl->p.raw_format == ASARF_RAW_ERS_LINES)
+        return set_error(ASARF_EUNSUPPORTED, "asarf_plan_rcmc_pairs: plans of u8 or i8 raw samples only");
     if (pl->nrv < 32) return set_error(ASARF_ESHAPE, "asarf_plan_rcmc_pairs: too few range samples");
     pl->ev_names.clear();
-    ST_CHECK(enqueue_to_spectrum(pl, raw));
+    ST_CHECK(enqueue_to_spectrum(pl, raw, false));
     PairArgs a{};
     a.img = pl->img;
     a.stride = pl->stride;

@@ -6,7 +6,11 @@
 // Timing: one warm-up launch, then the minimum of `reps` launches bracketed by HIP events.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <set>
+#include <string>
 #include <vector>
 
 #include "asarf.h"
@@ -241,4 +245,160 @@ extern "C" int asarf_autotune_fft(asarf_ctx* c, int n, int layout, int batch, in
     if (n_candidates) *n_candidates = ncand;
     clear_error();
     return ASARF_OK;
+}
+
+// ------------------------------------------------------------------------------------------ autotuner cache file
+// Text, one choice per line after a header naming the format version and the device architecture:
+//   asarf-tune 1 <gcnArchName>
+//   row <n> static <variant>          compile-time range-compression variant of length n
+//   row <n> radix <r1> <r2> ...       runtime radix order of length n
+//   col <n> <n1>                      column split n = n1 x (n / n1) (n1 = n: single pass)
+// Lines starting with '#' and blank lines are skipped.
+
+namespace {
+
+int device_arch(asarf_ctx* c, std::string* out) {
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
+    *out = prop.gcnArchName;
+    if (out->empty()) *out = "unknown";
+    for (char& ch : *out)
+        if (ch == ' ' || ch == '\t' || ch == '\n') ch = '_';
+    return ASARF_OK;
+}
+
+}  // namespace
+
+extern "C" int asarf_tune_save(asarf_ctx* c, const char* path) {
+    if (!c || !path) return set_error(ASARF_EINVAL, "asarf_tune_save: null argument");
+    int st = ctx_init(c);
+    if (st != ASARF_OK) return st;
+    std::string arch;
+    if ((st = device_arch(c, &arch)) != ASARF_OK) return st;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return set_error(ASARF_EINVAL, "asarf_tune_save: cannot open %s for writing", path);
+    bool ok = std::fprintf(f, "asarf-tune 1 %s\n", arch.c_str()) > 0;
+    for (const auto& kv : c->tuned_rows) {
+        const auto& rt = kv.second;
+        if (rt.use_static || !rt.plan) {
+            ok = ok && std::fprintf(f, "row %d static %d\n", kv.first, rt.static_variant) > 0;
+        } else {
+            ok = ok && std::fprintf(f, "row %d radix", kv.first) > 0;
+            for (int r : rt.plan->radix) ok = ok && std::fprintf(f, " %d", r) > 0;
+            ok = ok && std::fprintf(f, "\n") > 0;
+        }
+    }
+    for (const auto& kv : c->tuned_col_n1) ok = ok && std::fprintf(f, "col %d %d\n", kv.first, kv.second) > 0;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return set_error(ASARF_EINVAL, "asarf_tune_save: write to %s failed", path);
+    return clear_error();
+}
+
+extern "C" int asarf_tune_load(asarf_ctx* c, const char* path, int* n_loaded) {
+    if (!c || !path) return set_error(ASARF_EINVAL, "asarf_tune_load: null argument");
+    if (n_loaded) *n_loaded = 0;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return set_error(ASARF_EINVAL, "asarf_tune_load: cannot open %s", path);
+    // parse everything first (no device call): a malformed file changes nothing in the context
+    struct RowEntry {
+        int n;
+        bool is_static;
+        int variant;
+        std::vector<int> radix;
+    };
+    std::vector<RowEntry> rows;
+    std::vector<std::pair<int, int>> cols;
+    char line[1024];
+    int lineno = 0;
+    bool header = false;
+    std::string file_arch;
+    auto bad = [&](const char* what) {
+        std::fclose(f);
+        return set_error(ASARF_EINVAL, "asarf_tune_load: %s:%d: %s", path, lineno, what);
+    };
+    while (std::fgets(line, sizeof line, f)) {
+        ++lineno;
+        if (!std::strchr(line, '\n') && !std::feof(f)) return bad("line too long");
+        char* save = nullptr;
+        char* tok = strtok_r(line, " \t\r\n", &save);
+        if (!tok || tok[0] == '#') continue;
+        auto next_int = [&](int* v) {
+            char* t = strtok_r(nullptr, " \t\r\n", &save);
+            if (!t) return false;
+            char* end = nullptr;
+            const long x = std::strtol(t, &end, 10);
+            if (*end != '\0' || x < 0 || x > (1L << 26)) return false;
+            *v = (int)x;
+            return true;
+        };
+        if (!header) {
+            int version = 0;
+            if (std::strcmp(tok, "asarf-tune") != 0 || !next_int(&version) || version != 1)
+                return bad("not an asarf-tune version 1 file");
+            const char* a = strtok_r(nullptr, " \t\r\n", &save);
+            if (!a) return bad("header: no device architecture");
+            file_arch = a;
+            header = true;
+            continue;
+        }
+        if (std::strcmp(tok, "row") == 0) {
+            RowEntry e{};
+            if (!next_int(&e.n) || e.n < 2) return bad("row: length");
+            const char* kind = strtok_r(nullptr, " \t\r\n", &save);
+            if (kind && std::strcmp(kind, "static") == 0) {
+                e.is_static = true;
+                if (!next_int(&e.variant) || e.variant >= std::max(1, range_compress_static_variants(e.n)))
+                    return bad("row: compile-time variant out of range");
+            } else if (kind && std::strcmp(kind, "radix") == 0) {
+                int r = 0;
+                long prod = 1;
+                while (next_int(&r)) {
+                    if (r < 2 || (int)e.radix.size() >= kMaxPasses) return bad("row: radix");
+                    e.radix.push_back(r);
+                    prod *= r;
+                }
+                if (prod != e.n || !radix_plan_fits(e.n, kRowThreads, e.radix))
+                    return bad("row: radix order does not factor the length on the row kernel");
+            } else {
+                return bad("row: expected 'static' or 'radix'");
+            }
+            rows.push_back(std::move(e));
+        } else if (std::strcmp(tok, "col") == 0) {
+            int n = 0, n1 = 0;
+            if (!next_int(&n) || !next_int(&n1) || n < 2 || n1 < 2 || n1 > n || n % n1 != 0)
+                return bad("col: length and first-pass length");
+            cols.emplace_back(n, n1);
+        } else {
+            return bad("unknown keyword");
+        }
+    }
+    const bool read_err = std::ferror(f) != 0;
+    std::fclose(f);
+    if (read_err) return set_error(ASARF_EINVAL, "asarf_tune_load: read error on %s", path);
+    if (!header) return set_error(ASARF_EINVAL, "asarf_tune_load: %s is empty", path);
+    int st = ctx_init(c);
+    if (st != ASARF_OK) return st;
+    std::string arch;
+    if ((st = device_arch(c, &arch)) != ASARF_OK) return st;
+    if (arch != file_arch)
+        return set_error(ASARF_EINVAL, "asarf_tune_load: %s was tuned on %s, this device is %s", path,
+                         file_arch.c_str(), arch.c_str());
+    for (const auto& e : rows) {
+        asarf_ctx::RowTune rt;
+        rt.use_static = e.is_static;
+        rt.static_variant = e.is_static ? e.variant : 0;
+        if (!e.is_static) {
+            auto p = std::make_unique<FftPlanHost>();
+            p->n = e.n;
+            p->radix = e.radix;
+            p->d_tw = get_twiddles(c, e.n, &st);
+            if (!p->d_tw) return st;
+            rt.plan = p.get();
+            c->plans[std::make_pair(e.n, -1000 - c->tune_key++)] = std::move(p);
+        }
+        c->tuned_rows[e.n] = rt;
+    }
+    for (const auto& kv : cols) c->tuned_col_n1[kv.first] = kv.second;
+    if (n_loaded) *n_loaded = (int)(rows.size() + cols.size());
+    return clear_error();
 }

@@ -38,7 +38,11 @@ enum {
 #define ASARF_QUIRKS_REFERENCE (ASARF_Q1_IQ_PADDED_WIDTH | ASARF_Q2_COS_NO_SQRT | ASARF_Q4_DC_EXTRA_BLOCK)
 
 enum { ASARF_CF32 = 0, ASARF_F32 = 1 };                     /* image element types */
-enum { ASARF_RAW_CF32 = 0, ASARF_RAW_U8 = 1, ASARF_RAW_I8 = 2 }; /* raw sample formats (I,Q interleaved) */
+/* raw sample formats (I,Q interleaved). ASARF_RAW_ERS_LINES: ERS IM level-0 lines as the parser leaves them
+ * (asarf_parse_ers_im_packet): samples_per_line u8 (I, Q) pairs per line, line_bytes apart, each line placed
+ * swst_mult x (SWST code - minimum) range samples to the right, as ConvertErsImSamplesToComplex does
+ * (envisat_format_kernels.cu:116-133); range samples no line sample lands on are NaN. */
+enum { ASARF_RAW_CF32 = 0, ASARF_RAW_U8 = 1, ASARF_RAW_I8 = 2, ASARF_RAW_ERS_LINES = 3 };
 enum { ASARF_FORWARD = 0, ASARF_INVERSE = 1 };
 enum { ASARF_FFT_ROWS = 0, ASARF_FFT_COLUMNS = 1 };
 
@@ -316,7 +320,7 @@ int asarf_fft_c2c(asarf_ctx* ctx, void* data, int n, int batch, long dist, int l
 typedef struct {
     int na, nr, nc;          /* azimuth lines, range samples, chirp samples */
     int nrp, nap;            /* FFT sizes, 0 -> reference padding rule */
-    int raw_format;          /* ASARF_RAW_U8 or ASARF_RAW_I8 */
+    int raw_format;          /* ASARF_RAW_U8, ASARF_RAW_I8 or ASARF_RAW_ERS_LINES */
     double prf, wavelength, range_spacing, slant_range_first, az_bandwidth_fraction;
     double vr_poly[3];
     size_t n_total_samples;  /* 0 -> na*nr */
@@ -328,6 +332,11 @@ typedef struct {
     int gr_range_size;       /* ground-range output samples */
     double ground_spacing;   /* 12.5 m */
     int lines_before, lines_after; /* extra lines around the requested sensing window (main.cc:137-142) */
+    /* ASARF_RAW_ERS_LINES only: I/Q pairs per line (nr = samples_per_line + swst_mult x (max - min SWST code)),
+     * bytes from one line to the next, range samples per SWST code step (4 for ERS IM) */
+    int samples_per_line;
+    long line_bytes;
+    int swst_mult;
 } asarf_focus_params;
 
 typedef struct {
@@ -356,6 +365,11 @@ size_t asarf_plan_device_bytes(const asarf_plan* plan);
  * (x_size = nrv, y_size = na); imp_out (device, may be NULL when params.imp == 0) receives up to
  * asarf_plan_imp_capacity rows x gr_range_size floats. */
 int asarf_plan_execute(asarf_plan* plan, const void* raw, asarf_image* slc, float* imp_out);
+/* asarf_plan_execute for a plan of ASARF_RAW_ERS_LINES: lines (device) as described there, swst (device, na int16
+ * SWST codes, one per line) and their minimum; the NaN fill and the padded copy of main_flow.cc:530-583 and
+ * main.cc:233-239 happen in the statistics and range-compression loads, with no intermediate image. */
+int asarf_plan_execute_lines(asarf_plan* plan, const void* lines, const short* swst, int min_swst, asarf_image* slc,
+                             float* imp_out);
 int asarf_plan_imp_capacity(const asarf_plan* plan, int* rows, int* cols);
 /* Copies the estimator outputs to the host (synchronises). */
 int asarf_plan_results(asarf_plan* plan, asarf_focus_results* results);
@@ -382,6 +396,13 @@ int asarf_plan_set_fft(asarf_plan* plan, int precision);
  * the chosen split (n = single pass). *best_ms = its time, *n_candidates = candidates measured. Synchronises. */
 int asarf_autotune_fft(asarf_ctx* ctx, int n, int layout, int batch, int reps, int* choice, float* best_ms,
                        int* n_candidates);
+/* Autotuner cache file (no counterpart in the reference, whose transforms are cuFFT plans): keeps the tuner's
+ * choices across processes. Save writes the context's choices (text: a header with the device architecture, then
+ * one line per row or column length); load reads such a file into a context on a device of the same architecture
+ * and returns the number of choices in *n_loaded. A malformed file, or one tuned on another architecture, is
+ * ASARF_EINVAL and changes nothing (a malformed file is rejected before any device call). */
+int asarf_tune_save(asarf_ctx* ctx, const char* path);
+int asarf_tune_load(asarf_ctx* ctx, const char* path, int* n_loaded);
 
 /* Many blocks of uneven size in one batch: each block gets its own plan on one of n_streams streams (round
  * robin; each stream has its own context that inherits ctx's autotuner choices). SEQUENTIAL runs the blocks

@@ -123,6 +123,37 @@ def test_validation_without_device():
         lib.asarf_ctx_destroy(ctx)
 
 
+@pytest.mark.parametrize("text, what", [
+    ("", "empty"),
+    ("# only a comment\n", "empty"),
+    ("asarf-tune 2 gfx950\n", "version 1"),
+    ("asarf-tune 1\n", "architecture"),
+    ("asarf-tune 1 gfx950\ncol 30240 11\n", "col:"),           # 11 does not divide 30240
+    ("asarf-tune 1 gfx950\ncol 30240\n", "col:"),
+    ("asarf-tune 1 gfx950\nrow 6144 static 99\n", "variant"),
+    ("asarf-tune 1 gfx950\nrow 6144 radix 8 8 8 8\n", "radix order"),   # product 4096
+    ("asarf-tune 1 gfx950\nrow 6144 radix 8 8 x\n", "radix order"),
+    ("asarf-tune 1 gfx950\nrow 6144 fast\n", "'static' or 'radix'"),
+    ("asarf-tune 1 gfx950\nrows 6144 static 0\n", "unknown keyword"),
+    ("asarf-tune 1 gfx950\ncol -5 1\n", "col:"),
+])
+def test_tuner_cache_rejects_malformed_files_without_device(tmp_path, text, what):
+    """A malformed autotuner cache file is refused while parsing, before any device call."""
+    lib = L.lib()
+    ctx = ctypes.c_void_p()
+    L.check(lib.asarf_ctx_create(0, None, ctypes.byref(ctx)))
+    try:
+        path = tmp_path / "tune.txt"
+        path.write_text(text)
+        n = ctypes.c_int(-1)
+        assert lib.asarf_tune_load(ctx, str(path).encode(), ctypes.byref(n)) == L.EINVAL
+        assert what in L.last_error()
+        assert n.value == 0
+        assert lib.asarf_tune_load(ctx, str(tmp_path / "missing.txt").encode(), ctypes.byref(n)) == L.EINVAL
+    finally:
+        lib.asarf_ctx_destroy(ctx)
+
+
 def test_memory_model_trading_without_device():
     """ReleaseMemory / InitExtPtr semantics (device_padded_image.cuh:49-61,104-111) are pure bookkeeping."""
     lib = L.lib()

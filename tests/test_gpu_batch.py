@@ -41,6 +41,43 @@ def test_autotuned_schedules_keep_parity():
     ctx.close()
 
 
+def test_tuner_cache_file_round_trip(tmp_path):
+    """Choices saved from one context load into another and drive its plans; the file names the device
+    architecture and every choice of the first context."""
+    ctx = focus.Context(0)
+    cfg = configs.asar_scene().scaled(na=700, nr=1400, n_targets_random=4)
+    nrp = focus.padding_size(cfg.nr + cfg.nc)
+    nap = focus.padding_size(cfg.na)
+    rows = focus.autotune(ctx, nrp, columns=False, batch=256)
+    cols = focus.autotune(ctx, nap, columns=True, batch=512)
+    path = tmp_path / "tune.txt"
+    focus.tune_save(ctx, str(path))
+    text = path.read_text().splitlines()
+    assert text[0].startswith("asarf-tune 1 gfx950")
+    assert f"col {nap} {cols['choice']}" in text
+    assert any(t.startswith(f"row {nrp} ") for t in text)
+    ctx.close()
+    ctx2 = focus.Context(0)
+    assert focus.tune_load(ctx2, str(path)) == 2
+    # saving the loaded context reproduces the file
+    path2 = tmp_path / "tune2.txt"
+    focus.tune_save(ctx2, str(path2))
+    assert path2.read_text() == path.read_text()
+    raw, _ = scene.generate(cfg)
+    plan = focus.Plan(ctx2, cfg)
+    plan.execute(torch.from_numpy(raw).cuda())
+    slc = plan.slc_tensor().cpu().numpy()
+    plan.close()
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    assert rel_rms(slc, ref) < 1e-4
+    # another architecture's file is refused and changes nothing
+    other = tmp_path / "other.txt"
+    other.write_text("asarf-tune 1 gfx942\ncol 30240 168\n")
+    with pytest.raises(RuntimeError, match="tuned on gfx942"):
+        focus.tune_load(ctx2, str(other))
+    ctx2.close()
+
+
 def test_batch_graph_equals_sequential_and_replays(ctx):
     cfgs = configs.batch_blocks(6, na_range=(300, 900))
     cfgs = [c.scaled(nr=min(c.nr, 1500)) for c in cfgs]
@@ -66,7 +103,7 @@ def test_batch_graph_equals_sequential_and_replays(ctx):
     b.close()
 
 
-@pytest.mark.parametrize("which,nvar", [("asar", 8), ("ers", 6), ("ers8192", 6)])
+@pytest.mark.parametrize("which,nvar", [("asar", 10), ("ers", 6), ("ers8192", 7)])
 def test_range_compression_variants_match_oracle(ctx, which, nvar):
     """Every compile-time range-compression variant the autotuner may pick gives the oracle's SLC."""
     import os

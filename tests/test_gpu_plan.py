@@ -102,18 +102,23 @@ def run_plan_env(ctx, cfg, env):
 
 @pytest.mark.parametrize("which", ["ers", "asar"])
 def test_plan_fused_azimuth_core_matches_separate_kernels(ctx, which):
-    """The fused pass-B/RCMC/matched-filter/pass-B' kernel against the separate kernels on the same raw data."""
+    """The fused pass-B/RCMC/matched-filter/pass-B' kernel (the default where the azimuth length splits with a
+    compiled second pass: 720 = 8 x 90, 1512 = 18 x 84) against the separate kernels (ASARF_AZ_FUSED=0, balanced
+    split) on the same raw data, and both against the oracle."""
     if which == "ers":
         cfg = configs.ers_block().scaled(na=700, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6))
     else:
-        cfg = configs.asar_scene().scaled(na=640, nr=1400, n_targets_random=6)
-    _, slc_f, res_f, _ = run_plan_env(ctx, cfg, {"ASARF_AZ_FUSED": "1"})
+        cfg = configs.asar_scene().scaled(na=1500, nr=1400, n_targets_random=6)
+    raw, slc_f, res_f, _ = run_plan(ctx, cfg)
     _, slc_s, res_s, _ = run_plan_env(ctx, cfg, {"ASARF_AZ_FUSED": "0"})
     assert res_f["azimuth_core_fused"] and not res_s["azimuth_core_fused"]
     assert 0 < res_f["rcmc_shift_bound"] <= 8
     err = rel_rms(slc_f, slc_s)
-    print(f"{which} fused vs separate rel rms {err:.3e}")
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    err_f, err_s = rel_rms(slc_f, ref), rel_rms(slc_s, ref)
+    print(f"{which} fused vs separate rel rms {err:.3e}; vs oracle fused {err_f:.3e} separate {err_s:.3e}")
     assert err < 1e-6
+    assert err_f < SLC_TOL and err_s < SLC_TOL
 
 
 def test_plan_large_migration_takes_separate_path(ctx):
@@ -121,7 +126,7 @@ def test_plan_large_migration_takes_separate_path(ctx):
     separate kernels, and the SLC still matches the oracle."""
     cfg = configs.ers_block().scaled(na=700, nr=1500, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6),
                                      wavelength=0.236)
-    raw, slc, res, _ = run_plan_env(ctx, cfg, {"ASARF_AZ_FUSED": "1"})
+    raw, slc, res, _ = run_plan(ctx, cfg)
     assert not res["azimuth_core_fused"] and res["rcmc_shift_bound"] > 8
     ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
     err = rel_rms(slc, ref)
@@ -143,10 +148,12 @@ def test_plan_tiled_and_row_rcmc_kernels(ctx, wavelength):
     assert err < SLC_TOL
 
 
-@pytest.mark.parametrize("nr,offset", [(1400, 0), (1400, 2), (1402, 0)])
+@pytest.mark.parametrize("nr,offset", [(1400, 0), (1400, 8), (1400, 2), (1402, 0), (1404, 0)])
 def test_plan_statistics_load_width(ctx, nr, offset):
-    """The raw statistics pass reads 4 samples per lane with 8-byte loads when nr % 4 == 0 and the raw buffer is
-    8-byte aligned, one sample otherwise: both give the oracle's I/Q statistics, Doppler centroid and SLC."""
+    """The raw statistics pass reads 8 samples per lane with 16-byte loads and packed dot products when
+    nr % 8 == 0 and the raw buffer is 16-byte aligned (nr = 1400: Doppler blocks of 43 columns, so every other
+    block starts inside a 2-sample word), 4 samples with 8-byte loads when only nr % 4 == 0 or 8-byte alignment
+    holds, one sample otherwise: each gives the oracle's I/Q statistics, Doppler centroid and SLC."""
     cfg = configs.asar_scene().scaled(na=640, nr=nr, n_targets_random=4)
     raw, _ = scene.generate(cfg)
     buf = torch.empty(raw.size + 16, dtype=torch.uint8, device="cuda")
