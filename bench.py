@@ -78,6 +78,117 @@ def cpu_baseline(cfg, raw, min_seconds=10.0, max_reps=3):
     }
 
 
+def _rel_rms(a, b):
+    import numpy as np
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def _quality(cfg, slc, ref):
+    """SLC relative RMS error and worst point-target PSLR/ISLR (dB) and IRW (relative) deltas against the FP64
+    oracle over every target of the scene, range and azimuth cuts."""
+    from asarf import analysis, scene
+    tx, ty = scene.targets(cfg)
+    worst = {"pslr_db": 0.0, "islr_db": 0.0, "irw_rel": 0.0}
+    n = 0
+    for x, y in zip(tx, ty):
+        try:
+            mr = analysis.point_target(ref, x, y)
+        except ValueError:
+            continue
+        mg = analysis.point_target(slc, x, y)
+        n += 1
+        for ax in ("range", "azimuth"):
+            worst["pslr_db"] = max(worst["pslr_db"], abs(mg[ax]["pslr"] - mr[ax]["pslr"]))
+            worst["islr_db"] = max(worst["islr_db"], abs(mg[ax]["islr"] - mr[ax]["islr"]))
+            worst["irw_rel"] = max(worst["irw_rel"], abs(mg[ax]["irw"] / mr[ax]["irw"] - 1.0))
+    return {"slc_rel_rms": _rel_rms(slc, ref), "targets": n, **{k: round(v, 5) for k, v in worst.items()}}
+
+
+def _time_plan(plan, traw, steps, warmup=2):
+    import torch
+    for _ in range(warmup):
+        plan.execute(traw)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        plan.execute(traw)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def forms_probe(steps=10, train_pairs=10_000_000, train_steps=2400, threshold=1e-3):
+    """The two alternative forms against the plain FP32 path, each on its own workload, with quality measured
+    against the FP64 oracle on the same scene (outside the headline value):
+    * split-bf16 matrix-core transforms on the ERS block at range 8192 / azimuth 32768;
+    * the learned RCMC interpolator (trained on the plan's own 16-tap pairs, judged per cell at `threshold`,
+      16-tap sinc elsewhere) on the high-migration scene.
+    speedup = plain ms / form ms; matched = every target within 0.1 dB PSLR/ISLR of the oracle."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    from asarf import configs, focus, scene
+
+    def entry(cfg, plain_ms, plain_q, form_ms, form_q, **extra):
+        matched = form_q["pslr_db"] <= 0.1 and form_q["islr_db"] <= 0.1
+        return {"workload": cfg.name, "plain_fp32_ms": round(plain_ms, 4), "form_ms": round(form_ms, 4),
+                "speedup": round(plain_ms / form_ms, 3), "matched_quality": matched,
+                "plain_vs_oracle": plain_q, "form_vs_oracle": form_q, **extra}
+
+    out = {}
+    ctx = focus.Context(0)
+    # split-precision transforms
+    cfg = configs.ers_block_split()
+    raw, _ = scene.generate(cfg)
+    traw = torch.from_numpy(raw).cuda()
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg, nrp=cfg.nrp, nap=cfg.nap)
+    plan = focus.Plan(ctx, cfg)
+    ms_fp32 = _time_plan(plan, traw, steps)
+    q_fp32 = _quality(cfg, plan.slc_tensor().cpu().numpy(), ref)
+    plan.set_fft(True)
+    ms_bf16 = _time_plan(plan, traw, steps)
+    bf = plan.slc_tensor().cpu().numpy()
+    q_bf16 = _quality(cfg, bf, ref)
+    mask = np.abs(ref) > 0.1 * np.abs(ref).max()
+    phase = float(np.sqrt(np.mean(np.angle(bf[mask] * np.conj(ref[mask])) ** 2)))
+    plan.close()
+    out["split_bf16"] = entry(cfg, ms_fp32, q_fp32, ms_bf16, q_bf16, fft_sizes=[cfg.nrp, cfg.nap],
+                              phase_rms_rad=phase)
+    del raw, traw, ref, bf
+    torch.cuda.empty_cache()
+    # learned RCMC
+    cfg = configs.high_migration()
+    raw, _ = scene.generate(cfg)
+    traw = torch.from_numpy(raw).cuda()
+    ref, _ = po.focus_slc(scene.raw_to_complex(raw), cfg)
+    plan = focus.Plan(ctx, cfg)
+    ms_sinc = _time_plan(plan, traw, steps)
+    q_sinc = _quality(cfg, plan.slc_tensor().cpu().numpy(), ref)
+    t0 = time.perf_counter()
+    pairs = plan.rcmc_pairs(traw, train_pairs, seed=0x7A1)
+    val = plan.rcmc_pairs(traw, 1 << 18, seed=0x7A2)
+    net = focus.RcmcNet(ctx, seed=3)
+    net.train(pairs, batch=8192, steps=train_steps, lr=2e-3)
+    table = net.build_table(val, nd=8192, ns=2, judge_nd=256, threshold=threshold)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del pairs, val
+    plan.set_rcmc(focus.RCMC_LEARNED, net)
+    ms_learned = _time_plan(plan, traw, steps)
+    q_learned = _quality(cfg, plan.slc_tensor().cpu().numpy(), ref)
+    plan.close()
+    net.close()
+    out["learned_rcmc"] = entry(cfg, ms_sinc, q_sinc, ms_learned, q_learned, judge_threshold=threshold,
+                                judge_cells_accepted=int(table["n_ok"]), judge_cells=int(table["rms"].size),
+                                train_pairs=train_pairs, train_s=round(train_s, 2))
+    ctx.close()
+    return out
+
+
 def ingest_probe(reps=20):
     """Level-0 decoders at full scene size (synthetic bytes): ERS IM 30000 lines of 5616 u8 I/Q pairs and ASAR IM
     30000 FBAQ-4 records of 5616 samples, each into a 6400 x 30240 padded cf32 image (NaN fill and zero padding
@@ -280,6 +391,9 @@ def main():
     ap.add_argument("--stages-probe", action="store_true",
                     help="time the reference's stage sequence (I/Q correction, Doppler centroid, range compression, "
                          "range-Doppler) through the stage entry points at the ASAR scene size")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the forms object (split-bf16 and learned RCMC against the plain path on their own "
+                         "workloads, about half a minute)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (5 times): the known-size reference that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of the plan's kernels")
@@ -397,6 +511,13 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get(cfg.name, {}).get(dom)
+    # VALU issue occupancy per stage from the committed SQ counter pass (SQ_ACTIVE_INST_VALU against the SIMD
+    # cycles of the dispatch): near 1 means the kernel is bound by its instruction count rather than by HBM
+    valu_busy = {}
+    ifile = os.path.join(ROOT, "profiles", "issue.json")
+    if os.path.exists(ifile):
+        with open(ifile) as f:
+            valu_busy = {k: v["valu_busy"] for k, v in json.load(f).get(cfg.name, {}).items()}
 
     # measured device-to-device copy rate over a buffer the size of the plan's image: the second denominator of
     # the memory-bound stages (what a plain read + write of the same bytes achieves on this device)
@@ -436,6 +557,8 @@ def main():
             "imp": imp,
             "form": ("split-bf16 MFMA DFT passes, " if args.fft == "split_bf16" else "plain FP32 Stockham, ") + {"sinc16": "16-tap sinc RCMC", "sinc8": "8-tap sinc RCMC",
                                                  "learned": "learned 8-tap RCMC (16-tap sinc where judged)"}[args.rcmc],
+            "unit_of_work": "one raw input sample (azimuth line x range sample) taken from the device-resident raw "
+                            "buffer to the focused SLC and IMP; slc_output_msamples_per_s counts output samples",
         },
         "rcmc": rcmc_info,
         "autotune": tune,
@@ -451,6 +574,8 @@ def main():
             "ms": round(stages[dom], 4),
             "copy_gbs_measured": round(copy_gbs, 1),
             "frac_of_copy": round(achieved / copy_gbs, 4),
+            "valu_busy": valu_busy.get(dom),
+            "valu_busy_stages": valu_busy or None,
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "stages_gbs": {k: round(v, 1) for k, v in gbs.items()},
@@ -467,6 +592,10 @@ def main():
     if net is not None:
         net.close()
     ctx.close()
+    if not args.no_forms and args.fft == "fp32" and args.rcmc == "sinc16":
+        del traw
+        torch.cuda.empty_cache()
+        line["forms"] = forms_probe()
     print(json.dumps(line))
 
 
