@@ -427,13 +427,26 @@ extern "C" int asarf_plan_execute_lines(asarf_plan* pl, const void* lines, const
     return plan_execute(pl, lines, slc, imp_out);
 }
 
+// The interpolator a plan execution applies: a learned table whose judge accepted no cell is the 16-tap sinc
+// everywhere, so it runs as such (and may take the fused azimuth core).
+static int effective_interp(const asarf_plan* pl) {
+    if (pl->rcmc_interp == ASARF_RCMC_LEARNED && pl->net && pl->net->n_ok == 0) return ASARF_RCMC_SINC16;
+    return pl->rcmc_interp;
+}
+
+// Whether an execution takes the fused azimuth core path (the device-side shift bound still decides whether the
+// core or the gated separate kernels do the work).
+static bool core_path_of(const asarf_plan* pl) {
+    return pl->core && effective_interp(pl) == ASARF_RCMC_SINC16 && !pl->fft_mfma && !pl->rcmc_row_only;
+}
+
 static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out) {
     asarf_ctx* c = pl->ctx;
     const asarf_focus_params& p = pl->p;
     hipStream_t s = c->stream;
     pl->ev_names.clear();
     ST_CHECK(mark(pl, "start"));
-    const bool core_path = pl->core && pl->rcmc_interp == ASARF_RCMC_SINC16 && !pl->fft_mfma && !pl->rcmc_row_only;
+    const bool core_path = core_path_of(pl);
     ST_CHECK(enqueue_to_spectrum(pl, raw, core_path));
     float2* slc_buf = pl->img;  // where the SLC ends (img2 after an out-of-place last pass)
     RcmcArgs rc{};
@@ -447,10 +460,8 @@ static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float
     rc.prf = p.prf;
     rc.beta = p.az_bandwidth_fraction;
     rc.mode = 0;
-    rc.interp = pl->rcmc_interp;
-    // a table whose judge accepted no cell falls back to the 16-tap sinc everywhere: run that directly and skip
-    // the per-pixel cell lookups
-    if (rc.interp == ASARF_RCMC_LEARNED && pl->net && pl->net->n_ok == 0) rc.interp = ASARF_RCMC_SINC16;
+    // a table whose judge accepted no cell runs as the 16-tap sinc directly, without the per-pixel cell lookups
+    rc.interp = effective_interp(pl);
     if (pl->net) {
         rc.lut = pl->net->lut;
         rc.lut_ok = pl->net->lut_ok;
@@ -614,8 +625,7 @@ extern "C" int asarf_plan_results(asarf_plan* pl, asarf_focus_results* r) {
     r->crop_far = pl->far;
     r->imp_rows = pl->p.imp ? d.imp_rows : 0;
     r->rcmc_shift_bound = d.rcmc_reach;
-    const bool fused_ran = pl->core && pl->rcmc_interp == ASARF_RCMC_SINC16 && !pl->fft_mfma &&
-                           !pl->rcmc_row_only && d.rcmc_reach <= kAzCoreReach;
+    const bool fused_ran = core_path_of(pl) && d.rcmc_reach <= kAzCoreReach;
     r->azimuth_core_fused = fused_ran ? 1 : 0;
     return clear_error();
 }
