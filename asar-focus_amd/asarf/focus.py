@@ -310,12 +310,14 @@ def subset_copy(ctx: Context, img: L.Image, x0: int, y0: int, w: int, h: int, ds
 
 def focus_params(cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
                  orbit_height: float | None = None, lines_before: int = 0, lines_after: int = 0,
-                 srgr: tuple | None = None, azimuth_spacing: float | None = None, lines: tuple | None = None):
+                 srgr: tuple | None = None, azimuth_spacing: float | None = None, lines: tuple | None = None,
+                 fbaq: tuple | None = None):
     """asarf_focus_params of a scene configuration (and the SRGR fit when imp). lines_before / lines_after are the
     lines cached before the sensing start and after the sensing stop (main.cc:296-298). srgr = (grsr5, ground
     range size) replaces the spherical-earth fit (e.g. asarf_srgr_fit from an orbit); azimuth_spacing replaces
     vr / prf. lines = (samples_per_line, line_bytes, swst_mult) makes the input ERS level-0 lines
-    (ASARF_RAW_ERS_LINES; cfg.nr is then the range size after the SWST shifts)."""
+    (ASARF_RAW_ERS_LINES), fbaq = (record_bytes, swst_mult) ASAR FBAQ-4 records (ASARF_RAW_ASAR_FBAQ4); cfg.nr is
+    then the range size after the SWST shifts."""
     p = L.FocusParams()
     p.na, p.nr, p.nc = cfg.na, cfg.nr, cfg.nc
     p.nrp = cfg.nrp or padding_size(cfg.nr + cfg.nc)
@@ -324,6 +326,9 @@ def focus_params(cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool =
     if lines is not None:
         p.raw_format = L.RAW_ERS_LINES
         p.samples_per_line, p.line_bytes, p.swst_mult = int(lines[0]), int(lines[1]), int(lines[2])
+    if fbaq is not None:
+        p.raw_format = L.RAW_ASAR_FBAQ4
+        p.samples_per_line, p.line_bytes, p.swst_mult = 0, int(fbaq[0]), int(fbaq[1])
     p.prf, p.wavelength, p.range_spacing = cfg.prf, cfg.wavelength, cfg.range_spacing
     p.slant_range_first, p.az_bandwidth_fraction = cfg.r_first, cfg.az_bandwidth_fraction
     for i, v in enumerate(cfg.vr_poly):
@@ -357,11 +362,11 @@ class Plan:
     def __init__(self, ctx: Context, cfg: SceneConfig, quirks: int = L.QUIRKS_REFERENCE, imp: bool = False,
                  chirp: np.ndarray | None = None, orbit_height: float | None = None, lines_before: int = 0,
                  lines_after: int = 0, srgr: tuple | None = None, azimuth_spacing: float | None = None,
-                 lines: tuple | None = None):
+                 lines: tuple | None = None, fbaq: tuple | None = None):
         self.ctx = ctx
         self.cfg = cfg
         p, self.grsr, self.ng = focus_params(cfg, quirks, imp, orbit_height, lines_before, lines_after, srgr,
-                                             azimuth_spacing, lines)
+                                             azimuth_spacing, lines, fbaq)
         self.nrv = cfg.nr - cfg.nc
         self.params = p
         self.chirp = generate_chirp(cfg, p.nrp) if chirp is None else np.ascontiguousarray(chirp, np.complex64)
@@ -393,6 +398,21 @@ class Plan:
         assert lines.shape[0] == self.cfg.na and lines.shape[1] == self.params.line_bytes
         assert swst.is_cuda and swst.dtype == torch.int16 and swst.numel() == self.cfg.na
         L.check(L.lib().asarf_plan_execute_lines(self.h, lines.data_ptr(), swst.data_ptr(), int(min_swst),
+                                                 ctypes.byref(self.slc),
+                                                 self.imp_out.data_ptr() if self.imp_out is not None else None))
+        return self.slc
+
+    def execute_fbaq4(self, records: torch.Tensor, swst: torch.Tensor, min_swst: int, i_lut: torch.Tensor,
+                      q_lut: torch.Tensor):
+        """asarf_plan_execute_fbaq4: ASAR FBAQ-4 records [na, record_bytes] uint8, their SWST codes [na] int16 and
+        the two 4096-entry float32 reconstruction tables, on the device."""
+        assert records.is_cuda and records.dtype == torch.uint8 and records.is_contiguous()
+        assert records.shape[0] == self.cfg.na and records.shape[1] == self.params.line_bytes
+        assert swst.is_cuda and swst.dtype == torch.int16 and swst.numel() == self.cfg.na
+        assert i_lut.is_cuda and q_lut.is_cuda and i_lut.dtype == torch.float32 and q_lut.dtype == torch.float32
+        assert i_lut.is_contiguous() and q_lut.is_contiguous() and i_lut.numel() == q_lut.numel()
+        L.check(L.lib().asarf_plan_execute_fbaq4(self.h, records.data_ptr(), swst.data_ptr(), int(min_swst),
+                                                 i_lut.data_ptr(), q_lut.data_ptr(), i_lut.numel(),
                                                  ctypes.byref(self.slc),
                                                  self.imp_out.data_ptr() if self.imp_out is not None else None))
         return self.slc

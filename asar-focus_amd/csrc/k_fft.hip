@@ -138,7 +138,10 @@ __global__ __launch_bounds__(kRowThreads) void k_range_compress(RcArgs a) {
         const float qc = (q - c.dc_q - ic * c.sin_corr) * c.inv_cos;
         return make_float2(ic, qc);
     };
-    const int shift = FMT == RAW_ERS_LINES ? a.lines.shift(row) : 0;
+    const bool lines_fmt = FMT == RAW_ERS_LINES || FMT == RAW_ASAR_FBAQ4;
+    const int shift = lines_fmt ? a.lines.shift(row) : 0;
+    const unsigned char* rec = static_cast<const unsigned char*>(a.in) + (lines_fmt ? (long)row * a.lines.line_bytes : 0L);
+    const int rec_len = FMT == RAW_ASAR_FBAQ4 ? fbaq4_record_length(rec) : 0;
     auto gl_load = [&](int x) -> float2 {
         if constexpr (FMT == RAW_CF32) {
             const float2* in = static_cast<const float2*>(a.in) + (long)row * a.in_stride;
@@ -148,9 +151,14 @@ __global__ __launch_bounds__(kRowThreads) void k_range_compress(RcArgs a) {
             // NaN where no line sample lands becomes 0 (ZeroNaNs); padding is 0
             const int sx = x - shift;
             if (x >= nr_in || sx < 0 || sx >= a.lines.spl) return make_float2(0.f, 0.f);
-            const uchar2 r = reinterpret_cast<const uchar2*>(static_cast<const uint8_t*>(a.in) +
-                                                              (long)row * a.lines.line_bytes)[sx];
+            const uchar2 r = reinterpret_cast<const uchar2*>(rec)[sx];
             return correct((float)r.x, (float)r.y);
+        } else if constexpr (FMT == RAW_ASAR_FBAQ4) {
+            // the record's sample shifted by the SWST; NaN where none lands becomes 0 (ZeroNaNs); padding is 0
+            float2 r;
+            if (x >= nr_in || !fbaq4_sample(rec, rec_len, a.lines.line_bytes, a.lines.lut, x - shift, &r))
+                return make_float2(0.f, 0.f);
+            return correct(r.x, r.y);
         } else {
             if (x >= nr_in) return make_float2(0.f, 0.f);
             using P = typename RawPair<typename std::conditional<FMT == RAW_U8, uint8_t, int8_t>::type>::type;
@@ -202,7 +210,9 @@ hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
     if (a.mfma) return launch_rc_mfma(a, fmt, s);
     const size_t lds = sizeof(float2) * (size_t)(lds_row_elems(a.plan.n) + tw_lds_elems(a.plan.n));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
-    if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8 && fmt != RAW_ERS_LINES) return hipErrorInvalidValue;
+    if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8 && fmt != RAW_ERS_LINES && fmt != RAW_ASAR_FBAQ4)
+        return hipErrorInvalidValue;
+    if (fmt == RAW_ASAR_FBAQ4 && !a.lines.lut) return hipErrorInvalidValue;
     hipError_t serr = hipSuccess;
     if (!a.no_static && launch_range_compress_static(a, fmt, s, &serr)) return serr;
     const int m = plan_mask(a.plan);
@@ -215,6 +225,9 @@ hipError_t launch_range_compress(const RcArgs& a, int fmt, hipStream_t s) {
         else if (fmt == RAW_ERS_LINES)                                                                          \
             hipLaunchKernelGGL((k_range_compress<RAW_ERS_LINES, SET>), dim3(a.rows), dim3(kRowThreads), lds, s, \
                                a);                                                                              \
+        else if (fmt == RAW_ASAR_FBAQ4)                                                                         \
+            hipLaunchKernelGGL((k_range_compress<RAW_ASAR_FBAQ4, SET>), dim3(a.rows), dim3(kRowThreads), lds,   \
+                               s, a);                                                                           \
         else                                                                                                    \
             hipLaunchKernelGGL((k_range_compress<RAW_I8, SET>), dim3(a.rows), dim3(kRowThreads), lds, s, a);    \
         return hipGetLastError();                                                                               \

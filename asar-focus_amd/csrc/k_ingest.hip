@@ -179,4 +179,9 @@ hipError_t launch_ingest(const IngestArgs& args, int kind, hipStream_t s) {
 
 int fbaq4_index_host(int block, int code) { return fbaq4_index(block, code); }
 
+hipError_t launch_fbaq4_lut_t(const float* i_lut, const float* q_lut, float2* lut_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_fbaq4_lut_t, dim3(16), dim3(256), 0, s, i_lut, q_lut, lut_t);
+    return hipGetLastError();
+}
+
 }  // namespace asarf

@@ -321,6 +321,7 @@ __global__ __launch_bounds__(kThreads) void k_col_mfma(ColFftArgs a) {
 // --------------------------------------------------------------------------------------------- launchers --
 hipError_t launch_rc_mfma(const RcArgs& a, int fmt, hipStream_t s) {
     if (a.plan.n != kMfRowN) return hipErrorInvalidValue;
+    if (fmt != RAW_CF32 && fmt != RAW_U8 && fmt != RAW_I8) return hipErrorInvalidValue;
     const size_t lds = sizeof(float2) * (size_t)(kMfRowLds + tw_lds_elems(kMfRowN));
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
 #define RC_(F)                                                                                                   \

@@ -246,6 +246,11 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
             const float2 r = in[x];
             lds[lpos<PS>(x)] = correct(r.x, r.y);
         }
+    } else if constexpr (FMT == RAW_ASAR_FBAQ4) {
+        // the row's record decoded and shifted by its SWST: NaN where no sample lands becomes 0, padding is 0
+        const unsigned char* rec = static_cast<const unsigned char*>(a.in) + (long)row * a.lines.line_bytes;
+        fbaq4_row_decode<N, T>(rec, a.lines.line_bytes, a.lines.lut, a.lines.shift(row), nr_in, lane, correct,
+                               [&](int x, float2 v) { lds[lpos<PS>(x)] = v; });
     } else if constexpr (FMT == RAW_ERS_LINES) {
         // the row's line shifted by its SWST: NaN where no line sample lands becomes 0 (ZeroNaNs), padding is 0
         const uchar2* line =
@@ -370,6 +375,8 @@ static hipError_t launch(const RcArgs& a, int fmt, hipStream_t s) {
         hipLaunchKernelGGL((k_rc_pad<RAW_U8, N, T, PS, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
     else if (fmt == RAW_ERS_LINES)
         hipLaunchKernelGGL((k_rc_pad<RAW_ERS_LINES, N, T, PS, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
+    else if (fmt == RAW_ASAR_FBAQ4)
+        hipLaunchKernelGGL((k_rc_pad<RAW_ASAR_FBAQ4, N, T, PS, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
     else
         hipLaunchKernelGGL((k_rc_pad<RAW_I8, N, T, PS, WAVES, Seq>), dim3(a.rows), dim3(T), lds, s, a);
     return hipGetLastError();

@@ -70,10 +70,25 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
         const float qc = (q - c.dc_q - ic * c.sin_corr) * c.inv_cos;
         return make_float2(ic, qc);
     };
+    constexpr bool LINES_FMT = FMT == RAW_ERS_LINES || FMT == RAW_ASAR_FBAQ4;
+    const int shift = LINES_FMT ? a.lines.shift(row) : 0;
+    const unsigned char* rec = static_cast<const unsigned char*>(a.in) + (LINES_FMT ? (long)row * a.lines.line_bytes : 0L);
+    const int rec_len = FMT == RAW_ASAR_FBAQ4 ? fbaq4_record_length(rec) : 0;
     auto gl_load = [&](int x) -> float2 {
         if constexpr (FMT == RAW_CF32) {
             const float2* in = static_cast<const float2*>(a.in) + (long)row * a.in_stride;
             const float2 r = in[x];
+            return correct(r.x, r.y);
+        } else if constexpr (FMT == RAW_ERS_LINES) {
+            // the line's sample shifted by the SWST; NaN where none lands becomes 0 (ZeroNaNs); padding is 0
+            const int sx = x - shift;
+            if (x >= nr_in || sx < 0 || sx >= a.lines.spl) return make_float2(0.f, 0.f);
+            const uchar2 r = reinterpret_cast<const uchar2*>(rec)[sx];
+            return correct((float)r.x, (float)r.y);
+        } else if constexpr (FMT == RAW_ASAR_FBAQ4) {
+            float2 r;
+            if (x >= nr_in || !fbaq4_sample(rec, rec_len, a.lines.line_bytes, a.lines.lut, x - shift, &r))
+                return make_float2(0.f, 0.f);
             return correct(r.x, r.y);
         } else {
             if (x >= nr_in) return make_float2(0.f, 0.f);
@@ -88,7 +103,13 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(WAVES))) void
 
     // forward transform
     constexpr int R0 = Seq::at(0);
-    if constexpr (STAGE && FMT != RAW_CF32) {
+    if constexpr (FMT == RAW_ASAR_FBAQ4) {
+        // the record decoded into LDS in natural order (phased loads), then the first pass reads LDS
+        fbaq4_row_decode<N, T>(rec, a.lines.line_bytes, a.lines.lut, shift, nr_in, lane, correct, lds_store);
+        __syncthreads();
+        pass_load<R0, E, V0>(v, lane, T, N, lds_load);
+        __syncthreads();  // every staged element is read before the passes overwrite LDS
+    } else if constexpr (STAGE && (FMT == RAW_U8 || FMT == RAW_I8)) {
         // raw bytes with 16-byte loads (8 I/Q pairs per load) into LDS in natural order, then the first pass
         // reads LDS: one wide load instead of eight 2-byte loads per lane
         const uint8_t* rb = static_cast<const uint8_t*>(a.in) + (long)row * a.in_stride * 2;
@@ -178,6 +199,9 @@ static hipError_t launch_static(const RcArgs& a, int fmt, hipStream_t s) {
     else if (fmt == RAW_ERS_LINES)
         hipLaunchKernelGGL((k_rc_static<RAW_ERS_LINES, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T),
                            lds, s, a);
+    else if (fmt == RAW_ASAR_FBAQ4)
+        hipLaunchKernelGGL((k_rc_static<RAW_ASAR_FBAQ4, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows),
+                           dim3(T), lds, s, a);
     else
         hipLaunchKernelGGL((k_rc_static<RAW_I8, N, T, PADSHIFT, WAVES, Seq, STAGE, TW1>), dim3(a.rows), dim3(T), lds,
                            s, a);

@@ -24,17 +24,74 @@ constexpr int kAzCoreReach = 8;
 // Largest integer RCMC shift (pixels) covered by the column-tiled RCMC kernel's right halo.
 constexpr int kRcmcTileReach = 24;
 
-enum RawFormat { RAW_CF32 = 0, RAW_U8 = 1, RAW_I8 = 2, RAW_ERS_LINES = 3 };
+enum RawFormat { RAW_CF32 = 0, RAW_U8 = 1, RAW_I8 = 2, RAW_ERS_LINES = 3, RAW_ASAR_FBAQ4 = 4 };
 
-// ERS IM level-0 lines (RAW_ERS_LINES): line y holds spl u8 (I, Q) pairs at bytes + y * line_bytes; its sample s
-// lands on range sample s + mult * (swst[y] - min_swst); range samples below the row's logical width that no
-// sample lands on are NaN (zero after the I/Q correction, ZeroNaNs).
+// Level-0 lines. RAW_ERS_LINES: line y holds spl u8 (I, Q) pairs at bytes + y * line_bytes. RAW_ASAR_FBAQ4: record y
+// at bytes + y * line_bytes holds a 2-byte little-endian length L, then L bytes in blocks of 64 (a quantiser block
+// id and 63 code bytes, I code in the high nibble); sample s is code byte 64 (s / 63) + 1 + s % 63 when that is
+// below L, reconstructed from the block-major table lut (16 entries per block id). Sample s of line y lands on
+// range sample s + mult * (swst[y] - min_swst); range samples below the row's logical width that no sample lands
+// on are NaN (zero after the I/Q correction, ZeroNaNs).
 struct LinesDesc {
     const short* swst;
     int min_swst, mult, spl;
     long line_bytes;
+    const float2* lut;  // RAW_ASAR_FBAQ4: lut[16 b + c] = (I, Q) of block id b and converted code c
     __device__ __forceinline__ int shift(int y) const { return mult * ((int)swst[y] - min_swst); }
 };
+
+// FBAQ-4 sample s of the record rec (length len from its first two bytes) into *v; false where the record has no
+// such sample (Fbaq4IndexCalc, envisat_parse_util.cuh:75-84: codes 8..15 -> 7..0, 0..7 -> 8..15).
+__device__ __forceinline__ int fbaq4_record_length(const unsigned char* rec) { return (int)rec[0] | ((int)rec[1] << 8); }
+__device__ __forceinline__ bool fbaq4_sample(const unsigned char* rec, int len, long rec_bytes, const float2* lut,
+                                             int s, float2* v) {
+    if (s < 0) return false;
+    const int blk = s / 63;
+    const int k = 64 * blk + 1 + (s - 63 * blk);
+    if (k >= len || 2 + (long)k >= rec_bytes) return false;
+    const int bid = rec[2 + 64 * blk], code = rec[2 + k];
+    const int ci = code >> 4, cq = code & 15;
+    const float2* lt = lut + 16 * bid;
+    *v = make_float2(lt[ci > 7 ? 15 - ci : 8 + ci].x, lt[cq > 7 ? 15 - cq : 8 + cq].y);
+    return true;
+}
+
+// One row of N range samples from an FBAQ-4 record, lane `lane` of T taking x = lane, lane + T, ...: the record's
+// sample x - shift where one lands below nr_in (through correct()), else 0 (NaN after ZeroNaNs, or padding), handed
+// to store(x, value). Decoded in phases (every block id and code byte of the lane, then every table entry, then the
+// stores), so that all of a lane's dependent loads are in flight together.
+template <int N, int T, class Correct, class Store>
+__device__ __forceinline__ void fbaq4_row_decode(const unsigned char* rec, long rec_bytes, const float2* lut, int shift,
+                                                 int nr_in, int lane, Correct correct, Store store) {
+    constexpr int IT = (N + T - 1) / T;
+    const int len = fbaq4_record_length(rec);
+    int bid[IT], code[IT];
+    bool ok[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int x = lane + it * T;
+        const int s = x - shift;
+        const int blk = s >= 0 ? s / 63 : 0;
+        const int k = 64 * blk + 1 + (s - 63 * blk);
+        ok[it] = x < N && x < nr_in && s >= 0 && k < len && 2 + (long)k < rec_bytes;
+        bid[it] = code[it] = 0;
+        if (ok[it]) {
+            bid[it] = rec[2 + 64 * blk];
+            code[it] = rec[2 + k];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int x = lane + it * T;
+        float2 v = make_float2(0.f, 0.f);
+        if (ok[it]) {
+            const int ci = code[it] >> 4, cq = code[it] & 15;
+            const float2* lt = lut + 16 * bid[it];
+            v = correct(lt[ci > 7 ? 15 - ci : 8 + ci].x, lt[cq > 7 ? 15 - cq : 8 + cq].y);
+        }
+        if (x < N) store(x, v);
+    }
+}
 
 // Affine I/Q correction: I' = (I - dc_i) * inv_gain, Q' = (Q - dc_q - I' * sin_corr) * inv_cos.
 struct IqCoef {
@@ -90,7 +147,7 @@ struct RcArgs {
     int mfma;           // split-bfloat16 MFMA transforms (length 8192 only)
     int no_static;      // use the runtime-radix kernel even where a compile-time kernel exists (autotuner)
     int static_variant; // which compile-time variant of the length (autotuner; 0 = default)
-    LinesDesc lines;    // RAW_ERS_LINES
+    LinesDesc lines;    // RAW_ERS_LINES, RAW_ASAR_FBAQ4
 };
 
 struct ColFftArgs {
@@ -168,7 +225,7 @@ struct RawMomentsArgs {
     long long* partials; // [strip][tile][kRawStats + nslots*kRawLag]
     long long* corr;     // vec 8: [nstrips][nblocks][kRawLag] lag sums of the odd block-start columns (written for
                          // odd starts only)
-    LinesDesc lines;     // RAW_ERS_LINES: rows are shifted lines, NaN where no sample lands
+    LinesDesc lines;     // RAW_ERS_LINES, RAW_ASAR_FBAQ4: rows are shifted lines, NaN where no sample lands
 };
 
 struct RawReduceArgs {
@@ -178,10 +235,12 @@ struct RawReduceArgs {
     const long long* corr;  // vec 8: straddle-column lag sums ([nstrips][nblocks][kRawLag]) to move between
                             // neighbouring blocks (else null)
     long long* totals;  // [kRawStats + nblocks*kRawLag]
+    int f64;            // partials and totals hold FP64 sums (bit patterns in the int64 slots): RAW_ASAR_FBAQ4
 };
 
 struct FinalizeArgs {
     const long long* totals; // [kRawStats + nblocks*kRawLag]
+    int f64;                 // totals are FP64 bit patterns (RAW_ASAR_FBAQ4) instead of exact integer sums
     int nblocks, step, nr, na, x_stride;
     double n_total;
     int quirks;
@@ -305,6 +364,8 @@ hipError_t launch_zero_cols(float2* img, long stride, int rows, int c0, int c1, 
 hipError_t launch_ingest(const IngestArgs& a, int kind, hipStream_t s);  // kind 0 = ERS IM, 1 = ASAR FBAQ-4
 hipError_t launch_condition(const OutputArgs& a, int detected, hipStream_t s);
 int fbaq4_index_host(int block, int code);
+// Block-major copy of the two 4096-entry FBAQ-4 tables: lut_t[16 b + c] = (i_lut[256 c + b], q_lut[256 c + b]).
+hipError_t launch_fbaq4_lut_t(const float* i_lut, const float* q_lut, float2* lut_t, hipStream_t s);
 
 // Max dynamic LDS bytes per workgroup the launchers may request (set by the context at initialisation).
 size_t max_lds_bytes();

@@ -64,7 +64,8 @@ struct asarf_plan {
     bool rcmc_row_only = false;
     const asarf_rcmc_net* net = nullptr;
     int fft_mfma = 0;  // split-bfloat16 MFMA transforms
-    LinesDesc lines{};  // ERS level-0 lines of the current execution (asarf_plan_execute_lines)
+    LinesDesc lines{};  // level-0 lines of the current execution (asarf_plan_execute_lines / _fbaq4)
+    float2* fbaq_lut = nullptr;  // RAW_ASAR_FBAQ4: block-major copy of the reconstruction tables (4096 entries)
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -87,8 +88,8 @@ static int plan_alloc(asarf_plan* pl, void** p, size_t bytes) {
 }
 
 static void plan_free(asarf_plan* pl) {
-    void* bufs[] = {pl->img,    pl->img2,   pl->hconj, pl->col, pl->partials,
-                    pl->totals, pl->corr,   pl->res,   pl->ml,  pl->rsl};
+    void* bufs[] = {pl->img,    pl->img2, pl->hconj, pl->col, pl->partials, pl->totals,
+                    pl->corr,   pl->res,  pl->ml,    pl->rsl, pl->fbaq_lut};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : pl->ev) (void)hipEventDestroy(e);
@@ -98,8 +99,14 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     if (!c || !p || !chirp_iq || !out) return set_error(ASARF_EINVAL, "asarf_plan_create: null argument");
     if (p->na < 2 || p->nr < 32 || p->nc <= 0 || p->nc >= p->nr)
         return set_error(ASARF_ESHAPE, "asarf_plan_create: na=%d nr=%d nc=%d", p->na, p->nr, p->nc);
-    if (p->raw_format != ASARF_RAW_U8 && p->raw_format != ASARF_RAW_I8 && p->raw_format != ASARF_RAW_ERS_LINES)
-        return set_error(ASARF_EINVAL, "asarf_plan_create: raw_format %d (u8, i8 or ERS lines)", p->raw_format);
+    if (p->raw_format != ASARF_RAW_U8 && p->raw_format != ASARF_RAW_I8 && p->raw_format != ASARF_RAW_ERS_LINES &&
+        p->raw_format != ASARF_RAW_ASAR_FBAQ4)
+        return set_error(ASARF_EINVAL, "asarf_plan_create: raw_format %d (u8, i8, ERS lines or ASAR FBAQ-4 records)",
+                         p->raw_format);
+    if (p->raw_format == ASARF_RAW_ASAR_FBAQ4 && (p->line_bytes < 3 || p->swst_mult <= 0))
+        return set_error(ASARF_EINVAL,
+                         "asarf_plan_create: FBAQ-4 records need line_bytes (%ld) >= 3 and swst_mult (%d) > 0",
+                         p->line_bytes, p->swst_mult);
     if (p->raw_format == ASARF_RAW_ERS_LINES &&
         (p->samples_per_line <= 0 || p->samples_per_line > p->nr || p->line_bytes < 2L * p->samples_per_line ||
          p->swst_mult <= 0))
@@ -151,6 +158,7 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
                        p->line_bytes % 8 == 0)
                           ? 4
                           : 1;
+    if (p->raw_format == ASARF_RAW_ASAR_FBAQ4) pl->raw_vec = 1;  // FP64 moments, one column per lane
     pl->nstrips = (p->na + kStripRows - 1) / kStripRows;
     auto fail = [&](int st) {
         plan_free(pl);
@@ -201,6 +209,9 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     if ((st = plan_alloc(pl, (void**)&pl->corr, sizeof(long long) * (size_t)pl->nstrips * nblocks * kRawLag)))
         return fail(st);
     if ((st = plan_alloc(pl, (void**)&pl->res, sizeof(DevResults)))) return fail(st);
+    if (p->raw_format == ASARF_RAW_ASAR_FBAQ4 &&
+        (st = plan_alloc(pl, (void**)&pl->fbaq_lut, sizeof(float2) * 4096)))
+        return fail(st);
 
     // chirp spectrum conj(FFT(chirp))/n: inverse-direction transform of the conjugated chirp
     {
@@ -324,10 +335,12 @@ static int enqueue_to_spectrum(asarf_plan* pl, const void* raw, bool core_path) 
     rr.step = pl->step;
     rr.nblocks = pl->nblocks;
     rr.totals = pl->totals;
+    rr.f64 = p.raw_format == ASARF_RAW_ASAR_FBAQ4;
     HIP_CHECK(launch_reduce_raw(rr, s));
     ST_CHECK(mark(pl, "reduce"));
     FinalizeArgs fa{};
     fa.totals = pl->totals;
+    fa.f64 = rr.f64;
     fa.nblocks = pl->nblocks;
     fa.step = pl->step;
     fa.nr = p.nr;
@@ -409,7 +422,30 @@ extern "C" int asarf_plan_execute(asarf_plan* pl, const void* raw, asarf_image* 
     if (pl->p.imp && !imp_out) return set_error(ASARF_EINVAL, "asarf_plan_execute: IMP output required");
     if (pl->p.raw_format == ASARF_RAW_ERS_LINES)
         return set_error(ASARF_EINVAL, "asarf_plan_execute: a plan of ERS lines runs with asarf_plan_execute_lines");
+    if (pl->p.raw_format == ASARF_RAW_ASAR_FBAQ4)
+        return set_error(ASARF_EINVAL, "asarf_plan_execute: a plan of FBAQ-4 records runs with asarf_plan_execute_fbaq4");
     return plan_execute(pl, raw, slc, imp_out);
+}
+
+extern "C" int asarf_plan_execute_fbaq4(asarf_plan* pl, const void* records, const short* swst, int min_swst,
+                                        const float* i_lut, const float* q_lut, int lut_items, asarf_image* slc,
+                                        float* imp_out) {
+    if (!pl || !records || !swst || !i_lut || !q_lut || !slc)
+        return set_error(ASARF_EINVAL, "asarf_plan_execute_fbaq4: null argument");
+    if (pl->p.imp && !imp_out) return set_error(ASARF_EINVAL, "asarf_plan_execute_fbaq4: IMP output required");
+    if (pl->p.raw_format != ASARF_RAW_ASAR_FBAQ4)
+        return set_error(ASARF_EINVAL, "asarf_plan_execute_fbaq4: the plan was created for raw format %d",
+                         pl->p.raw_format);
+    if (lut_items < 4096)
+        return set_error(ASARF_ESHAPE, "asarf_plan_execute_fbaq4: the tables need 4096 entries, not %d", lut_items);
+    HIP_CHECK(launch_fbaq4_lut_t(i_lut, q_lut, pl->fbaq_lut, pl->ctx->stream));
+    pl->lines.swst = swst;
+    pl->lines.min_swst = min_swst;
+    pl->lines.mult = pl->p.swst_mult;
+    pl->lines.spl = 0;
+    pl->lines.line_bytes = pl->p.line_bytes;
+    pl->lines.lut = pl->fbaq_lut;
+    return plan_execute(pl, records, slc, imp_out);
 }
 
 extern "C" int asarf_plan_execute_lines(asarf_plan* pl, const void* lines, const short* swst, int min_swst,
@@ -645,7 +681,7 @@ extern "C" int asarf_plan_set_rcmc(asarf_plan* pl, int interp, const asarf_rcmc_
 extern "C" int asarf_plan_rcmc_pairs(asarf_plan* pl, const void* raw, long n_pairs, unsigned long long seed,
                                      float* pairs) {
     if (!pl || !raw || !pairs || n_pairs <= 0) return set_error(ASARF_EINVAL, "asarf_plan_rcmc_pairs: argument");
-    if (pl->p.raw_format == ASARF_RAW_ERS_LINES)
+    if (pl->p.raw_format != ASARF_RAW_U8 && pl->p.raw_format != ASARF_RAW_I8)
         return set_error(ASARF_EUNSUPPORTED, "asarf_plan_rcmc_pairs: plans of u8 or i8 raw samples only");
     if (pl->nrv < 32) return set_error(ASARF_ESHAPE, "asarf_plan_rcmc_pairs: too few range samples");
     pl->ev_names.clear();
@@ -674,6 +710,8 @@ extern "C" int asarf_plan_set_fft(asarf_plan* pl, int precision) {
         return clear_error();
     }
     if (precision != ASARF_FFT_SPLIT_BF16) return set_error(ASARF_EINVAL, "asarf_plan_set_fft: precision %d", precision);
+    if (pl->p.raw_format != ASARF_RAW_U8 && pl->p.raw_format != ASARF_RAW_I8)
+        return set_error(ASARF_EUNSUPPORTED, "asarf_plan_set_fft: split-bf16 transforms take u8 or i8 raw samples");
     auto col_ok = [](int n) { return n == 128 || n == 256; };
     if (pl->nrp != 8192)
         return set_error(ASARF_EUNSUPPORTED, "asarf_plan_set_fft: split-bf16 range transforms need nrp = 8192, not %d",

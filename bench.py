@@ -250,6 +250,114 @@ def ingest_probe(reps=20):
     return {"ingest_probe": res, "lines": lines, "samples_per_line": spl, "padded_image": [x_stride, y_stride]}
 
 
+def l0_plan_probe(steps=10):
+    """The plan fed with level-0 input at full scene size against the same plan on raw I/Q byte pairs:
+    * ASAR IM scene 30000 x 5616 as FBAQ-4 records (the scene's int8 samples quantised on the device: per block of
+      63 samples the quantiser step that spans its largest sample, 16 levels per step), SWST runs over 0-6 codes
+      (range size 5622, so the range transform stays 6144 long as in the raw-pair plan);
+    * ERS block 4096 x 5616 as u8 level-0 lines with SWST runs over 0-19 codes (range size 5692).
+    ms = one plan execution (SLC + IMP); Msamples/s counts the raw samples of the raw-pair plan."""
+    import numpy as np
+    import torch
+    from asarf import configs, focus, scene
+    ctx = focus.Context(0)
+    out = {}
+    g = torch.Generator(device="cuda").manual_seed(9)
+    # ASAR FBAQ-4
+    cfg = configs.asar_scene()
+    raw, _ = scene.generate(cfg)
+    traw = torch.from_numpy(raw).cuda()
+    plan = focus.Plan(ctx, cfg, imp=True)
+    ms_raw = _time_plan(plan, traw, steps)
+    plan.close()
+    na, spl = cfg.na, cfg.nr
+    nblk = (spl + 62) // 63
+    iq = traw.view(torch.int8).view(na, spl, 2).float()
+    iq = torch.nn.functional.pad(iq, (0, 0, 0, nblk * 63 - spl)).view(na, nblk, 63, 2)
+    amp = iq.abs().amax(dim=(2, 3)).clamp_min(1e-3)
+    blk = torch.clamp(torch.ceil((amp / 7.5 - 0.5) / 0.1), 0, 255)
+    step = (0.5 + 0.1 * blk)[:, :, None, None]
+    conv = torch.clamp(torch.round(iq / step + 7.5), 0, 15).to(torch.int32)
+    code = torch.where(conv >= 8, conv - 8, 15 - conv)
+    byte = ((code[..., 0] << 4) | code[..., 1]).to(torch.uint8)
+    data_len = 64 * (nblk - 1) + 1 + (spl - 63 * (nblk - 1))
+    rec_bytes = (2 + 64 * nblk + 15) // 16 * 16
+    rec = torch.zeros((na, rec_bytes), dtype=torch.uint8, device="cuda")
+    body = torch.cat([blk.to(torch.uint8)[:, :, None], byte], dim=2).reshape(na, 64 * nblk)
+    rec[:, 2:2 + 64 * nblk] = body
+    rec[:, 2 + data_len:] = 0
+    rec[:, 0] = data_len & 0xFF
+    rec[:, 1] = data_len >> 8
+    swst = (700 + torch.tensor([0, 3, 6], device="cuda")[torch.randint(0, 3, (na // 100 + 1,), device="cuda",
+                                                                        generator=g)]).repeat_interleave(100)[:na]
+    swst = swst.to(torch.int16).contiguous()
+    conv_l = torch.arange(16, device="cuda", dtype=torch.float32)[:, None]
+    i_lut = ((conv_l - 7.5) * (0.5 + 0.1 * torch.arange(256, device="cuda", dtype=torch.float32))[None, :]).reshape(-1)
+    q_lut = (1.02 * i_lut).contiguous()
+    fcfg = cfg.scaled(nr=spl + 6)
+    plan = focus.Plan(ctx, fcfg, imp=True, fbaq=(rec_bytes, 1))
+    run = lambda: plan.execute_fbaq4(rec, swst, 700, i_lut, q_lut)  # noqa: E731
+    for _ in range(2):
+        run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fbaq = e0.elapsed_time(e1) / steps
+    plan.set_timing(True)
+    run()
+    st = {k: round(v, 4) for k, v in plan.kernel_times().items()}
+    plan.set_timing(False)
+    plan.close()
+    out["asar_fbaq4_records"] = {"workload": cfg.name, "record_bytes": rec_bytes,
+                                 "fft_sizes": [focus.padding_size(fcfg.nr + fcfg.nc), focus.padding_size(na)],
+                                 "raw_pairs_plan_ms": round(ms_raw, 4),
+                                 "fbaq4_plan_ms": round(ms_fbaq, 4),
+                                 "fbaq4_msamples_per_s": round(na * spl / (ms_fbaq * 1e-3) / 1e6, 1),
+                                 "fbaq4_stages_ms": st}
+    del traw, iq, rec, body, byte, code, conv
+    torch.cuda.empty_cache()
+    # ERS lines
+    cfg = configs.ers_block()
+    raw, _ = scene.generate(cfg)
+    traw = torch.from_numpy(raw).cuda()
+    plan = focus.Plan(ctx, cfg, imp=True)
+    ms_raw = _time_plan(plan, traw, steps)
+    plan.close()
+    na, spl = cfg.na, cfg.nr
+    line_bytes = 2 * spl
+    lines = traw.view(na, line_bytes)
+    swst = (700 + torch.tensor([0, 9, 19], device="cuda")[torch.randint(0, 3, (na // 100 + 1,), device="cuda",
+                                                                        generator=g)]).repeat_interleave(100)[:na]
+    swst = swst.to(torch.int16).contiguous()
+    lcfg = cfg.scaled(nr=spl + 4 * 19)
+    plan = focus.Plan(ctx, lcfg, imp=True, lines=(spl, line_bytes, 4))
+    run = lambda: plan.execute_lines(lines, swst, 700)  # noqa: E731
+    for _ in range(2):
+        run()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_lines = e0.elapsed_time(e1) / steps
+    plan.set_timing(True)
+    run()
+    st = {k: round(v, 4) for k, v in plan.kernel_times().items()}
+    plan.set_timing(False)
+    plan.close()
+    out["ers_lines"] = {"workload": cfg.name, "fft_sizes": [lcfg.nrp or focus.padding_size(lcfg.nr + lcfg.nc),
+                                                             focus.padding_size(na)],
+                        "raw_pairs_plan_ms": round(ms_raw, 4), "lines_plan_ms": round(ms_lines, 4),
+                        "lines_msamples_per_s": round(na * spl / (ms_lines * 1e-3) / 1e6, 1), "lines_stages_ms": st}
+    ctx.close()
+    return {"l0_plan_probe": out}
+
+
 def stages_probe(reps=5):
     """The reference's call sequence (main.cc:243-283) through the stage entry points at the ASAR scene size:
     RawDataCorrection, CalculateDopplerCentroid, RangeCompression and RangeDopplerAlgorithm on one padded cf32
@@ -388,6 +496,8 @@ def main():
     ap.add_argument("--ingest-probe", action="store_true",
                     help="time the level-0 decoders (ERS u8, ASAR FBAQ-4) and the output conditioning at full scene "
                          "size and print their rates")
+    ap.add_argument("--l0-probe", action="store_true",
+                    help="the plan fed with FBAQ-4 records / ERS level-0 lines at full size against raw I/Q pairs")
     ap.add_argument("--stages-probe", action="store_true",
                     help="time the reference's stage sequence (I/Q correction, Doppler centroid, range compression, "
                          "range-Doppler) through the stage entry points at the ASAR scene size")
@@ -414,6 +524,9 @@ def main():
         return
     if args.stages_probe:
         print(json.dumps(stages_probe()))
+        return
+    if args.l0_probe:
+        print(json.dumps(l0_plan_probe()))
         return
     if args.copy_probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")

@@ -27,16 +27,23 @@ def _lines(cfg, spl, line_bytes, codes, rng):
     return lines, runs.astype(np.int16)
 
 
-@pytest.mark.parametrize("spl,pad,imp", [(1400, 32, False), (1400, 6, False), (1402, 8, False), (1400, 32, True)])
-def test_ers_lines_plan_matches_oracle(ctx, spl, pad, imp):
+# The last two cases reach the compile-time range kernels: 5616 + 4 * 19 range samples (+ a 704-sample chirp) pad
+# to 6400 (k_rc_static), 5424 + 4 * 3 to 6144 (k_rc_pad).
+@pytest.mark.parametrize("spl,pad,imp,steps", [(1400, 32, False, (0, 22, 44)), (1400, 6, False, (0, 22, 44)),
+                                               (1402, 8, False, (0, 22, 44)), (1400, 32, True, (0, 22, 44)),
+                                               (5616, 32, False, (0, 9, 19)), (5424, 16, False, (0, 2, 3))])
+def test_ers_lines_plan_matches_oracle(ctx, spl, pad, imp, steps):
     rng = np.random.default_rng(spl + pad)
-    codes = np.array([700, 722, 744])
-    base = configs.ers_block().scaled(na=1500, nr=spl, n_targets_az=2, n_targets_rg=2, tgt_az=(0.4, 0.6))
+    codes = 700 + np.array(steps)
+    base = configs.ers_block().scaled(na=1500 if spl < 5000 else 1100, nr=spl, n_targets_az=2, n_targets_rg=2,
+                                      tgt_az=(0.4, 0.6))
     lines, swst = _lines(base, spl, 2 * spl + pad, codes, rng)
     min_swst = int(swst.min())
     range_size = spl + 4 * (int(swst.max()) - min_swst)
     cfg = base.scaled(nr=range_size)
     plan = focus.Plan(ctx, cfg, imp=imp, lines=(spl, lines.shape[1], 4))
+    if spl > 5000:
+        assert plan.params.nrp in (6144, 6400)
     plan.execute_lines(torch.from_numpy(lines).cuda(), torch.from_numpy(swst).cuda(), min_swst)
     slc = plan.slc_tensor().cpu().numpy()
     res = plan.results()

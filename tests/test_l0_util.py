@@ -163,12 +163,16 @@ def test_product_types_reference_cases():
 
 
 # ------------------------------------------------------------------------------------- ASAR IM packets ----
-def asar_packet(rng, blocks, tail, swst=700, mode=0x54, dfh=29, flags=0x80, extra=0, sensing=(-3026, 77935, 870958)):
-    """A data set record laid out as envisat_im_parse.cc:433-600 reads it (big-endian fields)."""
-    data_len = 64 * blocks + tail
-    data = bytearray(rng.integers(0, 256, data_len, dtype=np.uint8).tobytes())
-    for b in range(blocks + (1 if tail else 0)):
-        data[64 * b] = int(rng.integers(0, 256))  # block id
+def asar_packet(rng, blocks, tail, swst=700, mode=0x54, dfh=29, flags=0x80, extra=0, sensing=(-3026, 77935, 870958),
+                data=None):
+    """A data set record laid out as envisat_im_parse.cc:433-600 reads it (big-endian fields); `data` (block ids and
+    code bytes) replaces the random sample blocks."""
+    if data is None:
+        data_len = 64 * blocks + tail
+        data = bytearray(rng.integers(0, 256, data_len, dtype=np.uint8).tobytes())
+        for b in range(blocks + (1 if tail else 0)):
+            data[64 * b] = int(rng.integers(0, 256))  # block id
+    data_len = len(data)
     hdr = struct.pack(">iII", *sensing)                                       # ISP sensing time
     hdr += struct.pack(">iIIHHHH", -3026, 77935, 870000, data_len + 29, 0, 0, 0)  # FEP (ISP length)
     hdr += struct.pack(">HHHH", 0x0A11, (1 << 14) | 513, data_len + 29, dfh)  # packet id, sequence, lengths
