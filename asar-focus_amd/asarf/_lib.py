@@ -16,7 +16,7 @@ OK, EINVAL, ESHAPE, EWORKSPACE, EHIP, ENOMEM, EUNSUPPORTED = range(7)
 Q1_IQ_PADDED_WIDTH, Q2_COS_NO_SQRT, Q4_DC_EXTRA_BLOCK = 1, 2, 4
 QUIRKS_REFERENCE = 7
 CF32, F32 = 0, 1
-RAW_CF32, RAW_U8, RAW_I8, RAW_ERS_LINES = 0, 1, 2, 3
+RAW_CF32, RAW_U8, RAW_I8, RAW_ERS_LINES, RAW_ASAR_FBAQ4 = 0, 1, 2, 3, 4
 FORWARD, INVERSE = 0, 1
 FFT_ROWS, FFT_COLUMNS = 0, 1
 
@@ -90,6 +90,31 @@ class Mjd(ctypes.Structure):
         return (self.days, self.seconds, self.micros)
 
 
+class DorisMeta(ctypes.Structure):
+    """asarf_doris_meta: L1 metadata of a DORIS orbit file."""
+    _fields_ = [("orbit_name", ctypes.c_char * 64), ("vector_source", ctypes.c_char * 3), ("phase", ctypes.c_char),
+                ("cycle", ctypes.c_uint), ("delta_ut1", ctypes.c_double), ("abs_orbit", ctypes.c_uint),
+                ("quality", ctypes.c_uint), ("state_vector_time", Mjd), ("pos", ctypes.c_double * 3),
+                ("vel", ctypes.c_double * 3)]
+
+
+class InsSwath(ctypes.Structure):
+    """asarf_ins_swath: an image-mode swath's parameters from the instrument characterisation file."""
+    _fields_ = [("radar_frequency", ctypes.c_float), ("radar_sampling_rate", ctypes.c_float),
+                ("range_gate_bias", ctypes.c_float), ("chirp_amplitude", ctypes.c_float * 4),
+                ("chirp_phase", ctypes.c_float * 4), ("chirp_duration", ctypes.c_float),
+                ("swst_rank", ctypes.c_ushort)]
+
+
+class Timing(ctypes.Structure):
+    """asarf_timing: the radar timing of the level-0 flow."""
+    _fields_ = [("prf", ctypes.c_double), ("line_time_interval", ctypes.c_double),
+                ("two_way_slant_range_time", ctypes.c_double), ("slant_range_first", ctypes.c_double),
+                ("wavelength", ctypes.c_double), ("range_spacing", ctypes.c_double), ("chirp_kr", ctypes.c_double),
+                ("chirp_duration", ctypes.c_double), ("chirp_bandwidth", ctypes.c_double),
+                ("chirp_samples", ctypes.c_int)]
+
+
 class Patc(ctypes.Structure):
     _fields_ = [("cyclic_counter", ctypes.c_ushort), ("mission", ctypes.c_char * 2), ("orbit_number", ctypes.c_uint),
                 ("epoch_1950", ctypes.c_int), ("milliseconds", ctypes.c_int), ("sbt_counter", ctypes.c_uint),
@@ -153,6 +178,43 @@ class Geometry(ctypes.Structure):
                 ("dc_poly", ctypes.c_double * 3), ("center_point", ctypes.c_double * 3)]
 
 
+class SqThresholds(ctypes.Structure):
+    """asarf_sq_thresholds: summary-quality thresholds of the processor configuration file."""
+    _fields_ = [(n, ctypes.c_float) for n in (
+        "thresh_chirp_broadening", "thresh_chirp_sidelobe", "thresh_chirp_islr", "thresh_input_mean",
+        "exp_input_mean", "thresh_input_std_dev", "exp_input_std_dev", "thresh_dop_cen", "thresh_dop_amb",
+        "thresh_output_mean", "exp_output_mean", "thresh_output_std_dev", "exp_output_std_dev",
+        "thresh_input_missing_lines", "thresh_input_gaps")] + [("lines_per_gap", ctypes.c_uint)]
+
+
+class L1Inputs(ctypes.Structure):
+    """asarf_l1_inputs: the metadata of a level-1 header."""
+    _cs = ctypes.c_char_p
+    _us = ctypes.c_ushort
+    _fields_ = [("l0_product_name", _cs), ("detected", ctypes.c_int), ("acquisition_station", _cs),
+                ("processing_station", _cs), ("software_version", _cs), ("processing_time", Mjd),
+                ("sensing_start", Mjd), ("sensing_stop", Mjd), ("first_line_time", Mjd), ("last_line_time", Mjd),
+                ("orbit", ctypes.POINTER(DorisMeta)), ("rel_orbit", ctypes.c_int), ("utc_sbt_time", Mjd),
+                ("sat_binary_time", ctypes.c_uint), ("clock_step", ctypes.c_uint), ("leap_utc", Mjd),
+                ("leap_sign", ctypes.c_int), ("leap_err", ctypes.c_char), ("product_err", ctypes.c_int),
+                ("swath", _cs), ("ascending", ctypes.c_int), ("polarization", _cs), ("echo_comp_method", _cs),
+                ("echo_comp_ratio", _cs), ("init_cal_method", _cs), ("init_cal_ratio", _cs), ("per_cal_method", _cs),
+                ("per_cal_ratio", _cs), ("noise_method", _cs), ("noise_ratio", _cs), ("configuration_file", _cs),
+                ("instrument_file", _cs), ("external_characterization_file", _cs),
+                ("external_calibration_file", _cs), ("orbit_dataset_name", _cs),
+                ("geometry", ctypes.POINTER(Geometry)), ("azimuth_spacing", ctypes.c_double),
+                ("two_way_slant_range_time", ctypes.c_double), ("range_sampling_rate", ctypes.c_double),
+                ("chirp_window", ctypes.c_int), ("chirp_alpha", ctypes.c_double), ("chirp_kr", ctypes.c_double),
+                ("azimuth_window_alpha", ctypes.c_double), ("grsr5", ctypes.POINTER(ctypes.c_double)),
+                ("first_sbt", ctypes.c_uint), ("first_mjd", Mjd), ("first_swst_code", _us), ("last_swst_code", _us),
+                ("pri_code", _us), ("tx_pulse_len_code", _us), ("tx_bw_code", _us), ("echo_win_len_code", _us),
+                ("up_code", _us), ("down_code", _us), ("resamp_code", _us), ("beam_adj_code", _us),
+                ("beam_set_num_code", _us), ("tx_monitor_code", _us), ("swst_changes", ctypes.c_uint),
+                ("swst_rank", ctypes.c_uint), ("tx_monitor_value", ctypes.c_float), ("iq", IqResults),
+                ("statistics", ctypes.c_double * 4), ("sq", SqThresholds), ("mds_records", ctypes.c_int),
+                ("mds_record_size", ctypes.c_int)]
+
+
 SIGNATURES = {
     "asarf_ctx_create": (_ci, [_ci, _vp, _P(_vp)]),
     "asarf_ctx_destroy": (_ci, [_vp]),
@@ -193,6 +255,17 @@ SIGNATURES = {
     "asarf_scene_center": (_ci, [_P(Geometry), _P(_cd), _P(_cd), _P(_cd)]),
     "asarf_aperture_pixels": (_ci, [_P(Geometry), _P(_ci), _ci, _P(_ci)]),
     "asarf_geolocation_tie_points": (_ci, [_P(Geometry), _ci, ctypes.c_void_p, _P(TiePoints)]),
+    "asarf_con_summary_quality": (_ci, [ctypes.c_char_p, ctypes.c_long, _ci, _P(SqThresholds)]),
+    "asarf_l1_header_size": (ctypes.c_long, [_ci]),
+    "asarf_l1_header": (_ci, [_P(L1Inputs), ctypes.c_void_p, ctypes.c_long, _P(ctypes.c_long)]),
+    "asarf_doris_orbit": (_ci, [ctypes.c_char_p, ctypes.c_long, _P(Mjd), _P(Mjd), _P(Osv), _P(Mjd), _ci, _P(_ci),
+                                _P(DorisMeta)]),
+    "asarf_doris_filename_window": (_ci, [ctypes.c_char_p, _P(Mjd), _P(Mjd)]),
+    "asarf_con_process_gains": (_ci, [ctypes.c_char_p, ctypes.c_long, _vp, _vp]),
+    "asarf_xca_scaling_factors": (_ci, [ctypes.c_char_p, ctypes.c_long, _vp, _vp]),
+    "asarf_calibration_constant": (_ci, [_ci, _ci, ctypes.c_float, ctypes.c_float, _P(ctypes.c_double)]),
+    "asarf_ins_swath_params": (_ci, [ctypes.c_char_p, ctypes.c_long, _ci, _P(InsSwath)]),
+    "asarf_radar_timing": (_ci, [_P(InsSwath), _ci, _ci, _ci, _ci, _P(Timing)]),
     "asarf_mjd_compare": (_ci, [_P(Mjd), _P(Mjd)]),
     "asarf_mjd_add_micros": (_ci, [_P(Mjd), ctypes.c_longlong, _P(Mjd)]),
     "asarf_counter_gap": (ctypes.c_ulonglong, [ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_ulonglong]),
@@ -226,6 +299,7 @@ SIGNATURES = {
     "asarf_plan_device_bytes": (_sz, [_vp]),
     "asarf_plan_execute": (_ci, [_vp, _vp, _P(Image), _vp]),
     "asarf_plan_execute_lines": (_ci, [_vp, _vp, _vp, _ci, _P(Image), _vp]),
+    "asarf_plan_execute_fbaq4": (_ci, [_vp, _vp, _vp, _ci, _vp, _vp, _ci, _P(Image), _vp]),
     "asarf_plan_imp_capacity": (_ci, [_vp, _P(_ci), _P(_ci)]),
     "asarf_plan_results": (_ci, [_vp, _P(FocusResults)]),
     "asarf_plan_set_timing": (_ci, [_vp, _ci]),

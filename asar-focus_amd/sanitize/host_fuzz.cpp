@@ -3,7 +3,9 @@
 // PATC and auxiliary files) is fed truncated, random and adversarial inputs from a seeded generator, and the
 // geometry functions degenerate orbits and sizes. An entry point may reject an input with an error status; a read
 // or write outside a buffer, an overflow the sanitizers catch, or a crash ends the run with a report.
+#include <algorithm>
 #include <cstdarg>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -218,6 +220,117 @@ void fuzz_small_files(int iters) {
     }
 }
 
+// DORIS orbit files of the ENVISAT size: well-formed records (one minute apart) with random bytes flipped, or
+// random text; CON, XCA and INS files of the right and wrong sizes; DORIS file names; radar timing parameters.
+void fuzz_aux_files(int iters) {
+    static const char* months[12] = {"JAN", "FEB", "MAR", "APR", "MAY", "JUN", "JUL", "AUG", "SEP", "OCT", "NOV", "DEC"};
+    for (int it = 0; it < iters; ++it) {
+        std::vector<unsigned char> v(206606, ' ');
+        const size_t base = 1247 + 378;
+        for (int k = 0; k < 1589; ++k) {
+            char line[160];
+            const int h = (k / 60) % 24, m = k % 60;
+            int n = std::snprintf(line, sizeof(line), "%02d-%s-2004 %02d:%02d:00.000000 -0.4 %05d %.3f %.3f %.3f %.6f "
+                                  "%.6f %.6f 0", 1 + (k / 1440) % 28, months[(it + k / 40000) % 12], h, m, 9000 + k,
+                                  7.1e6, 1.0e3 * k, -2.0e3, 7450.0, -1.0, 0.5);
+            std::string l(line, (size_t)n);
+            if (l.size() < 128) l.insert(11, 128 - l.size(), ' ');
+            l.resize(128);
+            std::memcpy(v.data() + base + 129 * (size_t)k, l.data(), 128);
+            v[base + 129 * (size_t)k + 128] = '\n';
+        }
+        const int flips = (int)(rng() % 40);
+        for (int f = 0; f < flips; ++f) v[rng() % v.size()] = (unsigned char)(rng() & 0xFF);
+        if (rng() % 8 == 0) v = random_bytes(rng() % 2 ? 206606 : 233954);
+        Exact p(v);
+        asarf_mjd a{1461 + (int)(rng() % 3), (unsigned)(rng() % 86400), 0}, b = a;
+        b.seconds = (unsigned)std::min<unsigned long long>(86399, a.seconds + rng() % 600);
+        std::vector<asarf_osv> osv(64);
+        std::vector<asarf_mjd> times(64);
+        int n_out = 0;
+        asarf_doris_meta meta;
+        (void)asarf_doris_orbit(p.p, (long)p.n, &a, &b, osv.data(), times.data(), (int)(rng() % 65), &n_out, &meta);
+        std::string name = "DOR_VOR_AXVF-P20120424_120200_20040101_215528_20040103_002328";
+        for (int f = 0; f < (int)(rng() % 4); ++f) name[rng() % name.size()] = (char)(32 + rng() % 95);
+        if (rng() % 4 == 0) name.resize(rng() % 70);
+        asarf_mjd w0, w1;
+        (void)asarf_doris_filename_window(name.c_str(), &w0, &w1);
+        float g1[7], g2[7];
+        auto con = random_bytes(rng() % 2 ? 5721 : (size_t)pick_len(5721));
+        Exact c(con);
+        (void)asarf_con_process_gains(c.p, (long)c.n, g1, g2);
+        auto xca = random_bytes(rng() % 2 ? 28177 : (size_t)pick_len(28177));
+        Exact x(xca);
+        (void)asarf_xca_scaling_factors(x.p, (long)x.n, g1, g2);
+        double k = 0;
+        (void)asarf_calibration_constant((int)(rng() & 1), (int)(rng() & 1), g1[0], g2[0], &k);
+        auto ins = random_bytes(rng() % 2 ? 173273 : (size_t)pick_len(173273));
+        Exact q(ins);
+        asarf_ins_swath sw{};
+        if (asarf_ins_swath_params(q.p, (long)q.n, (int)(rng() % 9) - 1, &sw) == ASARF_OK) {
+            asarf_timing t;
+            (void)asarf_radar_timing(&sw, (int)(rng() & 1), (int)(rng() % 5000) - 10, (int)(rng() % 2000) - 5,
+                                     (int)(rng() % 6), &t);
+        }
+    }
+}
+
+// The level-1 header writer on random metadata: product names, strings of every length, sizes, times and orbits
+// (values that do not fit their header fields must come back as errors).
+void fuzz_l1_header(int iters) {
+    std::vector<unsigned char> out((size_t)asarf_l1_header_size(1) + 64);
+    for (int it = 0; it < iters; ++it) {
+        const int n_osv = 2 + (int)(rng() % 10);
+        std::vector<asarf_osv> osv((size_t)n_osv);
+        for (int k = 0; k < n_osv; ++k) {
+            const double a = 1e-3 * k;
+            osv[(size_t)k] = asarf_osv{1.4e8 + 10.0 * k, {7.1e6 * std::cos(a), 7.1e6 * std::sin(a), 1e5},
+                                       {-7.4e3 * std::sin(a), 7.4e3 * std::cos(a), 10.0}};
+        }
+        asarf_geometry g{};
+        g.osv = osv.data();
+        g.n_osv = n_osv;
+        g.first_line_time = 1.4e8 + (double)(rng() % 30);
+        g.line_time_interval = 1.0 / (1000.0 + (double)(rng() % 1000));
+        g.azimuth_size = (int)(rng() % 5000);
+        g.range_size = (int)(rng() % 6000);
+        g.slant_range_first = 8e5 + (double)(rng() % 100000);
+        g.range_spacing = (rng() % 2) ? 7.8 : 12.5;
+        g.carrier_frequency = 5.3e9;
+        g.prf = 1652.0;
+        g.az_bandwidth_fraction = 0.8;
+        g.vr_poly[2] = 7000.0;
+        g.center_point[0] = 6.37e6;
+        auto rand_str = [&](size_t maxlen) {
+            std::string s((size_t)(rng() % (maxlen + 1)), 'A');
+            for (auto& c : s) c = (char)(32 + rng() % 95);
+            return s;
+        };
+        const std::string name = rand_str(80), station = rand_str(40), swath = rand_str(4), comp = rand_str(6);
+        asarf_doris_meta meta{};
+        meta.phase = 'A';
+        std::strcpy(meta.vector_source, "DP");
+        meta.pos[0] = (double)(int64_t)(rng() % 200000000) - 1e8;
+        meta.delta_ut1 = ((double)(rng() % 2000) - 1000.0) * 1e-3;
+        double grsr[5] = {0, 0, 0, 1.0, 0};
+        asarf_l1_inputs in{};
+        in.l0_product_name = name.c_str();
+        in.detected = (int)(rng() & 1);
+        in.acquisition_station = station.c_str();
+        in.orbit = (rng() % 4) ? &meta : nullptr;
+        in.swath = swath.c_str();
+        in.echo_comp_method = comp.c_str();
+        in.geometry = &g;
+        in.range_sampling_rate = (rng() % 8) ? 19.2e6 : 0.0;
+        in.grsr5 = (rng() % 4) ? grsr : nullptr;
+        in.up_code = (unsigned short)(rng() % 20);
+        in.mds_records = g.azimuth_size;
+        in.mds_record_size = 17 + g.range_size * (in.detected ? 2 : 4) + (int)(rng() % 2);
+        long n = 0;
+        (void)asarf_l1_header(&in, out.data(), (long)out.size() - (long)(rng() % 64), &n);
+    }
+}
+
 void fuzz_geometry(int iters) {
     for (int it = 0; it < iters; ++it) {
         const int n = (int)(rng() % 7);
@@ -264,6 +377,8 @@ int main(int argc, char** argv) {
     fuzz_headers(2000 * scale);
     fuzz_forecasts(300 * scale);
     fuzz_small_files(100 * scale);
+    fuzz_aux_files(40 * scale);
+    fuzz_l1_header(200 * scale);
     fuzz_geometry(60 * scale);
     std::printf("host_fuzz: done\n");
     return 0;

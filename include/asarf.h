@@ -41,8 +41,12 @@ enum { ASARF_CF32 = 0, ASARF_F32 = 1 };                     /* image element typ
 /* raw sample formats (I,Q interleaved). ASARF_RAW_ERS_LINES: ERS IM level-0 lines as the parser leaves them
  * (asarf_parse_ers_im_packet): samples_per_line u8 (I, Q) pairs per line, line_bytes apart, each line placed
  * swst_mult x (SWST code - minimum) range samples to the right, as ConvertErsImSamplesToComplex does
- * (envisat_format_kernels.cu:116-133); range samples no line sample lands on are NaN. */
-enum { ASARF_RAW_CF32 = 0, ASARF_RAW_U8 = 1, ASARF_RAW_I8 = 2, ASARF_RAW_ERS_LINES = 3 };
+ * (envisat_format_kernels.cu:116-133); range samples no line sample lands on are NaN. ASARF_RAW_ASAR_FBAQ4: ASAR IM
+ * FBAQ-4 records as asarf_parse_asar_im_packet writes them (2-byte little-endian length, then blocks of a
+ * quantiser id and 63 code bytes), line_bytes apart, decoded through the reconstruction tables and placed
+ * swst_mult x (SWST code - minimum) range samples to the right, as ConvertAsarImBlocksToComplex does
+ * (envisat_format_kernels.cu:160-210). */
+enum { ASARF_RAW_CF32 = 0, ASARF_RAW_U8 = 1, ASARF_RAW_I8 = 2, ASARF_RAW_ERS_LINES = 3, ASARF_RAW_ASAR_FBAQ4 = 4 };
 enum { ASARF_FORWARD = 0, ASARF_INVERSE = 1 };
 enum { ASARF_FFT_ROWS = 0, ASARF_FFT_COLUMNS = 1 };
 
@@ -320,7 +324,7 @@ int asarf_fft_c2c(asarf_ctx* ctx, void* data, int n, int batch, long dist, int l
 typedef struct {
     int na, nr, nc;          /* azimuth lines, range samples, chirp samples */
     int nrp, nap;            /* FFT sizes, 0 -> reference padding rule */
-    int raw_format;          /* ASARF_RAW_U8, ASARF_RAW_I8 or ASARF_RAW_ERS_LINES */
+    int raw_format;          /* ASARF_RAW_U8, ASARF_RAW_I8, ASARF_RAW_ERS_LINES or ASARF_RAW_ASAR_FBAQ4 */
     double prf, wavelength, range_spacing, slant_range_first, az_bandwidth_fraction;
     double vr_poly[3];
     size_t n_total_samples;  /* 0 -> na*nr */
@@ -332,8 +336,9 @@ typedef struct {
     int gr_range_size;       /* ground-range output samples */
     double ground_spacing;   /* 12.5 m */
     int lines_before, lines_after; /* extra lines around the requested sensing window (main.cc:137-142) */
-    /* ASARF_RAW_ERS_LINES only: I/Q pairs per line (nr = samples_per_line + swst_mult x (max - min SWST code)),
-     * bytes from one line to the next, range samples per SWST code step (4 for ERS IM) */
+    /* ASARF_RAW_ERS_LINES: I/Q pairs per line (nr = samples_per_line + swst_mult x (max - min SWST code)), bytes
+     * from one line to the next, range samples per SWST code step (4 for ERS IM). ASARF_RAW_ASAR_FBAQ4: line_bytes
+     * is the record pitch and swst_mult 1 (samples_per_line unused). */
     int samples_per_line;
     long line_bytes;
     int swst_mult;
@@ -370,6 +375,13 @@ int asarf_plan_execute(asarf_plan* plan, const void* raw, asarf_image* slc, floa
  * main.cc:233-239 happen in the statistics and range-compression loads, with no intermediate image. */
 int asarf_plan_execute_lines(asarf_plan* plan, const void* lines, const short* swst, int min_swst, asarf_image* slc,
                              float* imp_out);
+/* asarf_plan_execute for a plan of ASARF_RAW_ASAR_FBAQ4: records (device) as described there, swst (device, na int16
+ * codes) and their minimum, i_lut / q_lut (device, the instrument file's 4096-entry FBAQ-4 tables, see
+ * asarf_ins_fbaq4_tables). ConvertAsarImBlocksToComplex with the NaN fill and padded copy of main_flow.cc:530-583
+ * and main.cc:233-239, fused into the statistics (FP64 sums: the samples are table values) and range-compression
+ * loads, with no intermediate image. */
+int asarf_plan_execute_fbaq4(asarf_plan* plan, const void* records, const short* swst, int min_swst, const float* i_lut,
+                             const float* q_lut, int lut_items, asarf_image* slc, float* imp_out);
 int asarf_plan_imp_capacity(const asarf_plan* plan, int* rows, int* cols);
 /* Copies the estimator outputs to the host (synchronises). */
 int asarf_plan_results(asarf_plan* plan, asarf_focus_results* results);
@@ -572,6 +584,121 @@ typedef struct {
  * complete one of the four at the end of the SPH). Numbers are read as the reference reads them: from the first
  * digit 1-9 after '=', so a sign is dropped ("-0000000001" reads as 1). */
 int asarf_level0_mdsr(const unsigned char* file, long n_bytes, asarf_dsd* out);
+
+/* ------------------------------------------------------------------------- auxiliary files (host) */
+/* L1ProductMetadata of an orbit source (doris_orbit.h, filled by doris_orbit.cc:260-304). */
+typedef struct {
+    char orbit_name[64];       /* MPH PRODUCT */
+    char vector_source[3];     /* "DP" (DORIS PRECISE), "RO" (REAPER ORBIT) or "??", from SPH DS_NAME */
+    char phase;                /* first character of MPH PHASE */
+    unsigned cycle;            /* MPH CYCLE */
+    double delta_ut1;          /* s */
+    unsigned abs_orbit;
+    unsigned quality;
+    asarf_mjd state_vector_time;
+    double pos[3], vel[3];     /* m, m/s */
+} asarf_doris_meta;
+/* Parsable::CreateFromFile + CreateOrbitInfo (doris_orbit.cc:82-176): the state vectors of a DOR_VOR_AX file (its
+ * bytes, 206606 for ENVISAT or 233954 for ERS) from 5 minutes before start to 5 minutes after stop, as asarf_osv
+ * with t in seconds since 2000-01-01 (and, times != NULL, the exact times); fewer than 10 is an error (a corrupt
+ * file for the window). *meta (may be NULL): the L1 metadata of doris_orbit.cc:260-304. ASARF_EWORKSPACE when more
+ * than capacity vectors are kept (*n_out stays 0). */
+int asarf_doris_orbit(const unsigned char* file, long n_bytes, const asarf_mjd* start, const asarf_mjd* stop,
+                      asarf_osv* osv, asarf_mjd* times, int capacity, int* n_out, asarf_doris_meta* meta);
+/* The validity window in a DORIS orbit file name (doris_orbit.cc:178-246): DOR_VOR_AXVF-P<created>_<hhmmss>_
+ * <start ymd>_<hhmmss>_<stop ymd>_<hhmmss>, or the ERS REAPER form with one more HHMMSS item. ASARF_EINVAL when the
+ * name is not such a file name (the reference skips those). */
+int asarf_doris_filename_window(const char* name, asarf_mjd* valid_start, asarf_mjd* valid_stop);
+/* FindCONFile (envisat_aux_file.cc:41-65): process_gain_ims[7] and process_gain_imp[7] (swaths IS1-IS7) of a
+ * 5721-byte ASAR processor configuration file. */
+int asarf_con_process_gains(const unsigned char* con, long n_bytes, float* gain_ims, float* gain_imp);
+/* GetXca (envisat_aux_file.cc:99-131): the image-mode VV scaling factors (SLC and precision, swaths IS1-IS7) of a
+ * 28177-byte external calibration file; ASARF_EINVAL for a GADS DSR length other than 26552 or 0. */
+int asarf_xca_scaling_factors(const unsigned char* xca, long n_bytes, float* slc_vv, float* precision_vv);
+/* The product calibration constant of main.cc:336-337 (detected: gain_imp / sqrt(precision_vv)) and main.cc:384-388
+ * (SLC: gain_ims / sqrt(slc_vv), / 8 for ERS), in FP64; it scales the statistics and the conditioned records. */
+int asarf_calibration_constant(int detected, int ers, float process_gain, float scaling_factor, double* out);
+/* What the level-0 flow reads of an instrument characterisation file for one image-mode swath (main_flow.cc:299-300,
+ * 389-434): radar frequency and sampling rate, the nominal chirp, the range gate bias and the PRI count before the
+ * sampling window (mode_timelines[0].r_values[swath]). */
+typedef struct {
+    float radar_frequency, radar_sampling_rate, range_gate_bias;
+    float chirp_amplitude[4], chirp_phase[4], chirp_duration;
+    unsigned short swst_rank;
+} asarf_ins_swath;
+int asarf_ins_swath_params(const unsigned char* ins, long n_bytes, int swath_idx, asarf_ins_swath* out);
+/* The radar timing of main_flow.cc:389-460 (no range recalibration): chirp rate, duration, samples and bandwidth;
+ * PRF and line time; two-way slant range time and slant range to the first sample; wavelength and range spacing. */
+typedef struct {
+    double prf, line_time_interval, two_way_slant_range_time, slant_range_first, wavelength, range_spacing;
+    double chirp_kr, chirp_duration, chirp_bandwidth;
+    int chirp_samples;
+} asarf_timing;
+int asarf_radar_timing(const asarf_ins_swath* ins, int ers, int pri_code, int min_swst, int swst_mult,
+                       asarf_timing* out);
+
+/* Summary-quality thresholds and expected values of the processor configuration file (envisat_lvl0_parser.cc:211-233). */
+typedef struct {
+    float thresh_chirp_broadening, thresh_chirp_sidelobe, thresh_chirp_islr;
+    float thresh_input_mean, exp_input_mean, thresh_input_std_dev, exp_input_std_dev;
+    float thresh_dop_cen, thresh_dop_amb;
+    float thresh_output_mean, exp_output_mean, thresh_output_std_dev, exp_output_std_dev;
+    float thresh_input_missing_lines, thresh_input_gaps;
+    unsigned lines_per_gap;
+} asarf_sq_thresholds;
+/* The thresholds of a 5721-byte configuration file; the expected output mean / std dev are the IMP ones when
+ * detected, else the IMS ones. */
+int asarf_con_summary_quality(const unsigned char* con, long n_bytes, int detected, asarf_sq_thresholds* out);
+
+/* ----------------------------------------------------------------------------------- level-1 header (host) */
+/* What ConstructEnvisatFileHeader (envisat_lvl1_writer.cc:326-570) reads of the product's metadata. Times are mjd
+ * (days since 2000-01-01); geometry->first_line_time and the orbit times are seconds since 2000-01-01, the same
+ * instants as first_line_time. For a detected product the geometry describes the ground-range image (range_size
+ * ground samples, range_spacing the ground spacing) and grsr5 is its slant-to-ground polynomial. */
+typedef struct {
+    const char* l0_product_name;   /* the level-0 product's name: characters 7 and 9 become S / P and 1 */
+    int detected;                  /* 1: precision image (IMP, uint16 records), 0: SLC (IMS, int16 I/Q records) */
+    const char* acquisition_station, *processing_station, *software_version;
+    asarf_mjd processing_time;     /* the reference takes the wall clock; here the caller's, for repeatable files */
+    asarf_mjd sensing_start, sensing_stop, first_line_time, last_line_time;
+    const asarf_doris_meta* orbit; /* MPH orbit fields (NULL: blank) */
+    int rel_orbit;
+    asarf_mjd utc_sbt_time;
+    unsigned sat_binary_time, clock_step;
+    asarf_mjd leap_utc;
+    int leap_sign;
+    char leap_err;
+    int product_err;
+    const char* swath;             /* "IS1" .. "IS7" */
+    int ascending;
+    const char* polarization;      /* "V/V", ... */
+    const char* echo_comp_method, *echo_comp_ratio, *init_cal_method, *init_cal_ratio, *per_cal_method,
+        *per_cal_ratio, *noise_method, *noise_ratio;  /* NULL: "NONE" / blank */
+    const char* configuration_file, *instrument_file, *external_characterization_file, *external_calibration_file,
+        *orbit_dataset_name;       /* referenced auxiliary files ("" or NULL characterization file: not used) */
+    const asarf_geometry* geometry;
+    double azimuth_spacing, two_way_slant_range_time, range_sampling_rate;
+    int chirp_window;              /* the range chirp was Hamming windowed */
+    double chirp_alpha, chirp_kr, azimuth_window_alpha;
+    const double* grsr5;           /* detected products: ground-to-slant polynomial (NULL for SLC) */
+    unsigned first_sbt;
+    asarf_mjd first_mjd;
+    unsigned short first_swst_code, last_swst_code, pri_code, tx_pulse_len_code, tx_bw_code, echo_win_len_code,
+        up_code, down_code, resamp_code, beam_adj_code, beam_set_num_code, tx_monitor_code;
+    unsigned swst_changes, swst_rank;
+    float tx_monitor_value;        /* the converted auxiliary TX monitor level (the reference's 260-entry table) */
+    asarf_iq_results iq;
+    double statistics[4];          /* output mean I, Q (or amplitude), std dev I, Q, calibrated */
+    asarf_sq_thresholds sq;
+    int mds_records, mds_record_size;
+} asarf_l1_inputs;
+/* Bytes of the header (MPH through the geolocation grid) for an SLC (0) or detected (1) product. */
+long asarf_l1_header_size(int detected);
+/* ConstructEnvisatFileHeader: MPH (1247 bytes), SPH with 18 DSDs (6099), summary quality ADS (170), main processing
+ * parameters ADS (2009), Doppler centroid ADS (55), [slant-to-ground ADS (55)], 12 geolocation grid ADSRs (521 each),
+ * as the bytes before the MDS records. ASARF_EWORKSPACE when capacity is short; ASARF_EINVAL when a value does not
+ * fit its header field. */
+int asarf_l1_header(const asarf_l1_inputs* in, unsigned char* out, long capacity, long* n_out);
 
 /* FetchEnvisatL0ImForecastMeta (envisat_im_parse.cc:297-384): the ASAR IM records to focus between the sensing
  * times start and stop. *before_start / *after_stop: in, the lines allowed before the start and after the stop;
