@@ -232,6 +232,8 @@ SIGNATURES = {
     "asarf_image_release": (_ci, [_P(Image), _P(Workspace)]),
     "asarf_image_subset_copy": (_ci, [_vp, _P(Image), _ci, _ci, _ci, _ci, _vp, _sz]),
     "asarf_fft_padding_size": (_ci, [_ci]),
+    "asarf_fft_static_column_length": (_ci, [_ci]),
+    "asarf_fft_fast_range_size": (_ci, [_ci]),
     "asarf_generate_chirp": (_ci, [_P(ChirpInfo), _ci, _vp]),
     "asarf_polyfit": (_ci, [_vp, _vp, _ci, _ci, _vp]),
     "asarf_polyval": (_cd, [_vp, _ci, _cd]),

@@ -301,6 +301,17 @@ def padding_size(n: int) -> int:
     return L.lib().asarf_fft_padding_size(int(n))
 
 
+def fast_range_size(n: int) -> int:
+    """A range transform length for n = nr + nc with a compile-time range kernel (same lines up to FP32 rounding
+    as the reference's padding_size(n)); padding_size(n) when none is within 1.5 times it."""
+    return L.lib().asarf_fft_fast_range_size(int(n))
+
+
+def static_column_length(n: int) -> bool:
+    """True when azimuth sub-transforms of length n run a compile-time column kernel."""
+    return bool(L.lib().asarf_fft_static_column_length(int(n)))
+
+
 def subset_copy(ctx: Context, img: L.Image, x0: int, y0: int, w: int, h: int, dst: torch.Tensor):
     eb = 8 if img.dtype == L.CF32 else 4
     L.check(L.lib().asarf_image_subset_copy(ctx.h, ctypes.byref(img), x0, y0, w, h, dst.data_ptr(),

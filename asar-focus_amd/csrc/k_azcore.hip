@@ -11,7 +11,7 @@
 // The per-pixel arithmetic is k_rcmc_tile's (k_rda.hip) and the transforms are the Stockham passes of the column
 // kernels, so the core reproduces the separate kernels up to the rounding of a different N1 x N2 split.
 //
-// With N2 = 84 the tile (84 x 89 complex samples) and the twiddle tables fit the 64 KB a workgroup may allocate.
+// With N2 = 84 the tile (84 x 89 complex samples) and the twiddle tables take about 60 KB of LDS.
 // Phases (512 lanes, 16 per column transform, 32 columns per batch):
 //   1. three batches (the last 24 columns wide): load 32 staged columns of the group's N2 rows, forward
 //      transform (LDS exchange through the batch's own tile columns), bins left in the tile;
@@ -195,17 +195,44 @@ static hipError_t core_launch(const AzCoreArgs& a, size_t lds, bool prepare, hip
     return hipGetLastError();
 }
 
-// Second-pass lengths with a compiled core (the split of the azimuth FFT picks one of them).
-bool az_core_supported(int n2) { return n2 == 84 || n2 == 90 || n2 == 96 || n2 == 98; }
+// Second-pass lengths with a compiled core and their radix sequences, in the planner's order of preference (the
+// split of the azimuth FFT takes the first whose first-pass length has a compile-time column kernel). 84, 90, 96
+// and 98 were measured on the production scenes; the others cover the azimuth lengths those four do not divide
+// (powers of two, 3^a 5^b, ...). With 128 the tile takes about 92 KB of LDS (one workgroup per CU).
+#define ASARF_AZ_CORE_LENGTHS(X)                                                                                  \
+    X(84, 12, 7) X(90, 9, 10) X(98, 7, 14) X(96, 8, 12) X(80, 10, 8) X(100, 10, 10) X(72, 9, 8) X(105, 15, 7)     \
+    X(81, 9, 9) X(112, 16, 7) X(64, 8, 8) X(120, 12, 10) X(75, 15, 5) X(126, 14, 9) X(128, 16, 8) X(125, 25, 5)
+
+bool az_core_supported(int n2) {
+    switch (n2) {
+#define ASARF_AZ_LABEL(N, ...) case N:
+        ASARF_AZ_CORE_LENGTHS(ASARF_AZ_LABEL)
+#undef ASARF_AZ_LABEL
+            return true;
+        default:
+            return false;
+    }
+}
+
+int az_core_lengths(int* out, int capacity) {
+    static const int lengths[] = {
+#define ASARF_AZ_ITEM(N, ...) N,
+        ASARF_AZ_CORE_LENGTHS(ASARF_AZ_ITEM)
+#undef ASARF_AZ_ITEM
+    };
+    const int n = (int)(sizeof(lengths) / sizeof(lengths[0]));
+    for (int i = 0; i < n && i < capacity; ++i) out[i] = lengths[i];
+    return n;
+}
 
 static hipError_t core_dispatch(const AzCoreArgs& a, int n2, bool prepare, hipStream_t s) {
     const size_t lds = az_core_lds_bytes(n2, a.nbins);
     if (lds > max_lds_bytes()) return hipErrorInvalidValue;
     switch (n2) {
-        case 84: return core_launch<84, RadixSeq<12, 7>>(a, lds, prepare, s);
-        case 90: return core_launch<90, RadixSeq<9, 10>>(a, lds, prepare, s);
-        case 96: return core_launch<96, RadixSeq<8, 12>>(a, lds, prepare, s);
-        case 98: return core_launch<98, RadixSeq<7, 14>>(a, lds, prepare, s);
+#define ASARF_AZ_LAUNCH(N, ...) \
+    case N: return core_launch<N, RadixSeq<__VA_ARGS__>>(a, lds, prepare, s);
+        ASARF_AZ_CORE_LENGTHS(ASARF_AZ_LAUNCH)
+#undef ASARF_AZ_LAUNCH
         default: return hipErrorInvalidValue;
     }
 }

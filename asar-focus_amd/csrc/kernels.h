@@ -343,6 +343,7 @@ hipError_t launch_rcmc_tile(const RcmcArgs& a, hipStream_t s);
 size_t az_core_lds_bytes(int n2, int nbins);
 int az_core_tile_columns();
 bool az_core_supported(int n2);
+int az_core_lengths(int* out, int capacity);  // the compiled n2 in order of preference; returns their count
 hipError_t az_core_prepare(int n2, int nbins);
 hipError_t launch_az_core(const AzCoreArgs& a, int n2, hipStream_t s);
 void rcmc_window(float w[16]);

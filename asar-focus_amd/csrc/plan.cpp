@@ -179,15 +179,26 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     // the balanced split). Whether the core or the separate kernels run is decided on the device by the RCMC
     // shift bound (the core's range halo covers shifts up to kAzCoreReach pixels).
     const char* fz_env = std::getenv("ASARF_AZ_FUSED");
+    // The compiled second-pass lengths are tried in their order of preference, first those whose first-pass
+    // length n1 = nap / n2 has a compile-time column kernel, then any. ASARF_AZ_CORE_N2=<n2> tries only that one.
     if (!(fz_env && fz_env[0] == '0') && pl->cs.n2 > 1) {
-        for (int n2 : {84, 90, 98, 96}) {
-            ColSplit alt;
-            if (nap % n2 != 0 || get_col_split(c, nap, &alt, n2) != ASARF_OK || alt.n2 != n2) continue;
-            if (!az_core_supported(n2) || az_core_lds_bytes(n2, nap) > c->max_lds) continue;
-            if (az_core_prepare(n2, nap) != hipSuccess) continue;
-            pl->cs = alt;
-            pl->core = true;
-            break;
+        int lens[32];
+        const int nl = std::min(az_core_lengths(lens, 32), 32);
+        const char* n2_env = std::getenv("ASARF_AZ_CORE_N2");
+        const int n2_only = n2_env ? std::atoi(n2_env) : 0;
+        for (int round = 0; round < 2 && !pl->core; ++round) {
+            for (int i = 0; i < nl; ++i) {
+                const int n2 = lens[i];
+                if (nap % n2 != 0 || (n2_only > 0 && n2 != n2_only)) continue;
+                if (round == 0 && !(c->col_static && col_static_length(nap / n2))) continue;
+                ColSplit alt;
+                if (get_col_split(c, nap, &alt, n2) != ASARF_OK || alt.n2 != n2) continue;
+                if (!az_core_supported(n2) || az_core_lds_bytes(n2, nap) > c->max_lds) continue;
+                if (az_core_prepare(n2, nap) != hipSuccess) continue;
+                pl->cs = alt;
+                pl->core = true;
+                break;
+            }
         }
         clear_error();
     }

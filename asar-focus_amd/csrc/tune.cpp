@@ -402,3 +402,13 @@ extern "C" int asarf_tune_load(asarf_ctx* c, const char* path, int* n_loaded) {
     if (n_loaded) *n_loaded = (int)(rows.size() + cols.size());
     return clear_error();
 }
+
+extern "C" int asarf_fft_static_column_length(int n) { return col_static_length(n) ? 1 : 0; }
+
+extern "C" int asarf_fft_fast_range_size(int n) {
+    const int pad = asarf_fft_padding_size(n);
+    if (pad <= 0) return pad;
+    for (int len : {6144, 6400, 8192})
+        if (len >= pad && 2L * len <= 3L * pad && range_compress_static_variants(len) > 0) return len;
+    return pad;
+}

@@ -414,18 +414,26 @@ def run_batch(args):
     from asarf import configs, focus, scene
 
     cfgs = configs.batch_blocks(64)
+    if args.range_pad == "fast":
+        # range lines at the compile-time transform lengths (the same lines up to FP32 rounding)
+        cfgs = [c.scaled(nrp=focus.fast_range_size(c.nr + c.nc)) for c in cfgs]
     t0 = time.perf_counter()
     raws = [torch.from_numpy(scene.generate(c)[0]).cuda() for c in cfgs]
     gen_s = time.perf_counter() - t0
     ctx = focus.Context(0)
     t1 = time.perf_counter()
-    rows = sorted({focus.padding_size(c.nr + c.nc) for c in cfgs})
+    rows = sorted({c.nrp or focus.padding_size(c.nr + c.nc) for c in cfgs})
     cols = sorted({focus.padding_size(c.na) for c in cfgs})
     tuned = {"rows": {}, "columns": {}}
-    for n in rows:
-        tuned["rows"][n] = focus.autotune(ctx, n, columns=False, batch=512)["choice"]
-    for n in cols:
-        tuned["columns"][n] = focus.autotune(ctx, n, columns=True, batch=1024)["choice"]
+    if args.tune_file and os.path.exists(args.tune_file):
+        tuned = {"loaded": focus.tune_load(ctx, args.tune_file), "file": args.tune_file}
+    else:
+        for n in rows:
+            tuned["rows"][n] = focus.autotune(ctx, n, columns=False, batch=512)["choice"]
+        for n in cols:
+            tuned["columns"][n] = focus.autotune(ctx, n, columns=True, batch=1024)["choice"]
+        if args.tune_file:
+            focus.tune_save(ctx, args.tune_file)
     tune_s = time.perf_counter() - t1
     b = focus.Batch(ctx, n_streams=4)
     for c, r in zip(cfgs, raws):
@@ -464,7 +472,9 @@ def run_batch(args):
         "data": "synthetic (seeded Philox blocks: point targets + 10 dB noise)",
         "config": {"workload": "batch64_ers_asar_blocks", "blocks": len(cfgs), "raw_samples": samples,
                    "azimuth_lines": [c.na for c in cfgs], "range_samples": [c.nr for c in cfgs],
-                   "streams": 4, "form": "one captured graph (fork-join over 4 streams)"},
+                   "streams": 4, "form": "one captured graph (fork-join over 4 streams)",
+                   "range_transform": ("compile-time lengths >= the reference padding (asarf_fft_fast_range_size)"
+                                       if args.range_pad == "fast" else "reference padding")},
         "sequential_msamples_per_s": round(samples / seq_s / 1e6, 2),
         "sequential_ms": round(seq_s * 1e3, 3),
         "graph_speedup": round(seq_s / graph_s, 3),
@@ -504,6 +514,12 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the forms object (split-bf16 and learned RCMC against the plain path on their own "
                          "workloads, about half a minute)")
+    ap.add_argument("--tune-file", default=None,
+                    help="batch64: autotuner cache file; loaded when it exists (no tuning launches), else written "
+                         "after tuning")
+    ap.add_argument("--range-pad", choices=["fast", "reference"], default="fast",
+                    help="batch64: range transform lengths (fast: the compile-time length at most 1.5 times the "
+                         "reference's padding, same lines up to FP32 rounding)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (5 times): the known-size reference that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of the plan's kernels")

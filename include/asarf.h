@@ -415,6 +415,16 @@ int asarf_autotune_fft(asarf_ctx* ctx, int n, int layout, int batch, int reps, i
  * ASARF_EINVAL and changes nothing (a malformed file is rejected before any device call). */
 int asarf_tune_save(asarf_ctx* ctx, const char* path);
 int asarf_tune_load(asarf_ctx* ctx, const char* path, int* n_loaded);
+/* Transform lengths with compile-time kernels (no counterpart in the reference). 1 when azimuth sub-transforms of
+ * length n (a single column pass, or either pass of a two-pass split) run a compile-time kernel: every
+ * 2^a 3^b 5^c 7^d from 16 to 400; else 0. */
+int asarf_fft_static_column_length(int n);
+/* A range transform length for nr + nc samples (pass as asarf_focus_params.nrp): the smallest length with a
+ * compile-time range-compression kernel (6144, 6400, 8192) that is >= asarf_fft_padding_size(n) and at most 1.5
+ * times it, else asarf_fft_padding_size(n). The range-compressed lines are the same as with the reference's
+ * padding up to FP32 rounding: the transform is a zero-padded linear correlation, and the kept samples never see
+ * the wrap. The azimuth length is not open to this (its padding decides which lines wrap into the aperture). */
+int asarf_fft_fast_range_size(int n);
 
 /* Many blocks of uneven size in one batch: each block gets its own plan on one of n_streams streams (round
  * robin; each stream has its own context that inherits ctx's autotuner choices). SEQUENTIAL runs the blocks

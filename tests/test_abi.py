@@ -167,3 +167,26 @@ def test_memory_model_trading_without_device():
     L.check(lib.asarf_image_adopt(ctypes.byref(out), 100, 50, 128, 64, L.CF32, ctypes.byref(ws)))
     assert out.data == 0x4000 and ws.data is None and ws.bytes == 0
     assert lib.asarf_image_adopt(ctypes.byref(out), 1, 1, 1, 1, L.CF32, ctypes.byref(ws)) == L.EINVAL
+
+
+def test_static_length_queries_without_device():
+    """Column lengths with compile-time kernels are exactly the 2^a 3^b 5^c 7^d from 16 to 400; the fast range
+    length is the reference's padding or a compile-time length (6144, 6400, 8192) at most 1.5 times it."""
+    def smooth(n):
+        for p in (2, 3, 5, 7):
+            while n % p == 0:
+                n //= p
+        return n == 1
+
+    lib = L.lib()
+    for n in range(1, 1200):
+        assert lib.asarf_fft_static_column_length(n) == int(16 <= n <= 400 and smooth(n)), n
+    for n in range(100, 9000, 7):
+        pad, fast = lib.asarf_fft_padding_size(n), lib.asarf_fft_fast_range_size(n)
+        assert fast >= pad > n
+        if fast != pad:
+            assert fast in (6144, 6400, 8192) and 2 * fast <= 3 * pad, (n, pad, fast)
+        elif pad not in (6144, 6400, 8192):
+            assert not any(pad <= s <= 1.5 * pad for s in (6144, 6400, 8192)), (n, pad)
+    assert lib.asarf_fft_fast_range_size(5616 + 521) == 6144 and lib.asarf_fft_fast_range_size(4500) == 6144
+    assert lib.asarf_fft_fast_range_size(6200) == 6400 and lib.asarf_fft_fast_range_size(100) == 105
