@@ -12,13 +12,14 @@
 // kernels, so the core reproduces the separate kernels up to the rounding of a different N1 x N2 split.
 //
 // With N2 = 84 the tile (84 x 89 complex samples) and the twiddle tables take about 60 KB of LDS.
-// Phases (512 lanes, 16 per column transform, 32 columns per batch):
-//   1. three batches (the last 24 columns wide): load 32 staged columns of the group's N2 rows, forward
-//      transform (LDS exchange through the batch's own tile columns), bins left in the tile;
+// Phases (512 lanes; L = 4, 8 or 16 lanes per column transform, see core_lanes):
+//   1. forward transforms of the 88 staged columns of the group's N2 rows, packed L lanes per column into
+//      batches of at most 512 / L columns (LDS exchange through each column's own tile column), bins left in the
+//      tile;
 //   2. RCMC + matched filter: lane (wave w, column c) produces bins w, w + 8, ... of column c (a wave reads 64
 //      consecutive tile columns of one bin row); results held in registers until every tap is read, then
 //      written over the produced columns;
-//   3. two batches: inverse transform of 32 produced columns, inverse inter-pass twiddle, store.
+//   3. inverse transforms of the 64 produced columns (same packing), inverse inter-pass twiddle, store.
 #include "interp.h"
 #include "kernels.h"
 
@@ -29,10 +30,89 @@ constexpr int kAcOut = 64;                                    // produced column
 constexpr int kAcLeft = 8;                                    // first produced column inside the tile
 constexpr int kAcCols = kAcLeft + kAcOut + 8 + kAzCoreReach;  // staged columns per tile
 constexpr int kAcPitch = kAcCols + 1;
-constexpr int kAcBatch = 32;                                  // columns per transform batch
-constexpr int kAcLanes = kAcThreads / kAcBatch;               // lanes per column transform
-static_assert(kAcOut % kAcBatch == 0, "whole batches of produced columns");
 static_assert(kAcOut == 64, "phase 2 maps one wave onto the produced columns");
+
+// Lanes per column transform. Fewer lanes than the 16 of the column kernels: a length-N2 pass of radix R has only
+// N2 / R butterflies (7 for the radix-12 pass of 84), so 16 lanes leave most of them idle, and every wave of the
+// workgroup issues each pass. With L lanes the staged columns are packed L lanes each into as few waves as they
+// fill (wave slots past the last column skip the transform), at most 512 / L columns per batch. Chosen per length
+// from {4, 8, 16}: the fewest wave-issued butterfly elements over the forward (kAcCols columns) and inverse (kAcOut)
+// transforms, with at most 32 elements per lane (64 VGPRs of transform state).
+template <int N2, class Seq, int L>
+constexpr long core_issue_cost() {
+    long rounds = 0;
+    for (int p = 0; p < Seq::count; ++p) rounds += (long)((N2 / Seq::at(p) + L - 1) / L) * Seq::at(p);
+    long waves = 0;
+    for (int cols : {kAcCols, kAcOut}) {
+        const int cb = cols < kAcThreads / L ? cols : kAcThreads / L;
+        for (int base = 0; base < cols; base += cb) {
+            const int n = cols - base < cb ? cols - base : cb;
+            waves += (n * L + 63) / 64;
+        }
+    }
+    return waves * rounds;
+}
+
+template <int N2, class Seq>
+constexpr int core_lanes() {
+    int best = 16;
+    long best_cost = core_issue_cost<N2, Seq, 16>();
+    if (seq_max_elems<N2, Seq, 8>() <= 32 && core_issue_cost<N2, Seq, 8>() < best_cost) {
+        best = 8;
+        best_cost = core_issue_cost<N2, Seq, 8>();
+    }
+    if (seq_max_elems<N2, Seq, 4>() <= 32 && core_issue_cost<N2, Seq, 4>() < best_cost) best = 4;
+    return best;
+}
+
+// Length-N2 transforms of the tile columns [C0, C0 + NC) (tile column c at tl[i * kAcPitch + c]), L lanes each,
+// packed in batches of at most 512 / L columns: thread t of a batch of n columns takes column t % n, lane t / n.
+// INV = false: forward, from global memory (load(c, m) = row m of tile column c) into the tile; INV = true:
+// inverse, from the tile into global memory (store(c, n, value) = row n of tile column c). Every thread reaches
+// every barrier; threads past the batch's columns skip the arithmetic.
+template <bool INV, int N2, class Seq, int L, int C0, int NC, class Load, class Store>
+__device__ __forceinline__ void core_transforms(float2* tl, const TwLds& tw2, Load&& load, Store&& store) {
+    constexpr int E = seq_max_elems<N2, Seq, L>();
+    constexpr int CB = NC < kAcThreads / L ? NC : kAcThreads / L;
+    constexpr int NB = (NC + CB - 1) / CB;
+    const int tid = threadIdx.x;
+    float2 v[E];
+    static_for<0, NB>([&](auto bc) __attribute__((always_inline)) {
+        constexpr int base = decltype(bc)::value * CB;
+        constexpr int n = NC - base < CB ? NC - base : CB;
+        const bool act = tid < n * L;
+        const int c = C0 + base + tid % n;
+        const int lane = tid / n;
+        auto lds_store = [&](int i, float2 val) { tl[i * kAcPitch + c] = val; };
+        auto lds_load = [&](int i) { return tl[i * kAcPitch + c]; };
+        auto src_load = [&](int m) { return load(c, m); };
+        constexpr int R0 = INV ? Seq::rat(0) : Seq::at(0);
+        if constexpr (INV) {
+            if (act) pass_load<R0>(v, lane, L, N2, lds_load);
+            __syncthreads();  // every element of the column is read before the first exchange overwrites it
+        } else {
+            if (act) pass_load<R0>(v, lane, L, N2, src_load);
+        }
+        if (act) pass_compute<R0, E, INV>(v, lane, L, N2, 1, tw2);
+        static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int Rp = INV ? Seq::rat(p - 1) : Seq::at(p - 1);
+            constexpr int R = INV ? Seq::rat(p) : Seq::at(p);
+            constexpr int ns_prev = INV ? Seq::rprefix(p - 1) : Seq::prefix(p - 1);
+            constexpr int ns = INV ? Seq::rprefix(p) : Seq::prefix(p);
+            if (act) pass_store<Rp>(v, lane, L, N2, ns_prev, lds_store);
+            __syncthreads();
+            if (act) pass_load<R>(v, lane, L, N2, lds_load);
+            __syncthreads();
+            if (act) pass_compute<R, E, INV>(v, lane, L, N2, ns, tw2);
+        });
+        constexpr int RL = INV ? Seq::rat(Seq::count - 1) : Seq::at(Seq::count - 1);
+        if (act)
+            pass_visit<RL>(v, lane, L, N2, [&](int k, float2& val, bool ok) {
+                if (ok) store(c, k, val);
+            });
+    });
+}
 
 template <int N2, class Seq>
 __global__ __launch_bounds__(kAcThreads) void k_az_core(AzCoreArgs a) {
@@ -50,43 +130,17 @@ __global__ __launch_bounds__(kAcThreads) void k_az_core(AzCoreArgs a) {
     float2* tl = lds;  // [N2][kAcPitch]
     const TwLds tw2 = tw_fill(lds + N2 * kAcPitch, a.tw2, N2, tid, kAcThreads);
     const TwLds twf = tw_fill(lds + N2 * kAcPitch + tw_lds_elems(N2), a.tw_full, a.nbins, tid, kAcThreads);
-    constexpr int E = seq_max_elems<N2, Seq, kAcLanes>();
-    const int b = tid & (kAcBatch - 1);
-    const int lane = tid / kAcBatch;
+    constexpr int L = core_lanes<N2, Seq>();
     const long row0 = (long)k1 * N2;
-    float2 v[E];
 
-    // ---- 1. forward pass B of the staged columns
-    for (int bt = 0; bt < (kAcCols + kAcBatch - 1) / kAcBatch; ++bt) {
-        const int c = bt * kAcBatch + b;
-        const bool cin = c < kAcCols;  // lanes past the tile (last batch) transform zeros and touch no LDS
-        const int x = x0 - kAcLeft + c;
-        const bool xin = cin && x >= 0 && x < a.nrv;  // columns outside the image are zero (never read as taps)
-        const float2* src = a.in + (xin ? x : 0);
-        auto gl_load = [&](int m) -> float2 {
-            return xin ? src[(row0 + m) * a.stride] : make_float2(0.f, 0.f);
-        };
-        auto lds_store = [&](int i, float2 val) {
-            if (cin) tl[i * kAcPitch + c] = val;
-        };
-        auto lds_load = [&](int i) { return cin ? tl[i * kAcPitch + c] : make_float2(0.f, 0.f); };
-        constexpr int R0 = Seq::at(0);
-        pass_load<R0>(v, lane, kAcLanes, N2, gl_load);
-        pass_compute<R0, E, false>(v, lane, kAcLanes, N2, 1, tw2);
-        static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
-            constexpr int p = decltype(pc)::value;
-            constexpr int Rp = Seq::at(p - 1), R = Seq::at(p);
-            pass_store<Rp>(v, lane, kAcLanes, N2, Seq::prefix(p - 1), lds_store);
-            __syncthreads();
-            pass_load<R>(v, lane, kAcLanes, N2, lds_load);
-            __syncthreads();
-            pass_compute<R, E, false>(v, lane, kAcLanes, N2, Seq::prefix(p), tw2);
-        });
-        constexpr int RL = Seq::at(Seq::count - 1);
-        pass_visit<RL>(v, lane, kAcLanes, N2, [&](int k, float2& val, bool ok) {
-            if (ok && cin) tl[k * kAcPitch + c] = val;
-        });
-    }
+    // ---- 1. forward pass B of the staged columns (columns outside the image are zero, never read as taps)
+    core_transforms<false, N2, Seq, L, 0, kAcCols>(
+        tl, tw2,
+        [&](int c, int m) -> float2 {
+            const int x = x0 - kAcLeft + c;
+            return (x >= 0 && x < a.nrv) ? a.in[(row0 + m) * a.stride + x] : make_float2(0.f, 0.f);
+        },
+        [&](int c, int k, float2 val) { tl[k * kAcPitch + c] = val; });
     __syncthreads();
 
     // ---- 2. RCMC + matched filter of the produced columns (the arithmetic of k_rcmc_tile)
@@ -150,33 +204,14 @@ __global__ __launch_bounds__(kAcThreads) void k_az_core(AzCoreArgs a) {
     __syncthreads();
 
     // ---- 3. inverse pass B' of the produced columns, inverse inter-pass twiddle, store
-    for (int bt = 0; bt < kAcOut / kAcBatch; ++bt) {
-        const int c = kAcLeft + bt * kAcBatch + b;
-        const int x = x0 + bt * kAcBatch + b;
-        const bool xw = x < a.nrv;
-        auto lds_store = [&](int i, float2 val) { tl[i * kAcPitch + c] = val; };
-        auto lds_load = [&](int i) { return tl[i * kAcPitch + c]; };
-        constexpr int RI0 = Seq::rat(0);
-        pass_load<RI0>(v, lane, kAcLanes, N2, lds_load);
-        __syncthreads();  // every element of the column is read before the first exchange overwrites it
-        pass_compute<RI0, E, true>(v, lane, kAcLanes, N2, 1, tw2);
-        static_for<1, Seq::count>([&](auto pc) __attribute__((always_inline)) {
-            constexpr int p = decltype(pc)::value;
-            constexpr int Rp = Seq::rat(p - 1), R = Seq::rat(p);
-            pass_store<Rp>(v, lane, kAcLanes, N2, Seq::rprefix(p - 1), lds_store);
-            __syncthreads();
-            pass_load<R>(v, lane, kAcLanes, N2, lds_load);
-            __syncthreads();
-            pass_compute<R, E, true>(v, lane, kAcLanes, N2, Seq::rprefix(p), tw2);
-        });
-        float2* dst = a.out + (xw ? x : 0);
-        constexpr int RIL = Seq::rat(Seq::count - 1);
-        pass_visit<RIL>(v, lane, kAcLanes, N2, [&](int n, float2& val, bool ok) {
-            if (!ok || !xw) return;
+    core_transforms<true, N2, Seq, L, kAcLeft, kAcOut>(
+        tl, tw2, [&](int, int) { return make_float2(0.f, 0.f); },
+        [&](int c, int n, float2 val) {
+            const int x = x0 + c - kAcLeft;
+            if (x >= a.nrv) return;
             const float2 wn = twf(k1 * n);  // k1 * n < n1 * n2
-            dst[(row0 + n) * a.stride] = cmulc(val, wn);
+            a.out[(row0 + n) * a.stride + x] = cmulc(val, wn);
         });
-    }
 }
 
 size_t az_core_lds_bytes(int n2, int nbins) {

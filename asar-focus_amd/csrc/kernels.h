@@ -291,6 +291,18 @@ struct SrgrArgs {
     float* out;
 };
 
+// Multilook x RSL fused with slant-to-ground resampling (k_imp.hip): workgroup t multilooks the slant columns
+// [t * 255, t * 255 + 256) of a tile of output rows into LDS and resamples the ground columns
+// [g_start[t], g_start[t + 1]), those whose first source column r0 lies in [t * 255, t * 255 + 255).
+struct MlSrgrArgs {
+    MultilookArgs ml;     // ml.out unused
+    const int* r0;        // per ground column: first source slant column (clamped as k_srgr clamps it)
+    const float* fx;      // per ground column: interpolation fraction
+    const int* g_start;   // ml_srgr_tiles(ml.nr) + 1 entries
+    int out_nr;           // ground columns
+    float* out;           // out_rows x out_nr
+};
+
 // Level-0 sample decode into a padded cf32 image (k_ingest.hip).
 struct IngestArgs {
     const uint8_t* bytes;     // line / record y at bytes + y * line_bytes
@@ -361,6 +373,12 @@ hipError_t launch_finalize(const FinalizeArgs& a, hipStream_t s);
 hipError_t launch_multilook(const MultilookArgs& a, hipStream_t s);
 hipError_t launch_rsl(float* img, int nr, int rows, const float* tab, hipStream_t s);
 hipError_t launch_srgr(const SrgrArgs& a, hipStream_t s);
+// The source column and fraction of every ground column, as k_srgr computes them (out_nr entries each).
+hipError_t launch_srgr_index(const SrgrArgs& a, int* r0, float* fx, hipStream_t s);
+int ml_srgr_tiles(int nr);                 // workgroup columns of the fused kernel
+int ml_srgr_owned_columns();               // slant columns per workgroup that own ground columns (255)
+hipError_t ml_srgr_prepare(float ratio);   // raises the kernel's LDS limit for the ratio's row tile
+hipError_t launch_ml_srgr(const MlSrgrArgs& a, hipStream_t s);
 hipError_t launch_zero_cols(float2* img, long stride, int rows, int c0, int c1, hipStream_t s);
 hipError_t launch_ingest(const IngestArgs& a, int kind, hipStream_t s);  // kind 0 = ERS IM, 1 = ASAR FBAQ-4
 hipError_t launch_condition(const OutputArgs& a, int detected, hipStream_t s);

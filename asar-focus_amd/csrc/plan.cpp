@@ -53,6 +53,11 @@ struct asarf_plan {
     DevResults* res = nullptr;
     float* ml = nullptr;
     float* rsl = nullptr;
+    // fused multilook + SRGR (k_ml_srgr): per ground column source index and fraction, ownership ranges
+    bool ml_srgr = false;
+    int* gr_r0 = nullptr;
+    float* gr_fx = nullptr;
+    int* gr_start = nullptr;
     size_t bytes = 0;
     // results window
     int crop_pos[2] = {0, 0}, crop_neg[2] = {0, 0};  // {front, back}
@@ -89,7 +94,7 @@ static int plan_alloc(asarf_plan* pl, void** p, size_t bytes) {
 
 static void plan_free(asarf_plan* pl) {
     void* bufs[] = {pl->img,    pl->img2, pl->hconj, pl->col, pl->partials, pl->totals,
-                    pl->corr,   pl->res,  pl->ml,    pl->rsl, pl->fbaq_lut};
+                    pl->corr,   pl->res,  pl->ml,    pl->rsl, pl->fbaq_lut, pl->gr_r0, pl->gr_fx, pl->gr_start};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : pl->ev) (void)hipEventDestroy(e);
@@ -259,7 +264,6 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
         if (rows_pos <= 4 || rows_neg <= 4) return fail(set_error(ASARF_ESHAPE, "asarf_plan_create: crop leaves no lines"));
         pl->imp_cap = std::max((int)std::round((float)rows_pos / pl->ml_ratio),
                                (int)std::round((float)rows_neg / pl->ml_ratio));
-        if ((st = plan_alloc(pl, (void**)&pl->ml, sizeof(float) * (size_t)pl->imp_cap * pl->nr_s))) return fail(st);
         if ((st = plan_alloc(pl, (void**)&pl->rsl, sizeof(float) * (size_t)pl->nr_s))) return fail(st);
         // range spreading loss table over the cropped range axis (range_spreading_loss.cu:46-52)
         std::vector<float> tab(pl->nr_s);
@@ -270,6 +274,41 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
         }
         if (hipMemcpy(pl->rsl, tab.data(), sizeof(float) * (size_t)pl->nr_s, hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_error(ASARF_EHIP, "asarf_plan_create: RSL table upload"));
+        // Multilook and SRGR fused into one kernel (the detected slant-range image stays in LDS) where the ground
+        // columns' source columns, computed on the device as k_srgr computes them, never decrease; otherwise (or
+        // with ASARF_IMP_FUSED=0) the two kernels with the slant-range image between them.
+        const char* imp_env = std::getenv("ASARF_IMP_FUSED");
+        const int gr = p->gr_range_size;
+        if (!(imp_env && imp_env[0] == '0') && gr > 0 && ml_srgr_prepare(pl->ml_ratio) == hipSuccess) {
+            const int tiles = ml_srgr_tiles(pl->nr_s);
+            if ((st = plan_alloc(pl, (void**)&pl->gr_r0, sizeof(int) * (size_t)gr))) return fail(st);
+            if ((st = plan_alloc(pl, (void**)&pl->gr_fx, sizeof(float) * (size_t)gr))) return fail(st);
+            if ((st = plan_alloc(pl, (void**)&pl->gr_start, sizeof(int) * (size_t)(tiles + 1)))) return fail(st);
+            SrgrArgs sa{};
+            sa.in_nr = pl->nr_s;
+            for (int i = 0; i < 5; ++i) sa.grsr[i] = p->grsr_poly[i];
+            sa.dr = p->range_spacing;
+            sa.gs = p->ground_spacing;
+            sa.out_nr = gr;
+            std::vector<int> r0(gr);
+            if (launch_srgr_index(sa, pl->gr_r0, pl->gr_fx, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess ||
+                hipMemcpy(r0.data(), pl->gr_r0, sizeof(int) * (size_t)gr, hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(set_error(ASARF_EHIP, "asarf_plan_create: slant-to-ground index"));
+            if (std::is_sorted(r0.begin(), r0.end())) {
+                std::vector<int> start(tiles + 1);
+                for (int t = 0; t < tiles; ++t)
+                    start[t] = (int)(std::lower_bound(r0.begin(), r0.end(), t * ml_srgr_owned_columns()) - r0.begin());
+                start[tiles] = gr;
+                if (hipMemcpy(pl->gr_start, start.data(), sizeof(int) * (size_t)(tiles + 1), hipMemcpyHostToDevice) !=
+                    hipSuccess)
+                    return fail(set_error(ASARF_EHIP, "asarf_plan_create: slant-to-ground ranges"));
+                pl->ml_srgr = true;
+            }
+        }
+        if (!pl->ml_srgr &&
+            (st = plan_alloc(pl, (void**)&pl->ml, sizeof(float) * (size_t)pl->imp_cap * pl->nr_s)))
+            return fail(st);
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(ASARF_EHIP, "asarf_plan_create: sync"));
     *out = pl;
@@ -610,6 +649,18 @@ static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float
         ma.ratio = pl->ml_ratio;
         ma.rsl = pl->rsl;
         ma.out = pl->ml;
+        if (pl->ml_srgr) {
+            MlSrgrArgs fa{};
+            fa.ml = ma;
+            fa.r0 = pl->gr_r0;
+            fa.fx = pl->gr_fx;
+            fa.g_start = pl->gr_start;
+            fa.out_nr = p.gr_range_size;
+            fa.out = imp_out;
+            HIP_CHECK(launch_ml_srgr(fa, s));
+            ST_CHECK(mark(pl, "multilook_srgr"));
+            return clear_error();
+        }
         HIP_CHECK(launch_multilook(ma, s));
         ST_CHECK(mark(pl, "multilook_rsl"));
         SrgrArgs sa{};

@@ -42,6 +42,9 @@ def kernel_bytes(name, cfg, plan, res):
     if name == "srgr":
         front, back, near, far = res["crop"]
         return 4 * res["imp_rows"] * (nrv - near - far) + 4 * res["imp_rows"] * plan.ng
+    if name == "multilook_srgr":
+        front, back, near, far = res["crop"]
+        return 8 * (na - front - back) * (nrv - near - far) + 4 * res["imp_rows"] * plan.ng
     return None
 
 

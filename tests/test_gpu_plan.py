@@ -85,13 +85,13 @@ def test_plan_imp_matches_oracle_chain(ctx, nr):
     assert err < 2e-4
 
 
-def run_plan_env(ctx, cfg, env):
+def run_plan_env(ctx, cfg, env, **kw):
     """run_plan with environment overrides that the plan reads at creation."""
     import os
     old = {k: os.environ.get(k) for k in env}
     os.environ.update(env)
     try:
-        return run_plan(ctx, cfg)
+        return run_plan(ctx, cfg, **kw)
     finally:
         for k, v in old.items():
             if v is None:
@@ -204,3 +204,21 @@ def test_plan_size_sweep(ctx, which, na, nr):
     assert err < SLC_TOL
     xs = np.arange(0, cfg.nr - cfg.nc)
     assert np.abs(np.polyval(res["dc_poly"], xs) - np.polyval(rres["dc_poly"], xs)).max() < 1e-6
+
+
+@pytest.mark.parametrize("which", ["asar_1400", "asar_1401", "ers"])
+def test_plan_imp_fused_multilook_srgr_is_bit_identical(ctx, which):
+    """The fused multilook + SRGR kernel (the detected slant-range image kept in LDS) against the two kernels with
+    that image in HBM between them (ASARF_IMP_FUSED=0): the same IMP product bit for bit, over the device-side
+    crop and row count."""
+    if which == "ers":
+        cfg = configs.ers_block().scaled(na=1400, nr=1500, n_targets_az=2, n_targets_rg=2)
+    else:
+        cfg = configs.asar_scene().scaled(na=1200, nr=int(which[-4:]), n_targets_random=8)
+    _, _, res_f, imp_f = run_plan(ctx, cfg, imp=True)
+    _, _, res_s, imp_s = run_plan_env(ctx, cfg, {"ASARF_IMP_FUSED": "0"}, imp=True)
+    assert res_f["imp_rows"] == res_s["imp_rows"] and tuple(res_f["crop"]) == tuple(res_s["crop"])
+    rows = res_f["imp_rows"]
+    assert rows > 0
+    np.testing.assert_array_equal(imp_f[:rows], imp_s[:rows])
+
