@@ -109,6 +109,7 @@ extern "C" int asarf_batch_add(asarf_batch* b, const asarf_focus_params* p, cons
     asarf_batch::Block bl;
     bl.stream = (int)(b->blocks.size() % b->streams.size());
     ST_CHECK(asarf_plan_create(b->sctx[bl.stream], p, chirp_iq, &bl.plan));
+    plan_single_stream(bl.plan);
     bl.raw = raw;
     bl.imp = imp_out;
     b->blocks.push_back(bl);

@@ -107,6 +107,8 @@ int col_forward_pass(asarf_ctx* c, const ColSplit& cs, int pass, const float2* i
                      float2* img, long stride, int ncols, int mfma = 0, const int* fused_gate = nullptr);
 int col_inverse_pass_to(asarf_ctx* c, const ColSplit& cs, int pass, const float2* in, long in_stride, float2* out,
                         long out_stride, int ncols, int out_rows, float scale);
+// Runs the plan's azimuth tail as one chunk on its context's stream (batches: their blocks overlap already).
+void plan_single_stream(asarf_plan* plan);
 int col_inverse_pass(asarf_ctx* c, const ColSplit& cs, int pass, float2* img, long stride, int ncols, int out_rows,
                      float scale, int mfma = 0);
 }  // namespace asarf

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kAcThreads) void k_az_core(AzCoreArgs a) {
     const int w = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     if (w >= a.items) return;
     const int k1 = w / a.ntiles;
-    const int x0 = (w % a.ntiles) * kAcOut;
+    const int x0 = a.col0 + (w % a.ntiles) * kAcOut;
     const int tid = threadIdx.x;
     float2* tl = lds;  // [N2][kAcPitch]
     const TwLds tw2 = tw_fill(lds + N2 * kAcPitch, a.tw2, N2, tid, kAcThreads);

@@ -202,7 +202,8 @@ struct AzCoreArgs {
     int nrv;
     int nbins;             // n1 * n2
     int n1;
-    int ntiles;            // ceil(nrv / az_core_tile_columns())
+    int col0;              // first produced column (a multiple of az_core_tile_columns())
+    int ntiles;            // produced tiles, from col0 (the last may end at nrv)
     int items;             // n1 * ntiles
     const ColConst* col;
     double prf, beta;

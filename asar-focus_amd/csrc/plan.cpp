@@ -45,6 +45,13 @@ struct asarf_plan {
     float2* img = nullptr;
     float2* img2 = nullptr;  // second image: out-of-place target of the column passes and of the RCMC stage
     bool core = false;       // fused azimuth core: pass B + RCMC + matched filter + pass B' in one kernel
+    // Column chunks of the azimuth tail (core, then in-place pass A', per chunk) alternating between the context's
+    // stream and a second stream, each chunk's core after the previous chunk's: one chunk's pass A' (HBM-bound)
+    // overlaps the next chunk's core (VALU-bound).
+    int az_chunks = 1;
+    hipStream_t s1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::vector<hipEvent_t> ev_core;
     float2* hconj = nullptr;
     ColConst* col = nullptr;
     long long* partials = nullptr;
@@ -98,6 +105,10 @@ static void plan_free(asarf_plan* pl) {
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : pl->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : pl->ev_core) (void)hipEventDestroy(e);
+    if (pl->ev_fork) (void)hipEventDestroy(pl->ev_fork);
+    if (pl->ev_join) (void)hipEventDestroy(pl->ev_join);
+    if (pl->s1) (void)hipStreamDestroy(pl->s1);
 }
 
 extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, const float* chirp_iq, asarf_plan** out) {
@@ -309,6 +320,24 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
         if (!pl->ml_srgr &&
             (st = plan_alloc(pl, (void**)&pl->ml, sizeof(float) * (size_t)pl->imp_cap * pl->nr_s)))
             return fail(st);
+    }
+    if (pl->core) {
+        // chunks of at least 16 core tiles (64 columns each), at most 4; ASARF_AZ_CHUNKS=<n> overrides
+        const int tiles = (nrv + az_core_tile_columns() - 1) / az_core_tile_columns();
+        int chunks = std::max(1, std::min(4, tiles / 16));
+        const char* ch_env = std::getenv("ASARF_AZ_CHUNKS");
+        if (ch_env && std::atoi(ch_env) > 0) chunks = std::min(std::atoi(ch_env), tiles);
+        pl->az_chunks = chunks;
+        if (chunks > 1) {
+            if (hipStreamCreateWithFlags(&pl->s1, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming) != hipSuccess)
+                return fail(set_error(ASARF_EHIP, "asarf_plan_create: azimuth chunk stream"));
+            pl->ev_core.assign(chunks, nullptr);
+            for (hipEvent_t& e : pl->ev_core)
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                    return fail(set_error(ASARF_EHIP, "asarf_plan_create: azimuth chunk events"));
+        }
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(ASARF_EHIP, "asarf_plan_create: sync"));
     *out = pl;
@@ -526,6 +555,12 @@ static bool core_path_of(const asarf_plan* pl) {
     return pl->core && effective_interp(pl) == ASARF_RCMC_SINC16 && !pl->fft_mfma && !pl->rcmc_row_only;
 }
 
+namespace asarf {
+void plan_single_stream(asarf_plan* pl) {
+    if (pl) pl->az_chunks = 1;
+}
+}  // namespace asarf
+
 static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float* imp_out) {
     asarf_ctx* c = pl->ctx;
     const asarf_focus_params& p = pl->p;
@@ -564,25 +599,8 @@ static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float
         // and pass B' as separate kernels. Both variants are enqueued and the device-side bound makes exactly
         // one of them do the work.
         const int* gate = &pl->res->rcmc_reach;
-        AzCoreArgs ac{};
-        ac.in = pl->img2;
-        ac.out = pl->img;
-        ac.stride = pl->stride;
-        ac.nrv = pl->nrv;
-        ac.nbins = pl->nap;
-        ac.n1 = cs.n1;
-        ac.ntiles = (pl->nrv + az_core_tile_columns() - 1) / az_core_tile_columns();
-        ac.items = cs.n1 * ac.ntiles;
-        ac.col = pl->col;
-        ac.prf = p.prf;
-        ac.beta = p.az_bandwidth_fraction;
-        ac.tw2 = cs.p2->d_tw;
-        ac.tw_full = cs.full->d_tw;
-        ac.gate = gate;
-        rcmc_window(ac.window);
-        HIP_CHECK(launch_az_core(ac, cs.n2, s));
-        ST_CHECK(mark(pl, "azimuth_core_fused"));
-        // the separate kernels for shift bounds beyond the core's halo (img -> img2, then B' img2 -> img)
+        // the separate kernels for shift bounds beyond the core's halo first (pass B ran above; RCMC img -> img2,
+        // then B' img2 -> img), so that the core chunks below read an untouched pass-A output in img2
         rc.out = pl->img2;
         rc.fused_gate = gate;
         rc.tile_gate = gate;
@@ -593,12 +611,55 @@ static int plan_execute(asarf_plan* pl, const void* raw, asarf_image* slc, float
         ib.fused_gate = gate;
         HIP_CHECK(launch_col_fft(ib, true, s));
         ST_CHECK(mark(pl, "azimuth_core_wide"));
-        // 5. inverse pass A' (img -> img2), 1/Nap, rows < Na: the SLC ends in img2
-        ColFftArgs ia = col_args(pl->ctx, pl->img, pl->stride, pl->img2, pl->stride, pl->nrv, cs.n2, 1, cs.n2, n, 1,
-                                 cs.n2, p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
-        HIP_CHECK(launch_col_fft(ia, true, s));
+        AzCoreArgs ac{};
+        ac.in = pl->img2;
+        ac.out = pl->img;
+        ac.stride = pl->stride;
+        ac.nrv = pl->nrv;
+        ac.nbins = pl->nap;
+        ac.n1 = cs.n1;
+        ac.col = pl->col;
+        ac.prf = p.prf;
+        ac.beta = p.az_bandwidth_fraction;
+        ac.tw2 = cs.p2->d_tw;
+        ac.tw_full = cs.full->d_tw;
+        ac.gate = gate;
+        rcmc_window(ac.window);
+        // 4. fused azimuth core (img2 -> img) and 5. inverse pass A' in place (img), 1/Nap, rows < Na, per column
+        // chunk. Chunk k's core waits for chunk k - 1's core (on the other stream), so it runs beside chunk
+        // k - 1's pass A'; the core reads pass-A columns up to 16 past its chunk from img2, which no pass A'
+        // writes. With per-kernel timing the chunks run unsplit on one stream.
+        const int tw = az_core_tile_columns();
+        const int tiles = (pl->nrv + tw - 1) / tw;
+        // one chunk under stream capture: the second stream's fork and join are not taken into a caller's graph
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        HIP_CHECK(hipStreamIsCapturing(s, &cap));
+        const int chunks = (pl->timing || cap != hipStreamCaptureStatusNone) ? 1 : pl->az_chunks;
+        if (chunks > 1) {
+            HIP_CHECK(hipEventRecord(pl->ev_fork, s));
+            HIP_CHECK(hipStreamWaitEvent(pl->s1, pl->ev_fork, 0));
+        }
+        for (int k = 0; k < chunks; ++k) {
+            hipStream_t sk = (k % 2 == 0) ? s : pl->s1;
+            const int t0 = (int)((long)tiles * k / chunks), t1 = (int)((long)tiles * (k + 1) / chunks);
+            const int c0 = t0 * tw, c1 = std::min(t1 * tw, pl->nrv);
+            if (k > 0) HIP_CHECK(hipStreamWaitEvent(sk, pl->ev_core[k - 1], 0));
+            ac.col0 = c0;
+            ac.ntiles = t1 - t0;
+            ac.items = cs.n1 * ac.ntiles;
+            HIP_CHECK(launch_az_core(ac, cs.n2, sk));
+            if (chunks > 1) HIP_CHECK(hipEventRecord(pl->ev_core[k], sk));
+            if (chunks == 1) ST_CHECK(mark(pl, "azimuth_core_fused"));
+            ColFftArgs ia = col_args(pl->ctx, pl->img + c0, pl->stride, pl->img + c0, pl->stride, c1 - c0, cs.n2, 1,
+                                     cs.n2, n, 1, cs.n2, p.na, nullptr, 1.0f / (float)pl->nap, cs.p1);
+            HIP_CHECK(launch_col_fft(ia, true, sk));
+        }
+        if (chunks > 1) {
+            HIP_CHECK(hipEventRecord(pl->ev_join, pl->s1));
+            HIP_CHECK(hipStreamWaitEvent(s, pl->ev_join, 0));
+        }
         ST_CHECK(mark(pl, "azimuth_ifft_a"));
-        slc_buf = pl->img2;
+        slc_buf = pl->img;
     } else {
         // 4. RCMC (any interpolator) + azimuth matched filter (img -> img2): the column-tiled kernel when the
         // device-side shift bound fits its halo, else the row kernel; both are enqueued and the bound makes

@@ -222,3 +222,16 @@ def test_plan_imp_fused_multilook_srgr_is_bit_identical(ctx, which):
     assert rows > 0
     np.testing.assert_array_equal(imp_f[:rows], imp_s[:rows])
 
+
+
+@pytest.mark.parametrize("chunks", ["2", "3", "4"])
+def test_plan_azimuth_chunks_are_bit_identical(ctx, chunks):
+    """The azimuth tail in column chunks over two streams (each chunk's core waits for the previous chunk's, and
+    runs beside its in-place pass A') against one unsplit chunk: the same SLC and IMP bit for bit."""
+    cfg = configs.asar_scene().scaled(na=1200, nr=3200, n_targets_random=8)
+    _, slc_1, res_1, imp_1 = run_plan_env(ctx, cfg, {"ASARF_AZ_CHUNKS": "1"}, imp=True)
+    _, slc_k, res_k, imp_k = run_plan_env(ctx, cfg, {"ASARF_AZ_CHUNKS": chunks}, imp=True)
+    assert res_1["azimuth_core_fused"] and res_k["azimuth_core_fused"]
+    np.testing.assert_array_equal(slc_k, slc_1)
+    rows = res_1["imp_rows"]
+    np.testing.assert_array_equal(imp_k[:rows], imp_1[:rows])
