@@ -1,8 +1,8 @@
 // Column transforms with compile-time radix sequences (the kernel template and its launcher). The sub-transform
 // lengths of the four-step azimuth FFT that have such a kernel are every 2^a 3^b 5^c 7^d from 16 to 400, listed
 // below with their radix sequences and split over six compilation units (k_col_static*.hip) so they build in
-// parallel. Same workgroup layout and global access as k_col_fft (16 lanes per column, 16 or 32 columns, LDS rows
-// of COLS + 1 elements), but two large radices (composite DFTs in registers) replace three or four
+// parallel. Same workgroup layout and global access as k_col_fft (16 lanes per column, 16, 32 or 48 columns, LDS
+// rows of COLS + 1 elements), but two large radices (composite DFTs in registers) replace three or four
 // runtime-dispatched passes: one LDS exchange per transform instead of two or three, and compile-time index
 // arithmetic.
 //
@@ -61,7 +61,7 @@ __device__ __forceinline__ void col_static_item(const ColFftArgs& a, float2* lds
     const int ntiles = (a.ncols + COLS - 1) / COLS;
     const int tile = w % ntiles;
     const int g = w / ntiles;
-    const int b = threadIdx.x & (COLS - 1);
+    const int b = threadIdx.x % COLS;
     const int lane = threadIdx.x / COLS;
     const int col = tile * COLS + b;
     const bool col_ok = col < a.ncols;
@@ -167,11 +167,18 @@ inline hipError_t launch_cs_w(const ColFftArgs& a, bool inverse, hipStream_t s) 
 
 // Columns per workgroup: ColFftArgs::col_width, or by access pattern when 0 (32 where the transform's elements are
 // strided rows, as in passes A and A' of the four-step split, so each row access is 256 contiguous bytes; 16 for
-// contiguous sub-transforms, whose 16-column tiles measured faster). The strided passes stay 32 wide even where
-// the tile leaves room for one workgroup per CU (N = 336, 360: 0.57 ms against 1.2 ms for 16-column tiles on the
-// 30240-line azimuth transform).
+// contiguous sub-transforms, whose 16-column tiles measured faster). The strided passes stay at least 32 wide even
+// where the tile leaves room for one workgroup per CU (N = 336, 360: 0.57 ms against 1.2 ms for 16-column tiles
+// on the 30240-line azimuth transform). Lengths whose 32-column tile admits only one workgroup per CU (N > 310,
+// tile > 80 KB) take 48 columns where the 48-column tile fits: still one workgroup, but 12 waves instead of 8 (the
+// 360-point passes of the 30240-line transform: A 0.598 -> 0.574 ms, A' 0.545 -> 0.524 ms; 40 columns: slower).
 template <int N, class Seq>
 inline hipError_t launch_cs(const ColFftArgs& a, bool inverse, hipStream_t s) {
+    if constexpr ((size_t)N * 33 * sizeof(float2) > 80 * 1024) {
+        const size_t lds48 = sizeof(float2) * (size_t)(N * 49 + tw_lds_elems(N) +
+                                                       (a.tw_full ? tw_lds_elems(a.groups * N) : 0));
+        if (a.col_width == 0 && a.in_m > 1 && lds48 <= max_lds_bytes()) return launch_cs_w<N, Seq, 48>(a, inverse, s);
+    }
     const bool wide_fits = sizeof(float2) * (size_t)N * 33 <= max_lds_bytes();
     const int w = (a.col_width == 16 || a.col_width == 32) ? a.col_width : (a.in_m > 1 && wide_fits ? 32 : 16);
     return w == 32 ? launch_cs_w<N, Seq, 32>(a, inverse, s) : launch_cs_w<N, Seq, 16>(a, inverse, s);

@@ -1,7 +1,8 @@
 """The compile-time kernels that serve arbitrary scene sizes: column transforms of every 2^a 3^b 5^c 7^d length from
-16 to 400 (both tile widths) against numpy's FFT in FP64, the fused azimuth core at every compiled second-pass
-length against the separate kernels and the FP64 oracle, and range transforms padded past the reference's length
-(asarf_fft_fast_range_size) against the oracle at the reference's padding."""
+16 to 400 (16- and 32-column tiles, and the 48-column tiles of the long strided first passes) against numpy's FFT in
+FP64, the fused azimuth core at every compiled second-pass length against the separate kernels and the FP64
+oracle, and range transforms padded past the reference's length (asarf_fft_fast_range_size) against the oracle at
+the reference's padding."""
 import os
 
 import numpy as np
@@ -107,3 +108,31 @@ def test_fast_range_length_matches_reference_padding(ctx, nr, nc):
     err = rel_rms(slc, ref)
     print(f"nr {nr} nc {cfg.nc}: padding {pad} -> {fast}, slc rel rms {err:.3e}", flush=True)
     assert err < 1e-4
+
+
+def test_strided_first_pass_of_every_long_length(tmp_path):
+    """Two-pass column transforms n1 x 16 whose strided first pass has a long compile-time length (n1 > 310: the
+    48-column tiles), the split forced through the tuner cache file, against numpy's FFT in FP64."""
+    longs = [n for n in STATIC_LENGTHS if n > 310]
+    ctx = focus.Context(0)
+    path = str(tmp_path / "tune.txt")
+    focus.tune_save(ctx, path)  # the header with this device's architecture
+    ctx.close()
+    with open(path, "a") as f:
+        for n1 in longs:
+            f.write(f"col {n1 * 16} {n1}\n")
+    ctx = focus.Context(0)
+    try:
+        assert focus.tune_load(ctx, path) == len(longs)
+        for n1 in longs:
+            n = n1 * 16
+            x = (np.random.default_rng(n1).standard_normal((n, 37, 2)) @ [1, 1j]).astype(np.complex64)
+            t = torch.from_numpy(x).cuda()
+            focus.fft(ctx, t, L.FFT_COLUMNS)
+            err = rel_rms(t.cpu().numpy(), np.fft.fft(x.astype(np.complex128), axis=0))
+            assert err < 2e-6, (n1, err)
+            focus.fft(ctx, t, L.FFT_COLUMNS, inverse=True, scale=1.0 / n)
+            err = rel_rms(t.cpu().numpy(), x)
+            assert err < 4e-6, (n1, err)
+    finally:
+        ctx.close()
