@@ -8,8 +8,9 @@
 // returns to HBM: the spectrum makes one round trip to HBM instead of three (B out, RCMC in and out, B' in).
 //   staged columns:    [x0 - 8, x0 + 80)   (forward transforms of the 8 + 64 + 16 staged columns)
 //   produced columns:  [x0, x0 + 64)       (RCMC reads x + p - 8 .. x + p + 7, p = round(shift) <= kAzCoreReach)
-// The per-pixel arithmetic is k_rcmc_tile's (k_rda.hip) and the transforms are the Stockham passes of the column
-// kernels, so the core reproduces the separate kernels up to the rounding of a different N1 x N2 split.
+// The per-pixel arithmetic is k_rcmc_tile's (k_rda.hip; without the two zero-weight end taps of the window) and the
+// transforms are the Stockham passes of the column kernels, so the core reproduces the separate kernels up to the
+// rounding of a different N1 x N2 split.
 //
 // With N2 = 84 the tile (84 x 89 complex samples) and the twiddle tables take about 60 KB of LDS.
 // Phases (512 lanes; L = 4, 8 or 16 lanes per column transform, see core_lanes):
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kAcThreads) void k_az_core(AzCoreArgs a) {
                     const float d = (float)(shift - p);
                     // the gate guarantees pi <= kAzCoreReach; the clamp only keeps the taps inside the tile
                     const float2* tap = tl + m * kAcPitch + min(kAcLeft + cw + pi - 8, kAcCols - 16);
-                    val = (d == 0.0f) ? tap[8] : sinc16_interp(tap, ws, d);
+                    val = (d == 0.0f) ? tap[8] : sinc16_interior(tap, ws, d);
                 }
                 double fc = fb - cc.dc_off;
                 if (fc > half)
@@ -281,6 +282,8 @@ hipError_t az_core_prepare(int n2, int nbins) {
 
 hipError_t launch_az_core(const AzCoreArgs& a, int n2, hipStream_t s) {
     if (!a.in || !a.out || a.in == a.out || a.items <= 0) return hipErrorInvalidValue;
+    // the interpolation skips taps 0 and 15 (sinc16_interior): their window weights must be zero
+    if (!(std::fabs(a.window[0]) < 1e-12f && std::fabs(a.window[15]) < 1e-12f)) return hipErrorInvalidValue;
     return core_dispatch(a, n2, false, s);
 }
 
