@@ -47,7 +47,7 @@ struct asarf_plan {
     bool core = false;       // fused azimuth core: pass B + RCMC + matched filter + pass B' in one kernel
     // Column chunks of the azimuth tail (core, then in-place pass A', per chunk) alternating between the context's
     // stream and a second stream, each chunk's core after the previous chunk's: one chunk's pass A' (HBM-bound)
-    // overlaps the next chunk's core (VALU-bound).
+    // can overlap the next chunk's core (VALU-bound). Opt-in (ASARF_AZ_CHUNKS).
     int az_chunks = 1;
     hipStream_t s1 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -322,9 +322,12 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
             return fail(st);
     }
     if (pl->core) {
-        // chunks of at least 16 core tiles (64 columns each), at most 4; ASARF_AZ_CHUNKS=<n> overrides
+        // One chunk by default: with the 48-column tiles of the long strided passes (141 KB of LDS) a pass-A'
+        // workgroup cannot share a CU with a core workgroup (60 KB), and the two-stream chunks measured slower on
+        // the ASAR scene (3.20 ms unsplit against 3.26-3.28 ms with 2-5 chunks; with 32-column tiles they had
+        // gained 0.04 ms). ASARF_AZ_CHUNKS=<n> splits the tail into n chunks.
         const int tiles = (nrv + az_core_tile_columns() - 1) / az_core_tile_columns();
-        int chunks = std::max(1, std::min(4, tiles / 16));
+        int chunks = 1;
         const char* ch_env = std::getenv("ASARF_AZ_CHUNKS");
         if (ch_env && std::atoi(ch_env) > 0) chunks = std::min(std::atoi(ch_env), tiles);
         pl->az_chunks = chunks;
