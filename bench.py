@@ -48,6 +48,21 @@ def kernel_bytes(name, cfg, plan, res):
     return None
 
 
+def kernel_sources_sha1():
+    """SHA-1 over the library's kernel and host sources (asar-focus_amd/csrc, sorted by name): the key under which
+    the committed counter profiles were measured."""
+    import hashlib
+
+    h = hashlib.sha1()
+    d = os.path.join(ROOT, "asar-focus_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".h", ".cpp")):
+            h.update(name.encode())
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()
+
+
 def cpu_baseline(cfg, raw, min_seconds=10.0, max_reps=3):
     """FP64 oracle (the timed CPU baseline) on the same scene: the whole SLC chain (I/Q, Doppler, range
     compression, range-Doppler), repeated until about min_seconds of CPU work have been timed."""
@@ -638,18 +653,28 @@ def main():
     dom = max((k for k in stages if kernel_bytes(k, cfg, plan, res)), key=lambda k: stages[k])
     dom_bytes = kernel_bytes(dom, cfg, plan, res)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
-    traffic = None
+    # HBM traffic and VALU issue occupancy come from the committed counter passes (profiles/traffic.json,
+    # profiles/issue.json); each records the hash of the kernel sources it was measured on, and a figure whose
+    # sources have changed since is reported as null (stale) rather than replayed
+    src_sha = kernel_sources_sha1()
+    traffic, traffic_current = None, None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(cfg.name, {}).get(dom)
-    # VALU issue occupancy per stage from the committed SQ counter pass (SQ_ACTIVE_INST_VALU against the SIMD
-    # cycles of the dispatch): near 1 means the kernel is bound by its instruction count rather than by HBM
-    valu_busy = {}
+            tj = json.load(f)
+        traffic_current = tj.get("_sources_sha1") == src_sha
+        if traffic_current:
+            traffic = tj.get(cfg.name, {}).get(dom)
+    # VALU issue occupancy per stage (SQ_ACTIVE_INST_VALU against the SIMD cycles of the dispatch): near 1 means
+    # the kernel is bound by its instruction count rather than by HBM
+    valu_busy, issue_current = {}, None
     ifile = os.path.join(ROOT, "profiles", "issue.json")
     if os.path.exists(ifile):
         with open(ifile) as f:
-            valu_busy = {k: v["valu_busy"] for k, v in json.load(f).get(cfg.name, {}).items()}
+            ij = json.load(f)
+        issue_current = ij.get("_sources_sha1") == src_sha
+        if issue_current:
+            valu_busy = {k: v["valu_busy"] for k, v in ij.get(cfg.name, {}).items()}
 
     # measured device-to-device copy rate over a buffer the size of the plan's image: the second denominator of
     # the memory-bound stages (what a plain read + write of the same bytes achieves on this device)
@@ -708,6 +733,7 @@ def main():
             "frac_of_copy": round(achieved / copy_gbs, 4),
             "valu_busy": valu_busy.get(dom),
             "valu_busy_stages": valu_busy or None,
+            "profiles_current": {"traffic": traffic_current, "issue": issue_current},
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "stages_gbs": {k: round(v, 1) for k, v in gbs.items()},
