@@ -195,18 +195,27 @@ extern "C" int asarf_plan_create(asarf_ctx* c, const asarf_focus_params* p, cons
     // the balanced split). Whether the core or the separate kernels run is decided on the device by the RCMC
     // shift bound (the core's range halo covers shifts up to kAzCoreReach pixels).
     const char* fz_env = std::getenv("ASARF_AZ_FUSED");
-    // The compiled second-pass lengths are tried in their order of preference, first those whose first-pass
-    // length n1 = nap / n2 has a compile-time column kernel, then any. ASARF_AZ_CORE_N2=<n2> tries only that one.
+    // The compiled second-pass lengths are tried in their order of preference: first those whose first-pass
+    // length n1 = nap / n2 has a compile-time column kernel and where both the core's tile and the 32-column tile of
+    // the strided passes A and A' leave room for two workgroups per CU, then those with a compile-time first pass,
+    // then any. (On the 30240-line ASAR scene: 105 x 288 runs the plan in 3.13 ms against 3.21 ms for 84 x 360,
+    // whose 360-point passes hold one workgroup per CU.) ASARF_AZ_CORE_N2=<n2> tries only that one.
     if (!(fz_env && fz_env[0] == '0') && pl->cs.n2 > 1) {
         int lens[32];
         const int nl = std::min(az_core_lengths(lens, 32), 32);
         const char* n2_env = std::getenv("ASARF_AZ_CORE_N2");
         const int n2_only = n2_env ? std::atoi(n2_env) : 0;
-        for (int round = 0; round < 2 && !pl->core; ++round) {
+        const size_t half_lds = c->max_lds / 2;
+        auto two_per_cu = [&](int n1, int n2) {
+            const size_t pass_lds = sizeof(float2) * (size_t)(n1 * 33 + tw_lds_elems(n1) + tw_lds_elems(nap));
+            return pass_lds <= half_lds && az_core_lds_bytes(n2, nap) <= half_lds;
+        };
+        for (int round = 0; round < 3 && !pl->core; ++round) {
             for (int i = 0; i < nl; ++i) {
                 const int n2 = lens[i];
                 if (nap % n2 != 0 || (n2_only > 0 && n2 != n2_only)) continue;
-                if (round == 0 && !(c->col_static && col_static_length(nap / n2))) continue;
+                if (round < 2 && !(c->col_static && col_static_length(nap / n2))) continue;
+                if (round == 0 && !two_per_cu(nap / n2, n2)) continue;
                 ColSplit alt;
                 if (get_col_split(c, nap, &alt, n2) != ASARF_OK || alt.n2 != n2) continue;
                 if (!az_core_supported(n2) || az_core_lds_bytes(n2, nap) > c->max_lds) continue;
