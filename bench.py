@@ -453,7 +453,7 @@ def run_batch(args):
         if args.tune_file:
             focus.tune_save(ctx, args.tune_file)
     tune_s = time.perf_counter() - t1
-    b = focus.Batch(ctx, n_streams=4)
+    b = focus.Batch(ctx, n_streams=args.streams)
     for c, r in zip(cfgs, raws):
         b.add(c, r)
     samples = sum(c.na * c.nr for c in cfgs)
@@ -490,7 +490,7 @@ def run_batch(args):
         "data": "synthetic (seeded Philox blocks: point targets + 10 dB noise)",
         "config": {"workload": "batch64_ers_asar_blocks", "blocks": len(cfgs), "raw_samples": samples,
                    "azimuth_lines": [c.na for c in cfgs], "range_samples": [c.nr for c in cfgs],
-                   "streams": 4, "form": "one captured graph (fork-join over 4 streams)",
+                   "streams": args.streams, "form": f"one captured graph (fork-join over {args.streams} streams)",
                    "range_transform": ("compile-time lengths >= the reference padding (asarf_fft_fast_range_size)"
                                        if args.range_pad == "fast" else "reference padding")},
         "sequential_msamples_per_s": round(samples / seq_s / 1e6, 2),
@@ -535,6 +535,7 @@ def main():
     ap.add_argument("--tune-file", default=None,
                     help="batch64: autotuner cache file; loaded when it exists (no tuning launches), else written "
                          "after tuning")
+    ap.add_argument("--streams", type=int, default=4, help="batch64: streams the blocks are spread over")
     ap.add_argument("--range-pad", choices=["fast", "reference"], default="fast",
                     help="batch64: range transform lengths (fast: the compile-time length at most 1.5 times the "
                          "reference's padding, same lines up to FP32 rounding)")
