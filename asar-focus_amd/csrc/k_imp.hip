@@ -9,11 +9,12 @@ namespace asarf {
 // Each lane owns one range column of a tile of `rows` output rows. The |s|^2 of every input row the tile needs
 // (at most kMlSpan) is loaded first, all loads independent and in flight together, into the lane's own LDS
 // column (no barrier: a lane only reads what it wrote); the boxcars and the interpolation then run from LDS.
-// Neighbouring tiles share the rows of one output's boxcars (about 6 rows): with a 48-row span a tile produces 15
-// outputs of the ASAR scene (ratio 2.9) from 47 rows, 8 % more reads than the outputs consume (a 32-row span
-// made it 13 %). (A ring that walks each column once, 24 rows a batch, measured slower: each batch waits for
-// its own loads.)
-constexpr int kMlSpan = 48;     // input rows per tile
+// Neighbouring tiles share the rows of one output's boxcars (about 6 rows): with a 40-row span a tile produces 12
+// outputs of the ASAR scene (ratio 2.9) from 38 rows, about 10 % more reads than the outputs consume. The span
+// also sets the LDS of the fused multilook + SRGR kernel ((span + rows) x 1 KB): 40 rows leave room for three of
+// its workgroups per CU, 48 rows for two (0.376 -> 0.332 ms on the ASAR scene; 32 rows: 0.334 ms). (A ring that
+// walks each column once, 24 rows a batch, measured slower: each batch waits for its own loads.)
+constexpr int kMlSpan = 40;     // input rows per tile
 constexpr int kMlLanes = 256;
 
 // Output rows per tile: the span (rows - 1) * ratio + 6 must fit kMlSpan (with one row of margin for rounding).
